@@ -1,0 +1,79 @@
+"""In-tree build of the native libraries (no JIT cache: the .so files travel with the tree).
+
+    libmxmesh.so   host set-up, g++ -O3 -ffp-contract=off
+    libmxeuler.so  HIP kernels and the C ABI, hipcc for gfx950
+    oracle/_build/liboracle.so   the CPU oracle (test infrastructure), g++ -O3 -ffp-contract=off
+
+Every build passes -ffp-contract=off, as the reference builds host and device code
+(CMakeLists.txt:45-55), so that no product is fused into an FMA.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "autan_amd"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MXEULER_ARCH", "gfx950")
+
+CXXFLAGS = ["-std=c++20", "-O3", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra"]
+HIPFLAGS = [
+    f"--offload-arch={ARCH}",
+    "-std=c++20",
+    "-O3",
+    "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-rdc",
+    "-fPIC",
+    "-shared",
+    # Packed FP32 instructions stay off: the compiler would otherwise pair float operations.
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, cwd=ROOT)
+
+
+def build_mesh() -> Path:
+    out = PKG / "libmxmesh.so"
+    src = sorted((PKG / "csrc" / "mesh").glob("*.cpp"))
+    _run(["g++", *CXXFLAGS, "-o", out, *src])
+    return out
+
+
+def build_device() -> Path:
+    out = PKG / "libmxeuler.so"
+    _run([HIPCC, *HIPFLAGS, "-o", out, PKG / "csrc" / "device" / "stepper.hip"])
+    return out
+
+
+def build_oracle() -> Path:
+    out = ROOT / "oracle" / "_build" / "liboracle.so"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    flags = [f for f in CXXFLAGS if f != "-Wextra"]
+    _run(["g++", *flags, "-fopenmp", "-o", out, ROOT / "oracle" / "euler_oracle.cpp"])
+    return out
+
+
+def build_all() -> None:
+    if shutil.which("g++") is None:
+        raise RuntimeError("g++ not found")
+    build_mesh()
+    build_oracle()
+    build_device()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "all": build_all}[w]()

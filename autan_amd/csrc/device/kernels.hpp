@@ -1,0 +1,640 @@
+// HIP kernels of the explicit Euler residual loop for gfx950 (wave64).
+//
+// Device layout (the boundary of include/mxeuler.h converts to it once, at creation):
+//   cell records   prim  8 x C per cell (rho, u, v, w, p - p_ref, 3 pad): one 32 B record per
+//                        gather in fp32, so a stencil or face neighbour costs one cache line;
+//                  rec   20 x C per cell: the gradient (15, row 3 i + j) then the limiter (5);
+//   face records   fgeo  8 x C per face (n_x, n_y, n_z, |A|, r_left, 0) and frr 4 x C per
+//                        interior face (r_right, 0), r = rel(x_f, x_c) already rounded to C;
+//   stencil        sliced by wave: entry j of cell 64 s + l at soff[s] + 64 j + l, so the 64
+//                  lanes of a wave read 64 consecutive indices and weights per entry;
+//   incidences     CSR per cell, face index with bit 31 set where the cell is the neighbour;
+//   conserved u, stage and res stay FP64 structure of arrays, k * n + c, as at the boundary.
+// Every loop keeps the reference's summation order, so FP64 results can match the host bit for
+// bit: the store-gather order of its device path (codegen/src/autan_codegen/emit_gpu.py:189-238).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "bodies.hpp"
+
+namespace mxe {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kLimbs = 68;        // exact-sum limbs of 32 bits, limb i weighs 2^(32 i - 1088)
+constexpr int kWords = kLimbs + 3;  // then the counts of +inf, -inf and NaN terms
+constexpr int kSums = 5;
+
+// Results a step leaves for the host: one copy back per step.
+struct Small {
+  long long words[kSums][kWords];
+  unsigned long long hkey;  // ordered key of the least time step
+  int invalid_state;        // least cell with an invalid state, INT32_MAX for none
+  int invalid_residual;     // least cell with a non-finite residual
+};
+
+template <class C>
+struct Params {
+  C g, rgas, pref, minf;
+  C ur[5];
+  C scale[5];
+  double ref[5];
+  double p_ref;  // FP64 p_ref for the validity check, rounded there as the reference rounds it
+  double gamma;
+};
+
+template <class C>
+struct Dev {
+  int nc, nf, ni, nb;
+  const int2* lr;      // ni: owner, neighbour
+  const int* bowner;   // nb
+  const int* cfo;      // nc + 1
+  const int* inc;      // incidences
+  const long long* soff;  // per slice of 64 cells
+  const int* slen;     // per cell
+  const int* sidx;     // sliced entries: cell, or -(b + 1) for boundary face ni + b
+  const C* sw0;
+  const C* sw1;
+  const C* sw2;
+  const C* fgeo;    // 8 per face
+  const C* frr;     // 4 per interior face
+  const C* moment;  // 9 x nf SoA, or null (planar)
+  const C* vol;     // nc
+  const double* vol64;
+  const int* bkind;  // nb codes 0..5
+  const C* bc;       // 8 per boundary face
+  const C* source;   // 5 x nc SoA or null
+  C* prim;           // 8 per cell
+  C* prim_b;         // 8 per boundary face
+  C* rec;            // 20 per cell
+  C* fflux;          // 8 per face
+  double* res;       // 5 x nc
+  C* lam;            // nc
+  double* dt;        // nc
+  Small* small;
+};
+
+template <class C>
+__device__ __forceinline__ W5<C> load_w(const C* __restrict__ p, long long i) {
+  const C* q = p + 8 * i;
+  if constexpr (std::is_same_v<C, float>) {
+    const float4 a = *reinterpret_cast<const float4*>(q);
+    return {a.x, a.y, a.z, a.w, q[4]};
+  } else {
+    const double2 a = *reinterpret_cast<const double2*>(q);
+    const double2 b = *reinterpret_cast<const double2*>(q + 2);
+    return {a.x, a.y, b.x, b.y, q[4]};
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void store_w(C* __restrict__ p, long long i, const W5<C>& w) {
+  C* q = p + 8 * i;
+  if constexpr (std::is_same_v<C, float>) {
+    *reinterpret_cast<float4*>(q) = make_float4(w.r, w.u, w.v, w.w);
+    *reinterpret_cast<float4*>(q + 4) = make_float4(w.p, 0.f, 0.f, 0.f);
+  } else {
+    *reinterpret_cast<double2*>(q) = make_double2(w.r, w.u);
+    *reinterpret_cast<double2*>(q + 2) = make_double2(w.v, w.w);
+    *reinterpret_cast<double2*>(q + 4) = make_double2(w.p, 0.0);
+  }
+}
+
+// N consecutive values at a 16 B aligned address.
+template <class C, int N>
+__device__ __forceinline__ void load_n(const C* __restrict__ p, C* out) {
+  if constexpr (std::is_same_v<C, float>) {
+    static_assert(N % 4 == 0);
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+      const float4 v = reinterpret_cast<const float4*>(p)[i];
+      out[4 * i] = v.x;
+      out[4 * i + 1] = v.y;
+      out[4 * i + 2] = v.z;
+      out[4 * i + 3] = v.w;
+    }
+  } else {
+    static_assert(N % 2 == 0);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      const double2 v = reinterpret_cast<const double2*>(p)[i];
+      out[2 * i] = v.x;
+      out[2 * i + 1] = v.y;
+    }
+  }
+}
+
+template <class C, int N>
+__device__ __forceinline__ void store_n(C* __restrict__ p, const C* in) {
+  if constexpr (std::is_same_v<C, float>) {
+    static_assert(N % 4 == 0);
+#pragma unroll
+    for (int i = 0; i < N / 4; ++i) {
+      reinterpret_cast<float4*>(p)[i] = make_float4(in[4 * i], in[4 * i + 1], in[4 * i + 2], in[4 * i + 3]);
+    }
+  } else {
+    static_assert(N % 2 == 0);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) reinterpret_cast<double2*>(p)[i] = make_double2(in[2 * i], in[2 * i + 1]);
+  }
+}
+
+template <class C>
+__device__ __forceinline__ C infinity() {
+  if constexpr (std::is_same_v<C, float>) {
+    return __builtin_huge_valf();
+  } else {
+    return __builtin_huge_val();
+  }
+}
+
+// ------------------------------------------------------------------------ state and primitives
+
+// Euler::state then the primitives loop (physics/src/euler.cpp:925-932; kernels/euler.py:428-433):
+// s = State(u - U_ref), W = to_primitive(s).
+template <class C>
+__device__ __forceinline__ W5<C> primitive_of(const double* __restrict__ u, int nc, int c, const Params<C>& pa) {
+  C dq[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) dq[k] = static_cast<C>(u[static_cast<long long>(k) * nc + c] - pa.ref[k]);
+  return to_primitive(dq, pa.ur, pa.g);
+}
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_prim(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  store_w(d.prim, c, primitive_of(u, d.nc, c, pa));
+}
+
+// boundary_states (kernels/euler.py:442-466): the boundary value of every boundary face from its
+// owner's primitives.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d, Params<C> pa) {
+  const int b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= d.nb) return;
+  const W5<C> w = load_w(d.prim, d.bowner[b]);
+  C geo[8], data[8];
+  load_n<C, 8>(d.fgeo + 8LL * (d.ni + b), geo);
+  load_n<C, 8>(d.bc + 8LL * b, data);
+  const V3<C> n{geo[0], geo[1], geo[2]};
+  store_w(d.prim_b, b, boundary_value(w, d.bkind[b], data, n, pa.g, pa.rgas, pa.pref));
+}
+
+// ------------------------------------------------------------------------ gradient and limiter
+
+// gradients then limiter (kernels/euler.py:530-549, 563-608; reduction body
+// codegen/src/autan_codegen/emit_cpu.py:126-182), fused per cell: the bounds stay in registers.
+// Sums accumulate in FP64 and are stored in C; bounds and the limiter's minimum are taken in C,
+// ties keeping the first entry. Without the limiter phi = 1 (second order unlimited).
+template <class C, bool LIMITER>
+__global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d, Params<C> pa) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  const W5<C> wc = load_w(d.prim, c);
+  const C wcv[5] = {wc.r, wc.u, wc.v, wc.w, wc.p};
+  double acc[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  C lo[5], hi[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    lo[q] = infinity<C>();
+    hi[q] = -infinity<C>();
+  }
+  const long long base = d.soff[c / kWave] + (c % kWave);
+  const int len = d.slen[c];
+  for (int j = 0; j < len; ++j) {
+    const long long k = base + static_cast<long long>(j) * kWave;
+    const int e = d.sidx[k];
+    const C w0 = d.sw0[k], w1 = d.sw1[k], w2 = d.sw2[k];
+    const W5<C> wn = e >= 0 ? load_w(d.prim, e) : load_w(d.prim_b, -e - 1);
+    const C wnv[5] = {wn.r, wn.u, wn.v, wn.w, wn.p};
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const C dd = wnv[i] - wcv[i];
+      acc[3 * i] += static_cast<double>(dd * w0);
+      acc[3 * i + 1] += static_cast<double>(dd * w1);
+      acc[3 * i + 2] += static_cast<double>(dd * w2);
+      if (wnv[i] < lo[i]) lo[i] = wnv[i];
+      if (wnv[i] > hi[i]) hi[i] = wnv[i];
+    }
+  }
+  C out[20];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) out[q] = static_cast<C>(acc[q]);
+  if constexpr (LIMITER) {
+    C phi[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      lo[i] = smin(lo[i], wcv[i]);
+      hi[i] = smax(hi[i], wcv[i]);
+      phi[i] = infinity<C>();
+    }
+    const C vol = d.vol[c];
+    C eps2[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) eps2[i] = pa.scale[i] * vol;
+    for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+      const int e = d.inc[k];
+      const int f = e & 0x7fffffff;
+      C r[4];
+      if (e < 0) {
+        load_n<C, 4>(d.frr + 4LL * f, r);
+      } else {
+        load_n<C, 4>(d.fgeo + 8LL * f + 4, r);
+      }
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const C dd = out[3 * i] * r[0] + out[3 * i + 1] * r[1] + out[3 * i + 2] * r[2];
+        const C dmax = hi[i] - wcv[i], dmin = lo[i] - wcv[i];
+        const C dp = dd > C(0.0) ? dmax : dmin;
+        const C v = (dp * dp + eps2[i] + C(2.0) * dd * dp) / (dp * dp + C(2.0) * dd * dd + dd * dp + eps2[i]);
+        if (v < phi[i]) phi[i] = v;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[15 + i] = smin(phi[i], C(1.0));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[15 + i] = C(1.0);
+  }
+  store_n<C, 20>(d.rec + 20LL * c, out);
+}
+
+// ------------------------------------------------------------------------ face fluxes
+
+template <class C>
+__device__ __forceinline__ void load_rec(const C* __restrict__ rec, int c, C* g) {
+  load_n<C, 20>(rec + 20LL * c, g);
+}
+
+// {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for face f: the
+// area-weighted flux in C, stored for the per-cell gather (the store half of store_gather).
+template <class C, int FLUX, bool PLANAR, bool SECOND>
+__global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= d.nf) return;
+  C geo[8];
+  load_n<C, 8>(d.fgeo + 8LL * f, geo);
+  const V3<C> n{geo[0], geo[1], geo[2]};
+  const C area = geo[3];
+  F5<C> out;
+  if (f < d.ni) {
+    const int2 lr = d.lr[f];
+    W5<C> wl = load_w(d.prim, lr.x), wr = load_w(d.prim, lr.y);
+    C gl[20], gr[20];
+    W5<C> fl = wl, fr = wr;
+    if constexpr (SECOND) {
+      C rr[4];
+      load_n<C, 4>(d.frr + 4LL * f, rr);
+      load_rec(d.rec, lr.x, gl);
+      load_rec(d.rec, lr.y, gr);
+      fl = reconstruct(wl, gl, gl + 15, V3<C>{geo[4], geo[5], geo[6]}, pa.pref);
+      fr = reconstruct(wr, gr, gr + 15, V3<C>{rr[0], rr[1], rr[2]}, pa.pref);
+    }
+    const F5<C> fx = flux<FLUX>(fl, fr, n, pa.g, pa.minf, pa.pref);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out.f[k] = area * fx.f[k];
+    if constexpr (!PLANAR) {
+      C m[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+      if constexpr (!SECOND) {
+#pragma unroll
+        for (int q = 0; q < 20; ++q) gl[q] = gr[q] = C(0);
+      }
+      const W5<C> wm{C(0.5) * (fl.r + fr.r), C(0.5) * (fl.u + fr.u), C(0.5) * (fl.v + fr.v), C(0.5) * (fl.w + fr.w),
+                     C(0.5) * (fl.p + fr.p)};
+      const F5<C> wp = warp(wm, gl, gl + 15, gr, gr + 15, m, pa.g, pa.pref);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out.f[k] = out.f[k] + wp.f[k];
+    }
+  } else {
+    const int b = f - d.ni;
+    const W5<C> wl = load_w(d.prim, d.bowner[b]);
+    C gl[20];
+    W5<C> wf = wl;
+    if constexpr (SECOND || !PLANAR) {
+      if constexpr (SECOND) {
+        load_rec(d.rec, d.bowner[b], gl);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 20; ++q) gl[q] = C(0);
+      }
+    }
+    if constexpr (SECOND) wf = reconstruct(wl, gl, gl + 15, V3<C>{geo[4], geo[5], geo[6]}, pa.pref);
+    C data[8];
+    load_n<C, 8>(d.bc + 8LL * b, data);
+    const int kind = d.bkind[b];
+    const bool slip = kind == 2;
+    const bool wall = !PLANAR && kind >= 5;
+    if (slip || wall) {
+      const C wfx[5] = {C(0.0), wf.p * n.x, wf.p * n.y, wf.p * n.z, C(0.0)};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out.f[k] = area * wfx[k];
+      if constexpr (!PLANAR) {
+        C m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+        const C p4 = gl[15 + 4];
+        const C g4[3] = {gl[12], gl[13], gl[14]};
+        const C ww[5] = {C(0.0), p4 * (m[0] * g4[0] + m[3] * g4[1] + m[6] * g4[2]),
+                         p4 * (m[1] * g4[0] + m[4] * g4[1] + m[7] * g4[2]),
+                         p4 * (m[2] * g4[0] + m[5] * g4[1] + m[8] * g4[2]), C(0.0)};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) out.f[k] = out.f[k] + ww[k];
+      }
+    } else {
+      const W5<C> wb = boundary_value(wf, kind, data, n, pa.g, pa.rgas, pa.pref);
+      const F5<C> fx = flux<FLUX>(wf, wb, n, pa.g, pa.minf, pa.pref);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out.f[k] = area * fx.f[k];
+    }
+  }
+  C o8[8] = {out.f[0], out.f[1], out.f[2], out.f[3], out.f[4], C(0), C(0), C(0)};
+  store_n<C, 8>(d.fflux + 8LL * f, o8);
+}
+
+// sources (kernels/euler.py:611-616), then each cell's faces in its cell_faces order: the
+// interior faces' fluxes added with the cell's sign, then the boundary faces', each set summed
+// from zero in FP64 and added to res (codegen/src/autan_codegen/emit_cpu.py:506-564).
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_gather(Dev<C> d) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  const C nv = -d.vol[c];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, accb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+    const int e = d.inc[k];
+    const int f = e & 0x7fffffff;
+    C fx[8];
+    load_n<C, 8>(d.fflux + 8LL * f, fx);
+    if (f < d.ni) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) acc[q] += e < 0 ? -static_cast<double>(fx[q]) : static_cast<double>(fx[q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) accb[q] += static_cast<double>(fx[q]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const C s = d.source != nullptr ? d.source[static_cast<long long>(q) * d.nc + c] : C(0);
+    const double base = static_cast<double>(nv * s);
+    d.res[static_cast<long long>(q) * d.nc + c] = (base + acc[q]) + accb[q];
+  }
+}
+
+// ------------------------------------------------------------------------ time step
+
+// Order-preserving key of a double: a < b (not NaN) iff key(a) < key(b) as unsigned.
+__device__ __forceinline__ unsigned long long order_key(double x) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+  return (b >> 63) != 0 ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double key_value(unsigned long long k) {
+  const unsigned long long b = (k >> 63) != 0 ? (k & 0x7fffffffffffffffULL) : ~k;
+  double x;
+  __builtin_memcpy(&x, &b, 8);
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o /= 2) {
+    const T w = __shfl_xor(v, o, kWave);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o /= 2) {
+    const T w = __shfl_xor(v, o, kWave);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o /= 2) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// spectral_radius (kernels/euler.py:1091-1112), then local_time_step (dt_c = CFL V / lambda,
+// physics/src/device_stepper.cuh:83-89) and the least dt_c over the cells, exactly.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, Params<C> pa, double cfl, int store_dt) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long key = ~0ULL;
+  if (c < d.nc) {
+    const W5<C> wc = load_w(d.prim, c);
+    double acc = 0.0;
+    for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+      const int e = d.inc[k];
+      const int f = e & 0x7fffffff;
+      C geo[4];
+      load_n<C, 4>(d.fgeo + 8LL * f, geo);
+      const V3<C> n{geo[0], geo[1], geo[2]};
+      W5<C> wn;
+      if (f < d.ni) {
+        const int2 lr = d.lr[f];
+        wn = load_w(d.prim, e < 0 ? lr.x : lr.y);
+      } else {
+        wn = load_w(d.prim_b, f - d.ni);
+      }
+      const C lc = wave_speed(wc, n, pa.g, pa.pref);
+      const C ln = wave_speed(wn, n, pa.g, pa.pref);
+      acc += static_cast<double>(smax(lc, ln) * geo[3]);
+    }
+    const C lam = static_cast<C>(acc);
+    d.lam[c] = lam;
+    const double dtc = (cfl * d.vol64[c]) / static_cast<double>(lam);
+    if (store_dt) d.dt[c] = dtc;
+    if (!__builtin_isnan(dtc)) key = order_key(dtc);
+  }
+  key = wave_min(key);
+  if ((threadIdx.x % kWave) == 0 && key != ~0ULL) atomicMin(&d.small->hkey, key);
+}
+
+// The validity checks of physics/src/device_stepper.cuh:107-145 for cell c of state u and
+// residual res: the stored state read in C with the reference's gauge pressure.
+template <class C>
+__device__ __forceinline__ bool invalid_state(const double* __restrict__ u, int nc, int c, const Params<C>& pa) {
+  const C g = static_cast<C>(pa.gamma);
+  const C r0 = static_cast<C>(pa.ref[0]);
+  C dq[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) dq[k] = static_cast<C>(u[static_cast<long long>(k) * nc + c] - pa.ref[k]);
+  const C rho = r0 + dq[0];
+  C dm_m = 0, mr_mr = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const C mr = static_cast<C>(pa.ref[k + 1]);
+    dm_m = dm_m + dq[k + 1] * (C(2) * mr + dq[k + 1]);
+    mr_mr = mr_mr + mr * mr;
+  }
+  const C dk = (dm_m * r0 - mr_mr * dq[0]) / (rho * r0);
+  const C p = (g - C(1)) * (dq[4] - C(0.5) * dk) + static_cast<C>(pa.p_ref);
+  return !__builtin_isfinite(static_cast<double>(rho)) || !__builtin_isfinite(static_cast<double>(p)) ||
+         !(rho > C(0)) || !(p > C(0));
+}
+
+__device__ __forceinline__ void flag_least(int* slot, bool flagged, int c) {
+  int v = flagged ? c : INT32_MAX;
+  v = wave_min(v);
+  if ((threadIdx.x % kWave) == 0 && v != INT32_MAX) atomicMin(slot, v);
+}
+
+// One SSP-RK3 stage (physics/src/device_stepper.cuh:92-103): next = stage - (h / V) R,
+// stage = a u + b next; on the first stage `stage` is u itself. Then either the primitives of the
+// new stage for the next residual, or (last stage) the validity of the new state and residual.
+template <class C, bool PRIM, bool CHECK>
+__global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, Params<C> pa, double a, double b,
+                                               const double* __restrict__ u, const double* stage_in,
+                                               double* stage_out, const unsigned long long* hkey, double hvalue,
+                                               int local) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  bool bad_state = false, bad_res = false;
+  if (c < d.nc) {
+    const double h = local ? d.dt[c] : (hkey != nullptr ? key_value(*hkey) : hvalue);
+    const double hv = h / d.vol64[c];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const long long i = static_cast<long long>(k) * d.nc + c;
+      const double r = d.res[i];
+      const double next = stage_in[i] - hv * r;
+      stage_out[i] = a * u[i] + b * next;
+      if constexpr (CHECK) bad_res = bad_res || !__builtin_isfinite(r);
+    }
+    if constexpr (PRIM) store_w(d.prim, c, primitive_of(stage_out, d.nc, c, pa));
+    if constexpr (CHECK) bad_state = invalid_state(stage_out, d.nc, c, pa);
+  }
+  if constexpr (CHECK) {
+    flag_least(&d.small->invalid_state, bad_state, c);
+    flag_least(&d.small->invalid_residual, bad_res, c);
+  }
+}
+
+// DeviceStepper::evaluate's validity checks alone.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_check(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  bool bad_state = false, bad_res = false;
+  if (c < d.nc) {
+    bad_state = invalid_state(u, d.nc, c, pa);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bad_res = bad_res || !__builtin_isfinite(d.res[static_cast<long long>(k) * d.nc + c]);
+  }
+  flag_least(&d.small->invalid_state, bad_state, c);
+  flag_least(&d.small->invalid_residual, bad_res, c);
+}
+
+// ------------------------------------------------------------------------ exact norms
+
+// Adds each lane's term to the block's fixed-point limbs `acc` (LDS), exactly: the term split
+// into three 32-bit limbs from limb i as an exact sum splits it (runtime/src/exact_sum.cpp:25-48),
+// the wave's limbs summed lane-wide per limb index, one LDS atomic per limb per wave. Integer
+// sums commute, so the words do not depend on the launch or the order of the terms.
+__device__ __forceinline__ void exact_add_wave(long long* acc, double t, bool active) {
+  const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(t));
+  const int biased = static_cast<int>((bits >> 52) & 0x7ff);
+  const unsigned long long fraction = bits & ((1ULL << 52) - 1);
+  const bool negative = (bits >> 63) != 0;
+  if (active && biased == 0x7ff) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[fraction != 0 ? kLimbs + 2 : (negative ? kLimbs + 1 : kLimbs)]),
+              1ULL);
+  }
+  const bool finite = active && biased != 0x7ff && !(biased == 0 && fraction == 0);
+  int i = 1 << 30;
+  long long part[3] = {0, 0, 0};
+  if (finite) {
+    const unsigned long long m = biased == 0 ? fraction : (fraction | (1ULL << 52));
+    const int p = (biased == 0 ? -1074 : biased - 1075) + 1088;
+    i = p >> 5;
+    const int s = p & 31;
+    const unsigned long long lo = (m & 0xffffffffULL) << s, hi = (m >> 32) << s;
+    part[0] = static_cast<long long>(lo & 0xffffffffULL);
+    part[1] = static_cast<long long>((lo >> 32) + (hi & 0xffffffffULL));
+    part[2] = static_cast<long long>(hi >> 32);
+    if (negative) {
+      part[0] = -part[0];
+      part[1] = -part[1];
+      part[2] = -part[2];
+    }
+  }
+  const int first = wave_min(i);
+  const int last = wave_max(finite ? i : -1);
+  for (int j = first; j <= last + 2; ++j) {
+    const int dj = j - i;
+    const long long v = wave_sum(finite && dj >= 0 && dj < 3 ? part[dj] : 0LL);
+    if ((threadIdx.x % kWave) == 0 && v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&acc[j]), static_cast<unsigned long long>(v));
+  }
+}
+
+// The five sums of R_k^2 over the cells (EulerPhysics::record_norms), as exact fixed-point words
+// added into small->words (cleared beforehand). Grid-stride; each block normalises its limbs
+// into [0, 2^32) before adding them, so any number of blocks fits the int64 words.
+__global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ res, int nc, Small* small) {
+  __shared__ long long acc[kSums][kWords];
+  for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) (&acc[0][0])[t] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock + wave * kWave; base < nc; base += stride) {
+    const long long c = base + lane;
+    const bool active = c < nc;
+#pragma unroll
+    for (int k = 0; k < kSums; ++k) {
+      const double r = active ? res[k * static_cast<long long>(nc) + c] : 0.0;
+      exact_add_wave(acc[k], r * r, active);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kSums) {
+    long long* a = acc[threadIdx.x];
+    for (int l = 0; l + 1 < kLimbs; ++l) {
+      const long long carry = a[l] >> 32;
+      a[l] -= carry * (1LL << 32);
+      a[l + 1] += carry;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) {
+    const long long v = (&acc[0][0])[t];
+    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&small->words[0][0] + t), static_cast<unsigned long long>(v));
+  }
+}
+
+__global__ void k_clear(Small* small) {
+  for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) (&small->words[0][0])[t] = 0;
+  if (threadIdx.x == 0) {
+    small->hkey = 0xfff0000000000000ULL;  // order_key(+inf): no cell, an infinite step
+    small->invalid_state = INT32_MAX;
+    small->invalid_residual = INT32_MAX;
+  }
+}
+
+// ------------------------------------------------------------------------ math probe
+
+template <class C>
+__global__ void k_probe(const double* x, const double* y, long long n, double* s, double* q, double* e, double* l) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const C a = static_cast<C>(x[i]), b = static_cast<C>(y[i]);
+  s[i] = dsqrt(a);
+  q[i] = a / b;
+  e[i] = dexp(a);
+  l[i] = dlog(a);
+}
+
+}  // namespace mxe

@@ -1,0 +1,833 @@
+// The device stepper behind include/mxeuler.h: device layout, launches and the C ABI.
+//
+// Mirrors CudaStepper<P> (physics/src/device_stepper.cuh:200-529): one residual is the chain
+// primitives -> boundary states -> gradients + limiter -> sources + faces + boundaries; one step
+// is R(u) and its exact norms, the spectral radius and time step, and three SSP-RK3 stages.
+// Unlike the reference, a step makes no blocking read before its end: the least time step stays
+// on the device for the stages, and the norms, the time step and the validity flags come back in
+// one copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/mxeuler.h"
+#include "kernels.hpp"
+
+namespace mxe {
+
+namespace {
+
+void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("E1101: ") + what + ": " + hipGetErrorString(e));
+}
+
+int blocks(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+
+template <class T>
+class Buffer {
+ public:
+  Buffer() = default;
+  explicit Buffer(std::size_t n) : n_(n) {
+    if (n > 0) check(hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)), "hipMalloc");
+  }
+  Buffer(const Buffer&) = delete;
+  Buffer& operator=(const Buffer&) = delete;
+  Buffer(Buffer&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  Buffer& operator=(Buffer&& o) noexcept {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+    return *this;
+  }
+  ~Buffer() {
+    if (p_ != nullptr) (void)hipFree(p_);
+  }
+  void upload(const T* h, std::size_t n, hipStream_t s) {
+    if (n > n_) throw std::invalid_argument("upload larger than the buffer");
+    if (n > 0) check(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    *this = Buffer<T>(v.size());
+    upload(v.data(), v.size(), s);
+  }
+  void download(T* h, std::size_t n, hipStream_t s) const {
+    if (n > 0) check(hipMemcpyAsync(h, p_, n * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+  }
+  T* get() const { return p_; }
+  std::size_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+// Whole-number boundary codes as the bodies' midpoint comparisons read them
+// (kernels/euler.py:391-422).
+int code_of(double kind) {
+  if (kind < 0.5) return 0;
+  if (kind < 1.5) return 1;
+  if (kind < 2.5) return 2;
+  if (kind < 3.5) return 3;
+  if (kind < 4.5) return 4;
+  return 5;
+}
+
+}  // namespace
+
+class StepperBase {
+ public:
+  virtual ~StepperBase() = default;
+  virtual void upload(const double* u) = 0;
+  virtual void download(double* u) = 0;
+  virtual void residual(const double* u, double* res) = 0;
+  virtual void step(double cfl, bool has_dt, double dt, bool local, double* residuals, double* time_step,
+                    int* invalid) = 0;
+  virtual void evaluate(double* residuals, int* invalid) = 0;
+  virtual void set_second_order(bool on) = 0;
+  virtual void set_boundary_data(const double* bc) = 0;
+  virtual void set_source(const double* source) = 0;
+  virtual void spectral_radius(double* lam) = 0;
+  virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
+  virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
+                      int max_kernels, std::string* names) = 0;
+};
+
+template <class C>
+class Stepper final : public StepperBase {
+ public:
+  Stepper(const mxeuler_scheme& sc, const mxeuler_topology& t, const mxeuler_geometry& geo, const mxeuler_stencil& st,
+          const double* bc_kind, const double* bc, const double* source, hipStream_t stream)
+      : sc_(sc) {
+    validate(sc, t, geo, st);
+    if (stream != nullptr) {
+      stream_ = stream;
+    } else {
+      check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+      own_stream_ = true;
+    }
+    nc_ = t.n_cells;
+    nf_ = t.n_faces;
+    ni_ = t.n_interior;
+    nb_ = nf_ - ni_;
+    second_ = sc.second_order != 0;
+    planar_ = sc.planar != 0;
+    params(sc);
+    build_topology(t);
+    build_stencil(t, st);
+    build_geometry(t, geo);
+    set_boundary(bc_kind, bc);
+    set_source(source);
+    // State and work arrays.
+    u_ = Buffer<double>(5 * std::size_t(nc_));
+    stage_ = Buffer<double>(5 * std::size_t(nc_));
+    res_ = Buffer<double>(5 * std::size_t(nc_));
+    prim_ = Buffer<C>(8 * std::size_t(nc_));
+    prim_b_ = Buffer<C>(8 * std::size_t(std::max(nb_, 1)));
+    rec_ = Buffer<C>(20 * std::size_t(nc_));
+    fflux_ = Buffer<C>(8 * std::size_t(nf_));
+    lam_ = Buffer<C>(std::size_t(nc_));
+    dt_ = Buffer<double>(std::size_t(nc_));
+    small_ = Buffer<Small>(1);
+    check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small), hipHostMallocDefault), "hipHostMalloc");
+    check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
+    check(hipStreamSynchronize(stream_), "create");
+  }
+
+  ~Stepper() override {
+    (void)hipStreamSynchronize(stream_);
+    if (host_small_ != nullptr) (void)hipHostFree(host_small_);
+    if (own_stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  void upload(const double* u) override {
+    u_.upload(u, 5 * std::size_t(nc_), stream_);
+    check(hipStreamSynchronize(stream_), "upload");
+  }
+
+  void download(double* u) override {
+    u_.download(u, 5 * std::size_t(nc_), stream_);
+    check(hipStreamSynchronize(stream_), "download");
+  }
+
+  void residual(const double* u, double* res) override {
+    stage_.upload(u, 5 * std::size_t(nc_), stream_);
+    launch_prim(stage_.get());
+    residual_from_prim();
+    res_.download(res, 5 * std::size_t(nc_), stream_);
+    check(hipStreamSynchronize(stream_), "residual");
+  }
+
+  void step(double cfl, bool has_dt, double dt, bool local, double* residuals, double* time_step,
+            int* invalid) override {
+    enqueue_step(cfl, has_dt, dt, local);
+    finish(residuals, invalid);
+    *time_step = has_dt ? dt : key_value(host_small_->hkey);
+  }
+
+  void evaluate(double* residuals, int* invalid) override {
+    clear();
+    launch_prim(u_.get());
+    residual_from_prim();
+    launch("norms", [&] { k_norms<<<norm_blocks(), kBlock, 0, stream_>>>(res_.get(), nc_, small_.get()); });
+    launch("check", [&] { k_check<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, u_.get()); });
+    finish(residuals, invalid);
+  }
+
+  void set_second_order(bool on) override {
+    second_ = on;
+    check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
+    check(hipStreamSynchronize(stream_), "set_second_order");
+  }
+
+  void set_boundary_data(const double* bc) override {
+    std::vector<C> b(8 * std::size_t(std::max(nb_, 1)), C(0));
+    for (int i = 0; i < nb_; ++i) {
+      for (int k = 0; k < 5; ++k) b[8 * std::size_t(i) + k] = static_cast<C>(bc[std::size_t(k) * nb_ + i]);
+    }
+    bc_.upload(b, stream_);
+    check(hipStreamSynchronize(stream_), "set_boundary_data");
+  }
+
+  void set_source(const double* source) override {
+    if (source == nullptr) {
+      source_ = Buffer<C>();
+      return;
+    }
+    std::vector<C> s(5 * std::size_t(nc_));
+    for (std::size_t i = 0; i < s.size(); ++i) s[i] = static_cast<C>(source[i]);
+    source_.upload(s, stream_);
+    check(hipStreamSynchronize(stream_), "set_source");
+  }
+
+  void spectral_radius(double* lam) override {
+    clear();
+    launch("spectral", [&] { k_spectral<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, 1.0, 0); });
+    std::vector<C> l(static_cast<std::size_t>(nc_));
+    lam_.download(l.data(), l.size(), stream_);
+    check(hipStreamSynchronize(stream_), "spectral_radius");
+    for (int c = 0; c < nc_; ++c) lam[c] = static_cast<double>(l[std::size_t(c)]);
+  }
+
+  void arrays(double* prim, double* grad, double* phi, double* prim_b) override {
+    std::vector<C> p(prim_.size()), r(rec_.size()), pb(prim_b_.size());
+    prim_.download(p.data(), p.size(), stream_);
+    rec_.download(r.data(), r.size(), stream_);
+    prim_b_.download(pb.data(), pb.size(), stream_);
+    check(hipStreamSynchronize(stream_), "arrays");
+    const std::size_t n = std::size_t(nc_);
+    for (std::size_t c = 0; c < n; ++c) {
+      for (std::size_t k = 0; k < 5; ++k) {
+        if (prim != nullptr) prim[k * n + c] = p[8 * c + k];
+        if (phi != nullptr) phi[k * n + c] = r[20 * c + 15 + k];
+      }
+      for (std::size_t q = 0; q < 15 && grad != nullptr; ++q) grad[q * n + c] = r[20 * c + q];
+    }
+    for (std::size_t b = 0; b < std::size_t(nb_) && prim_b != nullptr; ++b) {
+      for (std::size_t k = 0; k < 5; ++k) prim_b[k * std::size_t(nb_) + b] = pb[8 * b + k];
+    }
+  }
+
+  int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches, int max_kernels,
+              std::string* names) override {
+    // Warm the path once, then time `steps` steps with an event around every launch.
+    enqueue_step(cfl, false, 0.0, local);
+    check(hipStreamSynchronize(stream_), "profile warm-up");
+    prof_ = true;
+    prof_names_.clear();
+    prof_events_.clear();
+    hipEvent_t start, stop;
+    check(hipEventCreate(&start), "hipEventCreate");
+    check(hipEventCreate(&stop), "hipEventCreate");
+    check(hipEventRecord(start, stream_), "hipEventRecord");
+    for (int s = 0; s < steps; ++s) enqueue_step(cfl, false, 0.0, local);
+    check(hipEventRecord(stop, stream_), "hipEventRecord");
+    check(hipEventSynchronize(stop), "hipEventSynchronize");
+    prof_ = false;
+    float total = 0.0f;
+    check(hipEventElapsedTime(&total, start, stop), "hipEventElapsedTime");
+    *ms_step = total / steps;
+    std::vector<std::string> order;
+    std::vector<double> sum;
+    std::vector<int> count;
+    for (std::size_t i = 0; i < prof_events_.size(); ++i) {
+      float ms = 0.0f;
+      check(hipEventElapsedTime(&ms, prof_events_[i].first, prof_events_[i].second), "hipEventElapsedTime");
+      auto it = std::find(order.begin(), order.end(), prof_names_[i]);
+      if (it == order.end()) {
+        order.push_back(prof_names_[i]);
+        sum.push_back(0.0);
+        count.push_back(0);
+        it = order.end() - 1;
+      }
+      const auto j = static_cast<std::size_t>(it - order.begin());
+      sum[j] += ms;
+      count[j] += 1;
+      (void)hipEventDestroy(prof_events_[i].first);
+      (void)hipEventDestroy(prof_events_[i].second);
+    }
+    prof_events_.clear();
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    const int n = std::min<int>(max_kernels, static_cast<int>(order.size()));
+    std::string joined;
+    for (int i = 0; i < n; ++i) {
+      ms_kernel[i] = sum[std::size_t(i)] / count[std::size_t(i)];
+      launches[i] = count[std::size_t(i)] / steps;
+      joined += (i ? ";" : "") + order[std::size_t(i)];
+    }
+    *names = joined;
+    return n;
+  }
+
+ private:
+  Dev<C> dev() {
+    Dev<C> d{};
+    d.nc = nc_;
+    d.nf = nf_;
+    d.ni = ni_;
+    d.nb = nb_;
+    d.lr = lr_.get();
+    d.bowner = bowner_.get();
+    d.cfo = cfo_.get();
+    d.inc = inc_.get();
+    d.soff = soff_.get();
+    d.slen = slen_.get();
+    d.sidx = sidx_.get();
+    d.sw0 = sw0_.get();
+    d.sw1 = sw1_.get();
+    d.sw2 = sw2_.get();
+    d.fgeo = fgeo_.get();
+    d.frr = frr_.get();
+    d.moment = moment_.get();
+    d.vol = vol_.get();
+    d.vol64 = vol64_.get();
+    d.bkind = bkind_.get();
+    d.bc = bc_.get();
+    d.source = source_.get();
+    d.prim = prim_.get();
+    d.prim_b = prim_b_.get();
+    d.rec = rec_.get();
+    d.fflux = fflux_.get();
+    d.res = res_.get();
+    d.lam = lam_.get();
+    d.dt = dt_.get();
+    d.small = small_.get();
+    return d;
+  }
+
+  static void validate(const mxeuler_scheme& sc, const mxeuler_topology& t, const mxeuler_geometry& geo,
+                       const mxeuler_stencil& st) {
+    if (t.n_cells <= 0 || t.n_faces <= 0 || t.n_interior < 0 || t.n_interior > t.n_faces) {
+      throw std::invalid_argument("topology sizes");
+    }
+    if (!t.owner || (t.n_interior > 0 && !t.neighbour) || !t.cell_face_offsets || !t.cell_faces) {
+      throw std::invalid_argument("topology arrays missing");
+    }
+    if (!geo.face_normal || !geo.face_area || !geo.r_left || !geo.r_right || !geo.cell_volume) {
+      throw std::invalid_argument("geometry arrays missing");
+    }
+    if (sc.planar && !geo.all_planar) {
+      throw std::invalid_argument("planar loops on a mesh with a face not proven planar");
+    }
+    if (!sc.planar && !geo.face_moment) throw std::invalid_argument("face moments missing for the full loops");
+    if (sc.second_order && (!st.offsets || !st.entries || !st.weights)) {
+      throw std::invalid_argument("gradient stencil missing");
+    }
+    if (sc.flux < 0 || sc.flux > 2) throw std::invalid_argument("unknown flux");
+    if (!(sc.mach_ref > 0.0)) throw std::invalid_argument("AUSM+-up needs a positive cut-off Mach number");
+    if (!(sc.reference[0] > 0.0)) throw std::invalid_argument("reference density must be positive");
+    const long long nc = t.n_cells, nf = t.n_faces, ni = t.n_interior;
+    if (t.cell_face_offsets[0] != 0 || t.cell_face_offsets[nc] > std::numeric_limits<int>::max()) {
+      throw std::invalid_argument("cell_face_offsets");
+    }
+    for (long long c = 0; c < nc; ++c) {
+      if (t.cell_face_offsets[c + 1] < t.cell_face_offsets[c]) throw std::invalid_argument("cell_face_offsets");
+    }
+    for (long long f = 0; f < nf; ++f) {
+      if (t.owner[f] < 0 || t.owner[f] >= nc) throw std::invalid_argument("owner out of range");
+      if (f < ni && (t.neighbour[f] < 0 || t.neighbour[f] >= nc || t.neighbour[f] == t.owner[f])) {
+        throw std::invalid_argument("neighbour out of range");
+      }
+    }
+    for (long long k = 0; k < t.cell_face_offsets[nc]; ++k) {
+      if (t.cell_faces[k] < 0 || t.cell_faces[k] >= nf) throw std::invalid_argument("cell_faces out of range");
+    }
+    for (long long c = 0; c < nc; ++c) {
+      for (long long k = t.cell_face_offsets[c]; k < t.cell_face_offsets[c + 1]; ++k) {
+        const long long f = t.cell_faces[k];
+        if (t.owner[f] != c && (f >= ni || t.neighbour[f] != c)) {
+          throw std::invalid_argument("cell_faces lists a face of another cell");
+        }
+      }
+    }
+    if (sc.second_order) {
+      if (st.offsets[0] != 0) throw std::invalid_argument("stencil offsets");
+      for (long long c = 0; c < nc; ++c) {
+        if (st.offsets[c + 1] < st.offsets[c]) throw std::invalid_argument("stencil offsets");
+      }
+      for (long long k = 0; k < st.offsets[nc]; ++k) {
+        const long long e = st.entries[k];
+        if (e >= nc || (e < 0 && (-e - 1 < ni || -e - 1 >= nf))) throw std::invalid_argument("stencil entry out of range");
+      }
+    }
+  }
+
+  void params(const mxeuler_scheme& sc) {
+    // physics/src/euler.cpp:238-245, in host code built with -ffp-contract=off.
+    const double* r = sc.reference;
+    const double p_ref = (sc.gamma - 1.0) * (r[4] - 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[0]);
+    const double k3 = std::pow(sc.venkat_k / sc.length, 3);
+    pa_.g = static_cast<C>(sc.gamma);
+    pa_.rgas = static_cast<C>(sc.gas_constant);
+    pa_.pref = static_cast<C>(p_ref);
+    pa_.minf = static_cast<C>(sc.mach_ref);
+    for (int k = 0; k < 5; ++k) {
+      pa_.ur[k] = static_cast<C>(r[k]);
+      pa_.scale[k] = static_cast<C>(k3 * sc.scale[k] * sc.scale[k]);
+      pa_.ref[k] = r[k];
+    }
+    pa_.p_ref = p_ref;
+    pa_.gamma = sc.gamma;
+  }
+
+  void build_topology(const mxeuler_topology& t) {
+    std::vector<int2> lr(std::size_t(std::max(ni_, 1)));
+    for (int f = 0; f < ni_; ++f) lr[std::size_t(f)] = make_int2(t.owner[f], t.neighbour[f]);
+    std::vector<int> bowner(std::size_t(std::max(nb_, 1)), 0);
+    for (int b = 0; b < nb_; ++b) bowner[std::size_t(b)] = t.owner[ni_ + b];
+    std::vector<int> cfo(std::size_t(nc_) + 1);
+    for (int c = 0; c <= nc_; ++c) cfo[std::size_t(c)] = static_cast<int>(t.cell_face_offsets[c]);
+    std::vector<int> inc(std::size_t(std::max(cfo.back(), 1)));
+    for (int c = 0; c < nc_; ++c) {
+      for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) {
+        const int f = t.cell_faces[k];
+        const bool right = f < ni_ && t.owner[f] != c;
+        inc[std::size_t(k)] = static_cast<int>(static_cast<unsigned>(f) | (right ? 0x80000000u : 0u));
+      }
+    }
+    lr_.upload(lr, stream_);
+    bowner_.upload(bowner, stream_);
+    cfo_.upload(cfo, stream_);
+    inc_.upload(inc, stream_);
+    check(hipStreamSynchronize(stream_), "topology");
+  }
+
+  void build_stencil(const mxeuler_topology& t, const mxeuler_stencil& st) {
+    const int slices = (nc_ + kWave - 1) / kWave;
+    std::vector<long long> soff(std::size_t(slices) + 1, 0);
+    std::vector<int> slen(std::size_t(nc_), 0);
+    const bool have = st.offsets != nullptr;
+    for (int s = 0; s < slices; ++s) {
+      int longest = 0;
+      for (int l = 0; l < kWave && s * kWave + l < nc_; ++l) {
+        const int c = s * kWave + l;
+        const int len = have ? static_cast<int>(st.offsets[c + 1] - st.offsets[c]) : 0;
+        slen[std::size_t(c)] = len;
+        longest = std::max(longest, len);
+      }
+      soff[std::size_t(s) + 1] = soff[std::size_t(s)] + static_cast<long long>(longest) * kWave;
+    }
+    const std::size_t total = std::max<std::size_t>(std::size_t(soff.back()), 1);
+    std::vector<int> idx(total, 0);
+    std::vector<C> w0(total, C(0)), w1(total, C(0)), w2(total, C(0));
+    const long long ns = have ? st.offsets[nc_] : 0;
+    for (int c = 0; have && c < nc_; ++c) {
+      const long long base = soff[std::size_t(c / kWave)] + c % kWave;
+      for (long long j = 0; j < st.offsets[c + 1] - st.offsets[c]; ++j) {
+        const long long k = st.offsets[c] + j;
+        const std::size_t at = std::size_t(base + j * kWave);
+        const int e = st.entries[k];
+        idx[at] = e >= 0 ? e : -((-e - 1) - t.n_interior) - 1;
+        w0[at] = static_cast<C>(st.weights[k]);
+        w1[at] = static_cast<C>(st.weights[ns + k]);
+        w2[at] = static_cast<C>(st.weights[2 * ns + k]);
+      }
+    }
+    soff_.upload(soff, stream_);
+    slen_.upload(slen, stream_);
+    sidx_.upload(idx, stream_);
+    sw0_.upload(w0, stream_);
+    sw1_.upload(w1, stream_);
+    sw2_.upload(w2, stream_);
+    check(hipStreamSynchronize(stream_), "stencil");
+  }
+
+  void build_geometry(const mxeuler_topology& t, const mxeuler_geometry& g) {
+    const std::size_t nf = std::size_t(nf_), ni = std::size_t(ni_), nc = std::size_t(nc_);
+    std::vector<C> fgeo(8 * nf), frr(4 * std::max<std::size_t>(ni, 1), C(0));
+    for (std::size_t f = 0; f < nf; ++f) {
+      for (std::size_t u = 0; u < 3; ++u) {
+        fgeo[8 * f + u] = static_cast<C>(g.face_normal[u * nf + f]);
+        fgeo[8 * f + 4 + u] = static_cast<C>(g.r_left[u * nf + f]);
+        if (f < ni) frr[4 * f + u] = static_cast<C>(g.r_right[u * nf + f]);
+      }
+      fgeo[8 * f + 3] = static_cast<C>(g.face_area[f]);
+      fgeo[8 * f + 7] = C(0);
+    }
+    fgeo_.upload(fgeo, stream_);
+    frr_.upload(frr, stream_);
+    if (!planar_) {
+      std::vector<C> m(9 * nf);
+      for (std::size_t i = 0; i < 9 * nf; ++i) m[i] = static_cast<C>(g.face_moment[i]);
+      moment_.upload(m, stream_);
+    }
+    std::vector<C> vol(nc);
+    std::vector<double> vol64(g.cell_volume, g.cell_volume + nc);
+    for (std::size_t c = 0; c < nc; ++c) vol[c] = static_cast<C>(g.cell_volume[c]);
+    vol_.upload(vol, stream_);
+    vol64_.upload(vol64, stream_);
+    (void)t;
+    check(hipStreamSynchronize(stream_), "geometry");
+  }
+
+  void set_boundary(const double* bc_kind, const double* bc) {
+    std::vector<int> kind(std::size_t(std::max(nb_, 1)), 0);
+    for (int b = 0; b < nb_; ++b) kind[std::size_t(b)] = code_of(bc_kind[b]);
+    bkind_.upload(kind, stream_);
+    if (nb_ > 0 && bc == nullptr) throw std::invalid_argument("boundary data missing");
+    set_boundary_data(bc);
+  }
+
+  template <class F>
+  void launch(const char* name, F&& f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof_) {
+      check(hipEventCreate(&a), "hipEventCreate");
+      check(hipEventCreate(&b), "hipEventCreate");
+      check(hipEventRecord(a, stream_), "hipEventRecord");
+    }
+    f();
+    check(hipGetLastError(), name);
+    if (prof_) {
+      check(hipEventRecord(b, stream_), "hipEventRecord");
+      prof_names_.emplace_back(name);
+      prof_events_.emplace_back(a, b);
+    }
+  }
+
+  int norm_blocks() const { return std::max(1, std::min(blocks(nc_), 2048)); }
+
+  void clear() {
+    launch("clear", [&] { k_clear<<<1, kBlock, 0, stream_>>>(small_.get()); });
+  }
+
+  void launch_prim(const double* u) {
+    launch("prim", [&] { k_prim<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, u); });
+  }
+
+  // Impl::cell_loops and Impl::faces from the primitives already stored.
+  void residual_from_prim() {
+    const Dev<C> d = dev();
+    if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d, pa_); });
+    if (second_) {
+      if (sc_.limiter) {
+        launch("grad_limiter", [&] { k_grad<C, true><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_); });
+      } else {
+        launch("grad", [&] { k_grad<C, false><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_); });
+      }
+    }
+    launch("flux", [&] { launch_flux(d); });
+    launch("gather", [&] { k_gather<C><<<blocks(nc_), kBlock, 0, stream_>>>(d); });
+  }
+
+  template <int FLUX>
+  void launch_flux_t(const Dev<C>& d) {
+    const int g = blocks(nf_);
+    if (planar_) {
+      if (second_) {
+        k_flux<C, FLUX, true, true><<<g, kBlock, 0, stream_>>>(d, pa_);
+      } else {
+        k_flux<C, FLUX, true, false><<<g, kBlock, 0, stream_>>>(d, pa_);
+      }
+    } else {
+      if (second_) {
+        k_flux<C, FLUX, false, true><<<g, kBlock, 0, stream_>>>(d, pa_);
+      } else {
+        k_flux<C, FLUX, false, false><<<g, kBlock, 0, stream_>>>(d, pa_);
+      }
+    }
+  }
+
+  void launch_flux(const Dev<C>& d) {
+    switch (sc_.flux) {
+      case MXEULER_ROE:
+        launch_flux_t<0>(d);
+        break;
+      case MXEULER_HLLC:
+        launch_flux_t<1>(d);
+        break;
+      default:
+        launch_flux_t<2>(d);
+        break;
+    }
+  }
+
+  // CudaStepper::step (physics/src/device_stepper.cuh:313-345) without a host round trip.
+  void enqueue_step(double cfl, bool has_dt, double dt, bool local) {
+    clear();
+    launch_prim(u_.get());
+    residual_from_prim();
+    launch("norms", [&] { k_norms<<<norm_blocks(), kBlock, 0, stream_>>>(res_.get(), nc_, small_.get()); });
+    const bool use_local = !has_dt && local;
+    if (!has_dt) {
+      launch("spectral", [&] {
+        k_spectral<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, cfl, use_local ? 1 : 0);
+      });
+    }
+    const unsigned long long* hkey = has_dt ? nullptr : &small_.get()->hkey;
+    // Shu and Osher (1988), eq. 2.18.
+    static constexpr double w[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
+    for (int s = 0; s < 3; ++s) {
+      if (s > 0) residual_from_prim();
+      const double* in = s == 0 ? u_.get() : stage_.get();
+      const Dev<C> d = dev();
+      if (s < 2) {
+        launch("rk_prim", [&] {
+          k_rk<C, true, false><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
+                                                                     stage_.get(), hkey, dt, use_local ? 1 : 0);
+        });
+      } else {
+        launch("rk_check", [&] {
+          k_rk<C, false, true><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
+                                                                     stage_.get(), hkey, dt, use_local ? 1 : 0);
+        });
+      }
+    }
+    std::swap(u_, stage_);
+  }
+
+  // The step's single copy back: norms, time step and validity.
+  void finish(double* residuals, int* invalid) {
+    check(hipMemcpyAsync(host_small_, small_.get(), sizeof(Small), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "step");
+    for (int k = 0; k < kSums; ++k) residuals[k] = std::sqrt(exact_value(host_small_->words[k]) / double(nc_));
+    const int a = host_small_->invalid_state, b = host_small_->invalid_residual;
+    *invalid = a != INT32_MAX ? a : (b != INT32_MAX ? b : -1);
+  }
+
+  // The sum the words hold, rounded once (runtime/src/exact_sum.cpp:63-100).
+  static double exact_value(const long long* in) {
+    constexpr double inf = std::numeric_limits<double>::infinity();
+    if (in[kLimbs + 2] > 0 || (in[kLimbs] > 0 && in[kLimbs + 1] > 0)) return std::numeric_limits<double>::quiet_NaN();
+    if (in[kLimbs] > 0) return inf;
+    if (in[kLimbs + 1] > 0) return -inf;
+    long long w[kLimbs];
+    for (int i = 0; i < kLimbs; ++i) w[i] = in[i];
+    auto normalize = [&w] {
+      for (int i = 0; i + 1 < kLimbs; ++i) {
+        const long long carry = w[i] >> 32;
+        w[i] -= carry * (1LL << 32);
+        w[i + 1] += carry;
+      }
+    };
+    normalize();
+    const bool negative = w[kLimbs - 1] < 0;
+    if (negative) {
+      for (long long& v : w) v = -v;
+      normalize();
+    }
+    int t = kLimbs - 1;
+    while (t >= 0 && w[t] == 0) --t;
+    if (t < 0) return 0.0;
+    auto limb = [&w](int k) { return k < 0 ? 0ULL : static_cast<unsigned long long>(w[k]); };
+    const unsigned long long top = limb(t);
+    if (top > 0xffffffffULL) return negative ? -inf : inf;
+    int b = 0;
+    while ((top >> b) != 0) ++b;
+    const unsigned long long below = (limb(t - 1) << 32) | limb(t - 2);
+    const unsigned long long lead = (top << (64 - b)) | (below >> b);
+    bool sticky = (below & ((1ULL << b) - 1)) != 0;
+    for (int k = t - 3; k >= 0 && !sticky; --k) sticky = limb(k) != 0;
+    unsigned long long mantissa = lead >> 11;
+    const unsigned long long rest = lead & 0x7ff;
+    if (rest > 0x400 || (rest == 0x400 && (sticky || (mantissa & 1) != 0))) ++mantissa;
+    int exponent = 32 * (t - 2) - 1088 + b + 11;
+    if (mantissa == (1ULL << 53)) {
+      mantissa >>= 1;
+      ++exponent;
+    }
+    const double magnitude = std::ldexp(static_cast<double>(mantissa), exponent);
+    return negative ? -magnitude : magnitude;
+  }
+
+  mxeuler_scheme sc_;
+  hipStream_t stream_ = nullptr;
+  bool own_stream_ = false;
+  int nc_ = 0, nf_ = 0, ni_ = 0, nb_ = 0;
+  bool second_ = true, planar_ = true;
+  Params<C> pa_{};
+  Buffer<int2> lr_;
+  Buffer<int> bowner_, cfo_, inc_, slen_, sidx_, bkind_;
+  Buffer<long long> soff_;
+  Buffer<C> sw0_, sw1_, sw2_, fgeo_, frr_, moment_, vol_, bc_, source_;
+  Buffer<double> vol64_;
+  Buffer<double> u_, stage_, res_, dt_;
+  Buffer<C> prim_, prim_b_, rec_, fflux_, lam_;
+  Buffer<Small> small_;
+  Small* host_small_ = nullptr;
+  bool prof_ = false;
+  std::vector<std::string> prof_names_;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events_;
+};
+
+}  // namespace mxe
+
+struct mxeuler_stepper {
+  std::unique_ptr<mxe::StepperBase> impl;
+  int nc = 0, nb = 0;
+};
+
+namespace {
+
+int fail(const std::exception& e, char* err, size_t err_len) {
+  if (err != nullptr && err_len > 0) {
+    std::strncpy(err, e.what(), err_len - 1);
+    err[err_len - 1] = '\0';
+  }
+  return 1;
+}
+
+template <class F>
+int guarded(char* err, size_t err_len, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    return fail(e, err, err_len);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mxeuler_create(int32_t policy, const mxeuler_scheme* scheme, const mxeuler_topology* topology,
+                   const mxeuler_geometry* geometry, const mxeuler_stencil* stencil, const double* bc_kind,
+                   const double* bc, const double* source, void* stream, mxeuler_stepper** out, char* err,
+                   size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!scheme || !topology || !geometry || !stencil || !out) throw std::invalid_argument("null argument");
+    auto h = std::make_unique<mxeuler_stepper>();
+    const auto s = static_cast<hipStream_t>(stream);
+    if (policy == MXEULER_FP64) {
+      h->impl = std::make_unique<mxe::Stepper<double>>(*scheme, *topology, *geometry, *stencil, bc_kind, bc, source, s);
+    } else if (policy == MXEULER_FP32_ACC64) {
+      h->impl = std::make_unique<mxe::Stepper<float>>(*scheme, *topology, *geometry, *stencil, bc_kind, bc, source, s);
+    } else {
+      throw std::invalid_argument("unknown precision policy");
+    }
+    h->nc = topology->n_cells;
+    h->nb = topology->n_faces - topology->n_interior;
+    *out = h.release();
+  });
+}
+
+void mxeuler_destroy(mxeuler_stepper* s) { delete s; }
+
+int mxeuler_upload(mxeuler_stepper* s, const double* u, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->upload(u); });
+}
+
+int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->download(u); });
+}
+
+int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->residual(u, res); });
+}
+
+int mxeuler_step(mxeuler_stepper* s, double cfl, int32_t has_dt, double dt, int32_t local, double residuals[5],
+                 double* time_step, int32_t* invalid_cell, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (has_dt && !(dt >= 0.0)) throw std::invalid_argument("time step must be non-negative");
+    int inv = -1;
+    s->impl->step(cfl, has_dt != 0, dt, local != 0, residuals, time_step, &inv);
+    *invalid_cell = inv;
+  });
+}
+
+int mxeuler_evaluate(mxeuler_stepper* s, double residuals[5], int32_t* invalid_cell, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    int inv = -1;
+    s->impl->evaluate(residuals, &inv);
+    *invalid_cell = inv;
+  });
+}
+
+int mxeuler_set_second_order(mxeuler_stepper* s, int32_t on, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->set_second_order(on != 0); });
+}
+
+int mxeuler_set_boundary_data(mxeuler_stepper* s, const double* bc, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (s->nb > 0 && bc == nullptr) throw std::invalid_argument("boundary data missing");
+    s->impl->set_boundary_data(bc);
+  });
+}
+
+int mxeuler_set_source(mxeuler_stepper* s, const double* source, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->set_source(source); });
+}
+
+int mxeuler_spectral_radius(mxeuler_stepper* s, double* lambda, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->spectral_radius(lambda); });
+}
+
+int mxeuler_arrays(mxeuler_stepper* s, double* prim, double* grad, double* phi, double* prim_b, char* err,
+                   size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->arrays(prim, grad, phi, prim_b); });
+}
+
+int mxeuler_profile(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local, double* ms_per_step,
+                    double* ms_per_kernel, int32_t* launches_per_step, int32_t max_kernels, int32_t* n_kernels,
+                    char* names, size_t names_len, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (steps <= 0) throw std::invalid_argument("steps must be positive");
+    std::string joined;
+    std::vector<int> launches(static_cast<std::size_t>(std::max(max_kernels, 0)));
+    const int n = s->impl->profile(steps, cfl, local != 0, ms_per_step, ms_per_kernel, launches.data(), max_kernels,
+                                   &joined);
+    for (int i = 0; i < n; ++i) launches_per_step[i] = launches[static_cast<std::size_t>(i)];
+    *n_kernels = n;
+    if (names != nullptr && names_len > 0) {
+      std::strncpy(names, joined.c_str(), names_len - 1);
+      names[names_len - 1] = '\0';
+    }
+  });
+}
+
+int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t n, double* out_sqrt,
+                       double* out_div, double* out_exp, double* out_log, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    using mxe::Buffer;
+    Buffer<double> dx(static_cast<std::size_t>(n)), dy(static_cast<std::size_t>(n)), s(static_cast<std::size_t>(n)),
+        q(static_cast<std::size_t>(n)), e(static_cast<std::size_t>(n)), l(static_cast<std::size_t>(n));
+    dx.upload(x, static_cast<std::size_t>(n), nullptr);
+    dy.upload(y, static_cast<std::size_t>(n), nullptr);
+    const int g = static_cast<int>((n + 255) / 256);
+    if (policy == 0) {
+      mxe::k_probe<double><<<g, 256>>>(dx.get(), dy.get(), n, s.get(), q.get(), e.get(), l.get());
+    } else {
+      mxe::k_probe<float><<<g, 256>>>(dx.get(), dy.get(), n, s.get(), q.get(), e.get(), l.get());
+    }
+    mxe::check(hipGetLastError(), "k_probe");
+    s.download(out_sqrt, static_cast<std::size_t>(n), nullptr);
+    q.download(out_div, static_cast<std::size_t>(n), nullptr);
+    e.download(out_exp, static_cast<std::size_t>(n), nullptr);
+    l.download(out_log, static_cast<std::size_t>(n), nullptr);
+    mxe::check(hipDeviceSynchronize(), "probe");
+  });
+}
+
+const char* mxeuler_version(void) { return "mxeuler 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,637 @@
+// Mesh assembly, generators, polyMesh reading and RCM renumbering.
+#include "mesh.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+namespace mx {
+
+namespace {
+
+std::size_t at(index_t i) { return static_cast<std::size_t>(i); }
+
+}  // namespace
+
+void finish(Mesh& m) {
+  const std::size_t nc = at(m.n_cells), nf = at(m.n_faces), ni = at(m.n_interior);
+  m.cell_face_offsets.assign(nc + 1, 0);
+  for (std::size_t f = 0; f < nf; ++f) {
+    ++m.cell_face_offsets[at(m.owner[f]) + 1];
+    if (f < ni) ++m.cell_face_offsets[at(m.neighbour[f]) + 1];
+  }
+  for (std::size_t c = 0; c < nc; ++c) m.cell_face_offsets[c + 1] += m.cell_face_offsets[c];
+  m.cell_faces.assign(static_cast<std::size_t>(m.cell_face_offsets[nc]), 0);
+  std::vector<offset_t> next(m.cell_face_offsets.begin(), m.cell_face_offsets.end() - 1);
+  // Faces in increasing order, so every cell's list comes out ascending.
+  for (std::size_t f = 0; f < nf; ++f) {
+    m.cell_faces[static_cast<std::size_t>(next[at(m.owner[f])]++)] = static_cast<index_t>(f);
+    if (f < ni) m.cell_faces[static_cast<std::size_t>(next[at(m.neighbour[f])]++)] = static_cast<index_t>(f);
+  }
+}
+
+Mesh assemble(index_t n_cells, std::vector<double> x, std::vector<double> y, std::vector<double> z,
+              RawFaces in) {
+  const std::size_t n = in.c0.size();
+  if (in.off.size() != n + 1 || in.c1.size() != n || in.zone.size() != n ||
+      (!in.shift.empty() && in.shift.size() != 3 * n)) {
+    throw std::invalid_argument("assemble: face arrays of different lengths");
+  }
+  const bool periodic = !in.shift.empty();
+  // Points moved into another cell's frame by a reversed periodic face: (point, shift) -> index.
+  std::map<std::tuple<index_t, double, double, double>, index_t> moved;
+  for (std::size_t i = 0; i < n; ++i) {
+    const index_t a = in.c0[i], b = in.c1[i];
+    if (a < 0 || a >= n_cells || b >= n_cells || a == b) {
+      throw std::invalid_argument("assemble: face " + std::to_string(i) + " has invalid cells");
+    }
+    if (b < 0 || a < b) continue;
+    std::swap(in.c0[i], in.c1[i]);
+    auto first = in.nodes.begin() + in.off[i], last = in.nodes.begin() + in.off[i + 1];
+    std::reverse(first, last);
+    if (periodic) {
+      double* s = &in.shift[3 * i];
+      if (s[0] != 0.0 || s[1] != 0.0 || s[2] != 0.0) {
+        // The nodes were in the old owner's frame; the new owner sees them moved by -shift.
+        for (auto it = first; it != last; ++it) {
+          const auto key = std::make_tuple(*it, s[0], s[1], s[2]);
+          auto found = moved.find(key);
+          if (found == moved.end()) {
+            const std::size_t p = at(*it);
+            x.push_back(x[p] - s[0]);
+            y.push_back(y[p] - s[1]);
+            z.push_back(z[p] - s[2]);
+            found = moved.emplace(key, static_cast<index_t>(x.size() - 1)).first;
+          }
+          *it = found->second;
+        }
+      }
+      for (int k = 0; k < 3; ++k) s[k] = -s[k];
+    }
+  }
+  std::vector<std::size_t> order(n);
+  std::iota(order.begin(), order.end(), std::size_t{0});
+  std::stable_sort(order.begin(), order.end(), [&](std::size_t p, std::size_t q) {
+    const bool bp = in.c1[p] < 0, bq = in.c1[q] < 0;
+    if (bp != bq) return bq;  // interior first
+    if (!bp) return in.c0[p] != in.c0[q] ? in.c0[p] < in.c0[q] : in.c1[p] < in.c1[q];
+    return in.zone[p] != in.zone[q] ? in.zone[p] < in.zone[q] : in.c0[p] < in.c0[q];
+  });
+  Mesh m;
+  m.n_cells = n_cells;
+  m.n_faces = static_cast<index_t>(n);
+  m.n_interior = static_cast<index_t>(std::count_if(in.c1.begin(), in.c1.end(), [](index_t b) { return b >= 0; }));
+  m.x = std::move(x);
+  m.y = std::move(y);
+  m.z = std::move(z);
+  m.owner.resize(n);
+  m.neighbour.resize(at(m.n_interior));
+  m.face_zone.resize(n - at(m.n_interior));
+  m.face_node_offsets.assign(n + 1, 0);
+  m.face_nodes.reserve(in.nodes.size());
+  bool any_shift = false;
+  if (periodic) {
+    for (double s : in.shift) any_shift = any_shift || s != 0.0;
+  }
+  if (any_shift) m.shift.assign(3 * at(m.n_interior), 0.0);
+  const std::size_t ni = at(m.n_interior);
+  for (std::size_t k = 0; k < n; ++k) {
+    const std::size_t i = order[k];
+    m.owner[k] = in.c0[i];
+    if (k < ni) {
+      m.neighbour[k] = in.c1[i];
+      if (any_shift) {
+        for (std::size_t u = 0; u < 3; ++u) m.shift[u * ni + k] = in.shift[3 * i + u];
+      }
+    } else {
+      m.face_zone[k - ni] = in.zone[i];
+    }
+    m.face_nodes.insert(m.face_nodes.end(), in.nodes.begin() + in.off[i], in.nodes.begin() + in.off[i + 1]);
+    m.face_node_offsets[k + 1] = static_cast<offset_t>(m.face_nodes.size());
+  }
+  finish(m);
+  return m;
+}
+
+// Generators ------------------------------------------------------------------------------------
+
+Mesh periodic_hex(int n, double lo, double hi) {
+  if (n < 3) throw std::invalid_argument("periodic_hex needs n >= 3");
+  const double len = hi - lo;
+  const int p = n + 1;
+  std::vector<double> x, y, z;
+  x.reserve(static_cast<std::size_t>(p) * p * p);
+  for (int k = 0; k < p; ++k) {
+    for (int j = 0; j < p; ++j) {
+      for (int i = 0; i < p; ++i) {
+        x.push_back(lo + len * i / n);
+        y.push_back(lo + len * j / n);
+        z.push_back(lo + len * k / n);
+      }
+    }
+  }
+  y.reserve(x.size());
+  z.reserve(x.size());
+  auto node = [&](int i, int j, int k) { return static_cast<index_t>(i + p * (j + p * k)); };
+  auto cell = [&](int i, int j, int k) { return static_cast<index_t>((i % n) + n * ((j % n) + n * (k % n))); };
+  RawFaces f;
+  const std::size_t nf = 3 * static_cast<std::size_t>(n) * n * n;
+  f.off.reserve(nf + 1);
+  f.nodes.reserve(4 * nf);
+  f.c0.reserve(nf);
+  f.c1.reserve(nf);
+  f.zone.reserve(nf);
+  f.shift.reserve(3 * nf);
+  auto push_shift = [&](double sx, double sy, double sz) {
+    f.shift.push_back(sx);
+    f.shift.push_back(sy);
+    f.shift.push_back(sz);
+  };
+  for (int k = 0; k < n; ++k) {
+    for (int j = 0; j < n; ++j) {
+      for (int i = 0; i < n; ++i) {
+        const index_t c = cell(i, j, k);
+        // +x face, normal +x: (j,k) (j+1,k) (j+1,k+1) (j,k+1) in the y-z plane.
+        f.add({node(i + 1, j, k), node(i + 1, j + 1, k), node(i + 1, j + 1, k + 1), node(i + 1, j, k + 1)}, c,
+              cell(i + 1, j, k));
+        push_shift(i == n - 1 ? len : 0.0, 0.0, 0.0);
+        // +y face, normal +y: (k,i) order: (i,k) (i,k+1) (i+1,k+1) (i+1,k).
+        f.add({node(i, j + 1, k), node(i, j + 1, k + 1), node(i + 1, j + 1, k + 1), node(i + 1, j + 1, k)}, c,
+              cell(i, j + 1, k));
+        push_shift(0.0, j == n - 1 ? len : 0.0, 0.0);
+        // +z face, normal +z: (i,j) (i+1,j) (i+1,j+1) (i,j+1).
+        f.add({node(i, j, k + 1), node(i + 1, j, k + 1), node(i + 1, j + 1, k + 1), node(i, j + 1, k + 1)}, c,
+              cell(i, j, k + 1));
+        push_shift(0.0, 0.0, k == n - 1 ? len : 0.0);
+      }
+    }
+  }
+  return assemble(static_cast<index_t>(n) * n * n, std::move(x), std::move(y), std::move(z), std::move(f));
+}
+
+namespace {
+
+// splitmix64: a small seeded generator, fixed so that meshes are reproducible everywhere.
+struct SplitMix {
+  std::uint64_t s;
+  std::uint64_t next() {
+    std::uint64_t v = (s += 0x9e3779b97f4a7c15ULL);
+    v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+    return v ^ (v >> 31);
+  }
+  // Uniform in [-1, 1).
+  double symmetric() { return static_cast<double>(next() >> 11) * 0x1.0p-52 - 1.0; }
+};
+
+// Faces of cells given as outward-oriented polygons, matched by their three smallest node ids.
+struct CellFaces {
+  std::vector<offset_t> off{0};
+  std::vector<index_t> nodes;
+  std::vector<index_t> cell;
+  void add(index_t c, std::initializer_list<index_t> n) {
+    nodes.insert(nodes.end(), n);
+    off.push_back(static_cast<offset_t>(nodes.size()));
+    cell.push_back(c);
+  }
+};
+
+// Boundary zone of an unmatched face, from its nodes.
+template <class ZoneOf>
+RawFaces match_faces(const CellFaces& cf, const ZoneOf& zone_of) {
+  const std::size_t n = cf.cell.size();
+  struct Key {
+    index_t a, b, c;
+    std::uint32_t i;
+  };
+  std::vector<Key> keys(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    std::array<index_t, 3> s{INT32_MAX, INT32_MAX, INT32_MAX};
+    for (auto k = cf.off[i]; k < cf.off[i + 1]; ++k) {
+      index_t v = cf.nodes[static_cast<std::size_t>(k)];
+      for (int q = 0; q < 3; ++q) {
+        if (v < s[q]) std::swap(v, s[q]);
+      }
+    }
+    keys[i] = {s[0], s[1], s[2], static_cast<std::uint32_t>(i)};
+  }
+  std::sort(keys.begin(), keys.end(), [](const Key& p, const Key& q) {
+    if (p.a != q.a) return p.a < q.a;
+    if (p.b != q.b) return p.b < q.b;
+    if (p.c != q.c) return p.c < q.c;
+    return p.i < q.i;
+  });
+  RawFaces out;
+  out.off.reserve(n / 2 + 16);
+  for (std::size_t k = 0; k < n;) {
+    const Key& p = keys[k];
+    const std::size_t i = p.i;
+    const bool pair = k + 1 < n && keys[k + 1].a == p.a && keys[k + 1].b == p.b && keys[k + 1].c == p.c;
+    if (pair && k + 2 < n && keys[k + 2].a == p.a && keys[k + 2].b == p.b && keys[k + 2].c == p.c) {
+      throw std::runtime_error("match_faces: a face shared by more than two cells");
+    }
+    out.nodes.insert(out.nodes.end(), cf.nodes.begin() + cf.off[i], cf.nodes.begin() + cf.off[i + 1]);
+    out.off.push_back(static_cast<offset_t>(out.nodes.size()));
+    out.c0.push_back(cf.cell[i]);
+    if (pair) {
+      out.c1.push_back(cf.cell[keys[k + 1].i]);
+      out.zone.push_back(-1);
+      k += 2;
+    } else {
+      out.c1.push_back(-1);
+      out.zone.push_back(zone_of(&cf.nodes[static_cast<std::size_t>(cf.off[i])],
+                                 static_cast<std::size_t>(cf.off[i + 1] - cf.off[i])));
+      k += 1;
+    }
+  }
+  return out;
+}
+
+}  // namespace
+
+Mesh kuhn_tets(int n, double jitter, std::uint64_t seed) {
+  if (n < 1) throw std::invalid_argument("kuhn_tets needs n >= 1");
+  const int p = n + 1;
+  const double h = 1.0 / n;
+  const std::size_t np = static_cast<std::size_t>(p) * p * p;
+  std::vector<double> x(np), y(np), z(np);
+  SplitMix rng{seed};
+  for (int k = 0; k < p; ++k) {
+    for (int j = 0; j < p; ++j) {
+      for (int i = 0; i < p; ++i) {
+        const std::size_t v = static_cast<std::size_t>(i + p * (j + p * k));
+        // Three draws per node in node order, used only on interior nodes, so a boundary node
+        // stays on its plane whatever the seed.
+        const double dx = rng.symmetric(), dy = rng.symmetric(), dz = rng.symmetric();
+        const bool inside = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+        x[v] = h * i + (inside ? jitter * h * dx : 0.0);
+        y[v] = h * j + (inside ? jitter * h * dy : 0.0);
+        z[v] = h * k + (inside ? jitter * h * dz : 0.0);
+      }
+    }
+  }
+  auto node = [&](int i, int j, int k) { return static_cast<index_t>(i + p * (j + p * k)); };
+  CellFaces cf;
+  const std::size_t nt = 6 * static_cast<std::size_t>(n) * n * n;
+  cf.off.reserve(4 * nt + 1);
+  cf.nodes.reserve(12 * nt);
+  cf.cell.reserve(4 * nt);
+  // The six tetrahedra about the diagonal v000 - v111: for each order (a, b, c) of the axes,
+  // {v000, v000 + e_a, v000 + e_a + e_b, v111}.
+  static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  index_t c = 0;
+  for (int k = 0; k < n; ++k) {
+    for (int j = 0; j < n; ++j) {
+      for (int i = 0; i < n; ++i) {
+        for (const auto& pm : perms) {
+          int q1[3] = {i, j, k};
+          q1[pm[0]] += 1;
+          int q2[3] = {q1[0], q1[1], q1[2]};
+          q2[pm[1]] += 1;
+          index_t t[4] = {node(i, j, k), node(q1[0], q1[1], q1[2]), node(q2[0], q2[1], q2[2]),
+                          node(i + 1, j + 1, k + 1)};
+          // Orient so that (t1 - t0) x (t2 - t0) . (t3 - t0) > 0 on the unjittered cube: the
+          // parity of the axis permutation decides it.
+          const bool even = (pm[0] == 0 && pm[1] == 1) || (pm[0] == 1 && pm[1] == 2) || (pm[0] == 2 && pm[1] == 0);
+          if (!even) std::swap(t[1], t[2]);
+          // Outward faces of a positive tetrahedron (0,1,2,3): (0,2,1), (0,1,3), (1,2,3), (0,3,2).
+          cf.add(c, {t[0], t[2], t[1]});
+          cf.add(c, {t[0], t[1], t[3]});
+          cf.add(c, {t[1], t[2], t[3]});
+          cf.add(c, {t[0], t[3], t[2]});
+          ++c;
+        }
+      }
+    }
+  }
+  const double eps = 0.25 * h;
+  auto zone_of = [&](const index_t* v, std::size_t count) {
+    auto all = [&](auto pred) {
+      for (std::size_t q = 0; q < count; ++q) {
+        if (!pred(at(v[q]))) return false;
+      }
+      return true;
+    };
+    if (all([&](std::size_t w) { return x[w] < eps; })) return 0;
+    if (all([&](std::size_t w) { return x[w] > 1.0 - eps; })) return 1;
+    if (all([&](std::size_t w) { return y[w] < eps; })) return 2;
+    if (all([&](std::size_t w) { return y[w] > 1.0 - eps; })) return 3;
+    if (all([&](std::size_t w) { return z[w] < eps; })) return 4;
+    return 5;
+  };
+  RawFaces faces = match_faces(cf, zone_of);
+  cf = CellFaces();
+  return assemble(c, std::move(x), std::move(y), std::move(z), std::move(faces));
+}
+
+Mesh bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0) {
+  if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("bump_hex needs positive counts");
+  const int px = nx + 1, py = ny + 1, pz = nz + 1;
+  const std::size_t np = static_cast<std::size_t>(px) * py * pz;
+  std::vector<double> x(np), y(np), z(np);
+  for (int k = 0; k < pz; ++k) {
+    for (int j = 0; j < py; ++j) {
+      for (int i = 0; i < px; ++i) {
+        const std::size_t v = static_cast<std::size_t>(i + px * (j + py * k));
+        const double xi = lx * i / nx;
+        const double t = (xi - x0) / w;
+        const double wall = h * std::exp(-t * t);
+        x[v] = xi;
+        y[v] = wall + (ly - wall) * j / ny;
+        z[v] = lz * k / nz;
+      }
+    }
+  }
+  auto node = [&](int i, int j, int k) { return static_cast<index_t>(i + px * (j + py * k)); };
+  auto cell = [&](int i, int j, int k) { return static_cast<index_t>(i + nx * (j + ny * k)); };
+  RawFaces f;
+  for (int k = 0; k < nz; ++k) {
+    for (int j = 0; j < ny; ++j) {
+      for (int i = 0; i < nx; ++i) {
+        const index_t c = cell(i, j, k);
+        const index_t xp = i + 1 < nx ? cell(i + 1, j, k) : -1;
+        const index_t yp = j + 1 < ny ? cell(i, j + 1, k) : -1;
+        const index_t zp = k + 1 < nz ? cell(i, j, k + 1) : -1;
+        f.add({node(i + 1, j, k), node(i + 1, j + 1, k), node(i + 1, j + 1, k + 1), node(i + 1, j, k + 1)}, c, xp,
+              xp < 0 ? 1 : -1);
+        f.add({node(i, j + 1, k), node(i, j + 1, k + 1), node(i + 1, j + 1, k + 1), node(i + 1, j + 1, k)}, c, yp,
+              yp < 0 ? 3 : -1);
+        f.add({node(i, j, k + 1), node(i + 1, j, k + 1), node(i + 1, j + 1, k + 1), node(i, j + 1, k + 1)}, c, zp,
+              zp < 0 ? 5 : -1);
+        if (i == 0) {
+          f.add({node(0, j, k), node(0, j, k + 1), node(0, j + 1, k + 1), node(0, j + 1, k)}, c, -1, 0);
+        }
+        if (j == 0) {
+          f.add({node(i, 0, k), node(i + 1, 0, k), node(i + 1, 0, k + 1), node(i, 0, k + 1)}, c, -1, 2);
+        }
+        if (k == 0) {
+          f.add({node(i, j, 0), node(i, j + 1, 0), node(i + 1, j + 1, 0), node(i + 1, j, 0)}, c, -1, 4);
+        }
+      }
+    }
+  }
+  return assemble(static_cast<index_t>(nx) * ny * nz, std::move(x), std::move(y), std::move(z), std::move(f));
+}
+
+// polyMesh ---------------------------------------------------------------------------------------
+
+namespace {
+
+std::string slurp(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("cannot open " + path);
+  std::ostringstream s;
+  s << in.rdbuf();
+  return s.str();
+}
+
+// The text after the FoamFile header and comments, from the list count on.
+std::size_t list_start(const std::string& s) {
+  std::size_t p = s.find("FoamFile");
+  p = p == std::string::npos ? 0 : s.find('}', p) + 1;
+  while (p < s.size()) {
+    if (s.compare(p, 2, "//") == 0) {
+      p = s.find('\n', p);
+    } else if (s.compare(p, 2, "/*") == 0) {
+      p = s.find("*/", p) + 2;
+    } else if (std::isdigit(static_cast<unsigned char>(s[p]))) {
+      return p;
+    } else {
+      ++p;
+    }
+  }
+  throw std::runtime_error("polyMesh: no list found");
+}
+
+std::vector<long> labels(const std::string& path) {
+  const std::string s = slurp(path);
+  std::size_t p = list_start(s);
+  char* end = nullptr;
+  const long n = std::strtol(s.c_str() + p, &end, 10);
+  const char* q = std::strchr(end, '(') + 1;
+  std::vector<long> out(static_cast<std::size_t>(n));
+  for (long i = 0; i < n; ++i) out[static_cast<std::size_t>(i)] = std::strtol(q, const_cast<char**>(&q), 10);
+  return out;
+}
+
+}  // namespace
+
+Mesh read_polymesh(const std::string& dir) {
+  // points
+  std::vector<double> x, y, z;
+  {
+    const std::string s = slurp(dir + "/points");
+    std::size_t p = list_start(s);
+    char* end = nullptr;
+    const long n = std::strtol(s.c_str() + p, &end, 10);
+    const char* q = std::strchr(end, '(') + 1;
+    for (long i = 0; i < n; ++i) {
+      q = std::strchr(q, '(') + 1;
+      x.push_back(std::strtod(q, const_cast<char**>(&q)));
+      y.push_back(std::strtod(q, const_cast<char**>(&q)));
+      z.push_back(std::strtod(q, const_cast<char**>(&q)));
+      q = std::strchr(q, ')') + 1;
+    }
+  }
+  RawFaces f;
+  {
+    const std::string s = slurp(dir + "/faces");
+    std::size_t p = list_start(s);
+    char* end = nullptr;
+    const long n = std::strtol(s.c_str() + p, &end, 10);
+    const char* q = std::strchr(end, '(') + 1;
+    for (long i = 0; i < n; ++i) {
+      const long k = std::strtol(q, const_cast<char**>(&q), 10);
+      q = std::strchr(q, '(') + 1;
+      for (long j = 0; j < k; ++j) f.nodes.push_back(static_cast<index_t>(std::strtol(q, const_cast<char**>(&q), 10)));
+      q = std::strchr(q, ')') + 1;
+      f.off.push_back(static_cast<offset_t>(f.nodes.size()));
+    }
+  }
+  const auto own = labels(dir + "/owner");
+  const auto nei = labels(dir + "/neighbour");
+  const std::size_t nf = f.off.size() - 1;
+  if (own.size() != nf) throw std::runtime_error("polyMesh: owner and faces differ in length");
+  long n_cells = 0;
+  for (long c : own) n_cells = std::max(n_cells, c + 1);
+  for (long c : nei) n_cells = std::max(n_cells, c + 1);
+  // boundary: patches in order, each nFaces from startFace.
+  std::vector<index_t> zone(nf, -1);
+  {
+    const std::string s = slurp(dir + "/boundary");
+    std::size_t p = 0;
+    index_t patch = 0;
+    while ((p = s.find("nFaces", p)) != std::string::npos) {
+      const long count = std::strtol(s.c_str() + p + 6, nullptr, 10);
+      const std::size_t q = s.find("startFace", p);
+      const long start = std::strtol(s.c_str() + q + 9, nullptr, 10);
+      for (long i = start; i < start + count; ++i) zone[static_cast<std::size_t>(i)] = patch;
+      ++patch;
+      p = q;
+    }
+  }
+  for (std::size_t i = 0; i < nf; ++i) {
+    f.c0.push_back(static_cast<index_t>(own[i]));
+    f.c1.push_back(i < nei.size() ? static_cast<index_t>(nei[i]) : -1);
+    f.zone.push_back(i < nei.size() ? -1 : zone[i]);
+  }
+  return assemble(static_cast<index_t>(n_cells), std::move(x), std::move(y), std::move(z), std::move(f));
+}
+
+// Renumbering ------------------------------------------------------------------------------------
+
+namespace {
+
+struct Graph {
+  std::vector<offset_t> off;
+  std::vector<index_t> adj;
+  [[nodiscard]] int degree(index_t c) const { return static_cast<int>(off[at(c) + 1] - off[at(c)]); }
+};
+
+Graph cell_graph(const Mesh& m) {
+  const std::size_t nc = at(m.n_cells);
+  Graph g;
+  g.off.assign(nc + 1, 0);
+  for (index_t f = 0; f < m.n_interior; ++f) {
+    ++g.off[at(m.owner[at(f)]) + 1];
+    ++g.off[at(m.neighbour[at(f)]) + 1];
+  }
+  for (std::size_t c = 0; c < nc; ++c) g.off[c + 1] += g.off[c];
+  g.adj.resize(static_cast<std::size_t>(g.off[nc]));
+  std::vector<offset_t> next(g.off.begin(), g.off.end() - 1);
+  for (index_t f = 0; f < m.n_interior; ++f) {
+    const index_t a = m.owner[at(f)], b = m.neighbour[at(f)];
+    g.adj[static_cast<std::size_t>(next[at(a)]++)] = b;
+    g.adj[static_cast<std::size_t>(next[at(b)]++)] = a;
+  }
+  // Two faces between the same cells count once.
+  std::vector<offset_t> off2(nc + 1, 0);
+  std::size_t w = 0;
+  for (std::size_t c = 0; c < nc; ++c) {
+    auto first = g.adj.begin() + g.off[c], last = g.adj.begin() + g.off[c + 1];
+    std::sort(first, last);
+    const auto uend = std::unique(first, last);
+    for (auto it = first; it != uend; ++it) g.adj[w++] = *it;
+    off2[c + 1] = static_cast<offset_t>(w);
+  }
+  g.adj.resize(w);
+  g.off.swap(off2);
+  return g;
+}
+
+// Breadth-first levels from root: the last level's cells and the depth.
+int bfs(const Graph& g, index_t root, std::vector<int>& level, std::vector<index_t>& last) {
+  std::vector<index_t> queue{root};
+  level[at(root)] = 0;
+  int depth = 0;
+  for (std::size_t h = 0; h < queue.size(); ++h) {
+    const index_t c = queue[h];
+    for (auto k = g.off[at(c)]; k < g.off[at(c) + 1]; ++k) {
+      const index_t d = g.adj[static_cast<std::size_t>(k)];
+      if (level[at(d)] < 0) {
+        level[at(d)] = level[at(c)] + 1;
+        depth = std::max(depth, level[at(d)]);
+        queue.push_back(d);
+      }
+    }
+  }
+  last.clear();
+  for (const index_t c : queue) {
+    if (level[at(c)] == depth) last.push_back(c);
+  }
+  for (const index_t c : queue) level[at(c)] = -1;
+  return depth;
+}
+
+// George and Liu (1979): move the root to a least-degree cell of the last level while the depth grows.
+index_t pseudo_peripheral(const Graph& g, index_t seed, std::vector<int>& level) {
+  std::vector<index_t> last;
+  index_t root = seed;
+  int depth = bfs(g, root, level, last);
+  for (;;) {
+    index_t best = last.front();
+    for (const index_t c : last) {
+      if (g.degree(c) < g.degree(best) || (g.degree(c) == g.degree(best) && c < best)) best = c;
+    }
+    std::vector<index_t> cand_last;
+    const int d = bfs(g, best, level, cand_last);
+    if (d <= depth) return root;
+    root = best;
+    depth = d;
+    last.swap(cand_last);
+  }
+}
+
+}  // namespace
+
+std::vector<index_t> rcm_order(const Mesh& m) {
+  const Graph g = cell_graph(m);
+  const std::size_t nc = at(m.n_cells);
+  std::vector<int> level(nc, -1);
+  std::vector<char> placed(nc, 0);
+  std::vector<index_t> order;
+  order.reserve(nc);
+  std::vector<index_t> by_degree(nc);
+  std::iota(by_degree.begin(), by_degree.end(), 0);
+  std::stable_sort(by_degree.begin(), by_degree.end(),
+                   [&](index_t a, index_t b) { return g.degree(a) < g.degree(b); });
+  std::vector<index_t> nb;
+  for (const index_t seed : by_degree) {
+    if (placed[at(seed)] != 0) continue;
+    const index_t root = pseudo_peripheral(g, seed, level);
+    std::size_t head = order.size();
+    order.push_back(root);
+    placed[at(root)] = 1;
+    while (head < order.size()) {
+      const index_t c = order[head++];
+      nb.clear();
+      for (auto k = g.off[at(c)]; k < g.off[at(c) + 1]; ++k) {
+        const index_t d = g.adj[static_cast<std::size_t>(k)];
+        if (placed[at(d)] == 0) nb.push_back(d);
+      }
+      std::sort(nb.begin(), nb.end(), [&](index_t a, index_t b) {
+        return g.degree(a) != g.degree(b) ? g.degree(a) < g.degree(b) : a < b;
+      });
+      for (const index_t d : nb) {
+        placed[at(d)] = 1;
+        order.push_back(d);
+      }
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
+
+Mesh renumber(const Mesh& m, const std::vector<index_t>& new_to_old) {
+  const std::size_t nc = at(m.n_cells), nf = at(m.n_faces), ni = at(m.n_interior);
+  if (new_to_old.size() != nc) throw std::invalid_argument("renumber: permutation size");
+  std::vector<index_t> old_to_new(nc, -1);
+  for (std::size_t i = 0; i < nc; ++i) {
+    const index_t o = new_to_old[i];
+    if (o < 0 || at(o) >= nc || old_to_new[at(o)] != -1) throw std::invalid_argument("renumber: not a permutation");
+    old_to_new[at(o)] = static_cast<index_t>(i);
+  }
+  RawFaces f;
+  f.off = m.face_node_offsets;
+  f.nodes = m.face_nodes;
+  f.c0.resize(nf);
+  f.c1.resize(nf);
+  f.zone.resize(nf);
+  if (m.periodic()) f.shift.resize(3 * nf, 0.0);
+  for (std::size_t i = 0; i < nf; ++i) {
+    f.c0[i] = old_to_new[at(m.owner[i])];
+    f.c1[i] = i < ni ? old_to_new[at(m.neighbour[i])] : -1;
+    f.zone[i] = i < ni ? -1 : m.face_zone[i - ni];
+    if (m.periodic() && i < ni) {
+      for (std::size_t u = 0; u < 3; ++u) f.shift[3 * i + u] = m.shift[u * ni + i];
+    }
+  }
+  return assemble(m.n_cells, m.x, m.y, m.z, std::move(f));
+}
+
+}  // namespace mx

@@ -1,0 +1,150 @@
+/* The explicit compressible-Euler residual loop on one MI355X (gfx950), behind a plain C ABI.
+ *
+ * A drop-in for the reference's device stepper: make_device_stepper<P>
+ * (physics/include/autan/physics/euler_device.hpp:49-54, physics/src/euler_device.cu:8-23) and the
+ * DeviceStepper<P> methods step / evaluate / residual / set_second_order / upload / download
+ * (physics/include/autan/physics/euler_device.hpp:22-41), implemented by CudaStepper<P> in
+ * physics/src/device_stepper.cuh:200-529. Where the reference's stepper reads the host Euler<P>
+ * (data(), stencil(), gradient_weights(), scheme(), planar(); device_stepper.cuh:209-311), this
+ * ABI takes the same arrays: mesh topology, packed FP64 geometry, the gradient stencil with its
+ * weights, boundary codes and data, and the volume source. Conserved states and residuals cross
+ * the boundary as FP64 structure-of-arrays, component k of cell c at k * n_cells + c.
+ *
+ * Conventions (the reference's C ABI precedent, physics/mms/capi.cpp:126-347): every call
+ * returning int returns 0 on success, 1 on error with a message in err[0 .. err_len) that starts
+ * with the reference's error code where one exists (E1101 for a device failure). Handles are
+ * opaque. Calls are synchronous: each returns once its results are on the host. One handle is
+ * driven by one host thread.
+ */
+#ifndef MXEULER_H
+#define MXEULER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision policies (codegen/include/autan/kernels/runtime.hpp:73-99). */
+enum { MXEULER_FP64 = 0, MXEULER_FP32_ACC64 = 1 };
+/* Numerical fluxes (physics/include/autan/physics/euler.hpp:27). */
+enum { MXEULER_ROE = 0, MXEULER_HLLC = 1, MXEULER_AUSM_UP = 2 };
+
+/* Scheme and Gas (physics/include/autan/physics/euler.hpp:39-71): p_ref and the limiter scales
+ * (K / length)^3 scale_k^2 are derived from these as physics/src/euler.cpp:238-245 derives them. */
+typedef struct {
+  int32_t flux;         /* MXEULER_ROE, MXEULER_HLLC or MXEULER_AUSM_UP */
+  int32_t second_order; /* linear reconstruction with least-squares gradients */
+  int32_t limiter;      /* Venkatakrishnan limiter */
+  int32_t planar;       /* the loops without warped-face terms; refused unless all_planar */
+  double gamma;
+  double gas_constant;
+  double mach_ref;  /* AUSM+-up cut-off Mach number, > 0 */
+  double venkat_k;
+  double length;
+  double scale[5];     /* W units: rho, u, v, w, p */
+  double reference[5]; /* U_ref (the freestream), reference[0] > 0 */
+} mxeuler_scheme;
+
+/* Mesh topology in the reference's canonical order (core/include/autan/core/mesh.hpp:277-282):
+ * interior faces first with owner < neighbour, each cell's faces ascending in cell_faces. */
+typedef struct {
+  int32_t n_cells;
+  int32_t n_faces;
+  int32_t n_interior;
+  const int32_t* owner;             /* n_faces */
+  const int32_t* neighbour;         /* n_interior */
+  const int64_t* cell_face_offsets; /* n_cells + 1 */
+  const int32_t* cell_faces;        /* cell_face_offsets[n_cells] */
+} mxeuler_topology;
+
+/* Packed geometry (kernels::PackedGeometry, codegen/include/autan/kernels/geometry.hpp): FP64,
+ * component-major. r_left / r_right are x_f - x_c of the owner / neighbour in FP64 (the operands
+ * of the kernels' rel(x_f, x_c)); on a periodic face r_right uses the neighbour's image. */
+typedef struct {
+  const double* face_normal; /* 3 x n_faces, unit, owner -> neighbour */
+  const double* face_area;   /* n_faces */
+  const double* r_left;      /* 3 x n_faces */
+  const double* r_right;     /* 3 x n_faces (interior faces read) */
+  const double* face_moment; /* 9 x n_faces; may be NULL when every face is proven planar */
+  const double* cell_volume; /* n_cells */
+  int32_t all_planar;        /* every face moment is an exact zero */
+} mxeuler_geometry;
+
+/* The gradient stencil (kernels::CellStencil): entries of cell c at offsets[c] .. offsets[c+1],
+ * a cell index or -(f + 1) for boundary face f; weights 3 x n_entries (Euler::gradient_weights). */
+typedef struct {
+  const int64_t* offsets;
+  const int32_t* entries;
+  const double* weights;
+} mxeuler_stencil;
+
+typedef struct mxeuler_stepper mxeuler_stepper;
+
+/* make_device_stepper: uploads everything; bc_kind holds one code (BoundaryKind 0..5, whole
+ * numbers as FP64) and bc five values (5 x n_boundary) per boundary face; source is 5 x n_cells
+ * per unit volume, or NULL for zero. stream is a hipStream_t to run on, or NULL for the
+ * library's own stream. second_order and planar are those of the scheme. */
+int mxeuler_create(int32_t policy, const mxeuler_scheme* scheme, const mxeuler_topology* topology,
+                   const mxeuler_geometry* geometry, const mxeuler_stencil* stencil, const double* bc_kind,
+                   const double* bc, const double* source, void* stream, mxeuler_stepper** out, char* err,
+                   size_t err_len);
+void mxeuler_destroy(mxeuler_stepper* s);
+
+/* DeviceStepper::upload / download: the FP64 conserved state u (5 x n_cells). */
+int mxeuler_upload(mxeuler_stepper* s, const double* u, char* err, size_t err_len);
+int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len);
+
+/* DeviceStepper::residual: R(u) into res (both 5 x n_cells, FP64); the stepper's state is kept. */
+int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len);
+
+/* DeviceStepper::step: one SSP-RK3 step of the uploaded state. residuals receives the RMS norms
+ * of R(u) at the start of the step (sqrt(sum_c R_kc^2 / N), summed exactly); time_step the step
+ * used: dt when has_dt, else min_c CFL V_c / lambda_c (with local, each cell its own).
+ * invalid_cell receives the least cell whose new state has a non-finite or non-positive density
+ * or pressure, else the least with a non-finite residual, else -1. */
+int mxeuler_step(mxeuler_stepper* s, double cfl, int32_t has_dt, double dt, int32_t local, double residuals[5],
+                 double* time_step, int32_t* invalid_cell, char* err, size_t err_len);
+
+/* DeviceStepper::evaluate: R(u) of the uploaded state, its norms and validity. */
+int mxeuler_evaluate(mxeuler_stepper* s, double residuals[5], int32_t* invalid_cell, char* err, size_t err_len);
+
+/* DeviceStepper::set_second_order: gradients and limiter zero without reconstruction, limiter 1
+ * when second order is unlimited. */
+int mxeuler_set_second_order(mxeuler_stepper* s, int32_t on, char* err, size_t err_len);
+
+/* Euler::set_boundary_data and Euler::set_source (physics/include/autan/physics/euler.hpp:127,
+ * 217): new boundary data (5 x n_boundary) and volume source (5 x n_cells, or NULL for zero). */
+int mxeuler_set_boundary_data(mxeuler_stepper* s, const double* bc, char* err, size_t err_len);
+int mxeuler_set_source(mxeuler_stepper* s, const double* source, char* err, size_t err_len);
+
+/* Euler::spectral_radius after a residual (n_cells, widened to FP64). */
+int mxeuler_spectral_radius(mxeuler_stepper* s, double* lambda, char* err, size_t err_len);
+
+/* The arrays of the last residual, widened to FP64, for tests: prim (5 x n_cells, p gauge),
+ * grad (15 x n_cells, row (3 i + j)), phi (5 x n_cells), prim_b (5 x n_boundary). Any may be NULL. */
+int mxeuler_arrays(mxeuler_stepper* s, double* prim, double* grad, double* phi, double* prim_b, char* err,
+                   size_t err_len);
+
+/* Timing of the device work of `steps` steps (no host transfers between them): the average
+ * milliseconds per step and per launch of each kernel, measured with HIP events on the stepper's
+ * stream. names receives a ';'-separated list of the kernel names in the order of ms_per_kernel
+ * (up to max_kernels). Returns the number of kernels through *n_kernels. */
+int mxeuler_profile(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local, double* ms_per_step,
+                    double* ms_per_kernel, int32_t* launches_per_step, int32_t max_kernels, int32_t* n_kernels,
+                    char* names, size_t names_len, char* err, size_t err_len);
+
+/* Device math probe for the tests: sqrt, division, exp and log of x (and x / y) on the device in
+ * FP64 (policy 0) or FP32 (policy 1), n values each, widened to FP64. */
+int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t n, double* out_sqrt,
+                       double* out_div, double* out_exp, double* out_log, char* err, size_t err_len);
+
+/* Version string of the library. */
+const char* mxeuler_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,227 @@
+"""ctypes wrapper of the CPU oracle (oracle/euler_oracle.cpp): test infrastructure only.
+
+`Oracle(mesh, scheme, policy, ...)` takes the same arrays as the device stepper and evaluates the
+reference's residual, SSP-RK3 step, norms and validity on the host in one thread.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import functools
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+
+_d = ctypes.c_double
+_i32 = ctypes.c_int32
+_p = ctypes.c_void_p
+
+
+class _Scheme(ctypes.Structure):
+    _fields_ = [
+        ("flux", ctypes.c_int),
+        ("second_order", ctypes.c_int),
+        ("limiter", ctypes.c_int),
+        ("planar", ctypes.c_int),
+        ("gamma", _d),
+        ("gas_constant", _d),
+        ("mach_ref", _d),
+        ("venkat_k", _d),
+        ("length", _d),
+        ("scale", _d * 5),
+        ("reference", _d * 5),
+    ]
+
+
+class _Inputs(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", _i32),
+        ("n_faces", _i32),
+        ("n_interior", _i32),
+        ("owner", _p),
+        ("neighbour", _p),
+        ("cell_face_offsets", _p),
+        ("cell_faces", _p),
+        ("face_normal", _p),
+        ("face_area", _p),
+        ("r_left", _p),
+        ("r_right", _p),
+        ("face_moment", _p),
+        ("cell_volume", _p),
+        ("stencil_offsets", _p),
+        ("stencil", _p),
+        ("stencil_weights", _p),
+        ("bc_kind", _p),
+        ("bc", _p),
+        ("source", _p),
+    ]
+
+
+@functools.cache
+def lib() -> ctypes.CDLL:
+    if not LIB.exists():
+        raise RuntimeError(f"{LIB} is not built: run autan_amd/build.py oracle")
+    L = ctypes.CDLL(str(LIB))
+    L.oracle_exact_sum.restype = _d
+    L.oracle_exact_sum.argtypes = [_p, ctypes.c_int64]
+    L.oracle_venkat.restype = _d
+    L.oracle_venkat.argtypes = [_d, _d, _d, _d]
+    L.oracle_destroy.argtypes = [_p]
+    L.oracle_residual.argtypes = [_p, _p, _p, ctypes.c_int]
+    L.oracle_spectral_radius.argtypes = [_p, _p]
+    L.oracle_face_fluxes.argtypes = [_p, _p]
+    L.oracle_arrays.argtypes = [_p] * 7
+    L.oracle_set_second_order.argtypes = [_p, ctypes.c_int]
+    L.oracle_first_invalid.argtypes = [_p, _p, _p]
+    L.oracle_step.argtypes = [_p, _p, _d, ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _p, _p, _p]
+    L.oracle_flux.argtypes = [ctypes.c_int, ctypes.c_int, _p, _p, _p, _d, _d, _d, _p]
+    L.oracle_physical_flux.argtypes = [_p, _p, _d, _d, _p]
+    L.oracle_boundary_value.argtypes = [_p, _d, _p, _p, _d, _d, _d, _p]
+    L.oracle_to_primitive.argtypes = [_p, _p, _d, _p]
+    L.oracle_warp.argtypes = [_p] * 6 + [_d, _d, _p]
+    return L
+
+
+def _a(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def exact_sum(x: np.ndarray) -> float:
+    x = _a(x).reshape(-1)
+    return lib().oracle_exact_sum(x.ctypes.data, x.size)
+
+
+def flux(kind: int, wl, wr, n, gamma=1.4, minf=0.5, pref=0.0, policy=0) -> np.ndarray:
+    out = np.empty(5)
+    wl, wr, n = _a(wl), _a(wr), _a(n)
+    lib().oracle_flux(policy, kind, wl.ctypes.data, wr.ctypes.data, n.ctypes.data, gamma, minf, pref, out.ctypes.data)
+    return out
+
+
+def physical_flux(w, n, gamma=1.4, pref=0.0) -> np.ndarray:
+    out = np.empty(5)
+    w, n = _a(w), _a(n)
+    lib().oracle_physical_flux(w.ctypes.data, n.ctypes.data, gamma, pref, out.ctypes.data)
+    return out
+
+
+def boundary_value(wg, kind, data, n, gamma=1.4, rgas=287.0, pref=0.0) -> np.ndarray:
+    out = np.empty(5)
+    wg, data, n = _a(wg), _a(data), _a(n)
+    lib().oracle_boundary_value(wg.ctypes.data, float(kind), data.ctypes.data, n.ctypes.data, gamma, rgas, pref,
+                                out.ctypes.data)
+    return out
+
+
+def to_primitive(dq, ur, gamma=1.4) -> np.ndarray:
+    out = np.empty(5)
+    dq, ur = _a(dq), _a(ur)
+    lib().oracle_to_primitive(dq.ctypes.data, ur.ctypes.data, gamma, out.ctypes.data)
+    return out
+
+
+def venkat(d, dmax, dmin, eps2) -> float:
+    return lib().oracle_venkat(d, dmax, dmin, eps2)
+
+
+def warp(w, gl, pl, gr, pr, m, gamma=1.4, pref=0.0) -> np.ndarray:
+    out = np.empty(5)
+    args = [_a(v) for v in (w, gl, pl, gr, pr, m)]
+    lib().oracle_warp(*[a.ctypes.data for a in args], gamma, pref, out.ctypes.data)
+    return out
+
+
+class Oracle:
+    """The reference's host residual path on one mesh; arrays FP64 SoA (5 x n_cells)."""
+
+    def __init__(self, mesh, scheme, policy="fp64", bc_kind=None, bc=None, source=None) -> None:
+        self.n = mesh.n_cells
+        self.nf = mesh.n_faces
+        self.nb = mesh.n_boundary
+        planar = mesh.all_planar if scheme.planar is None else scheme.planar
+        s = _Scheme()
+        s.flux = {"roe": 0, "hllc": 1, "ausm+up": 2}[scheme.flux] if isinstance(scheme.flux, str) else scheme.flux
+        s.second_order = int(scheme.second_order)
+        s.limiter = int(scheme.limiter)
+        s.planar = int(planar)
+        s.gamma, s.gas_constant, s.mach_ref = scheme.gamma, scheme.gas_constant, scheme.mach_ref
+        s.venkat_k, s.length = scheme.venkat_k, scheme.length
+        s.scale[:] = [float(v) for v in scheme.scale]
+        s.reference[:] = [float(v) for v in scheme.reference]
+        keep = []
+
+        def a(x, dtype=None):
+            x = np.ascontiguousarray(x) if dtype is None else np.ascontiguousarray(x, dtype=dtype)
+            keep.append(x)
+            return x.ctypes.data
+
+        nb = self.nb
+        kind = np.zeros(nb) if bc_kind is None else bc_kind
+        data = np.zeros(5 * nb) if bc is None else np.asarray(bc).reshape(-1)
+        inp = _Inputs(
+            mesh.n_cells, mesh.n_faces, mesh.n_interior,
+            a(mesh.owner), a(mesh.neighbour) if mesh.n_interior else a(np.zeros(1, np.int32)),
+            a(mesh.cell_face_offsets), a(mesh.cell_faces),
+            a(mesh.face_normal), a(mesh.face_area), a(mesh.r_left), a(mesh.r_right), a(mesh.face_moment),
+            a(mesh.cell_volume), a(mesh.stencil_offsets), a(mesh.stencil) if mesh.n_stencil else a(np.zeros(1, np.int32)),
+            a(mesh.stencil_weights) if mesh.n_stencil else a(np.zeros(3)),
+            a(kind, np.float64) if nb else a(np.zeros(1)), a(data, np.float64) if nb else a(np.zeros(5)),
+            None if source is None else a(np.asarray(source).reshape(-1), np.float64),
+        )  # fmt: skip
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        pol = {"fp64": 0, "fp32+acc64": 1}[policy] if isinstance(policy, str) else int(policy)
+        if lib().oracle_create(pol, ctypes.byref(s), ctypes.byref(inp), ctypes.byref(h), err, 1024) != 0:
+            raise RuntimeError(err.value.decode())
+        self.h = h
+
+    def __del__(self) -> None:
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def residual(self, u, gather: bool = True) -> np.ndarray:
+        u = _a(u).reshape(-1)
+        res = np.empty(5 * self.n)
+        lib().oracle_residual(self.h, u.ctypes.data, res.ctypes.data, int(gather))
+        return res
+
+    def spectral_radius(self) -> np.ndarray:
+        lam = np.empty(self.n)
+        lib().oracle_spectral_radius(self.h, lam.ctypes.data)
+        return lam
+
+    def face_fluxes(self) -> np.ndarray:
+        f = np.empty(5 * self.nf)
+        lib().oracle_face_fluxes(self.h, f.ctypes.data)
+        return f
+
+    def arrays(self) -> dict[str, np.ndarray]:
+        n = self.n
+        out = {k: np.empty(m * n) for k, m in (("prim", 5), ("grad", 15), ("phi", 5), ("wmin", 5), ("wmax", 5))}
+        out["prim_b_all"] = np.empty(5 * self.nf)
+        lib().oracle_arrays(self.h, *[out[k].ctypes.data for k in ("prim", "grad", "phi", "wmin", "wmax", "prim_b_all")])
+        pb = out.pop("prim_b_all").reshape(5, self.nf)[:, self.nf - self.nb :]
+        out["prim_b"] = np.ascontiguousarray(pb).reshape(-1)
+        return out
+
+    def set_second_order(self, on: bool) -> None:
+        lib().oracle_set_second_order(self.h, int(on))
+
+    def first_invalid(self, u, res=None) -> int | None:
+        u = _a(u).reshape(-1)
+        r = None if res is None else _a(res).reshape(-1)
+        c = lib().oracle_first_invalid(self.h, u.ctypes.data, None if r is None else r.ctypes.data)
+        return None if c < 0 else c
+
+    def step(self, u, cfl=0.0, dt=None, local=False, gather=True):
+        """One SSP-RK3 step of u (copied): (new u, norms, time step, invalid cell)."""
+        u = _a(u).reshape(-1).copy()
+        r = np.empty(5)
+        h = ctypes.c_double()
+        inv = ctypes.c_int()
+        lib().oracle_step(self.h, u.ctypes.data, cfl, int(dt is not None), 0.0 if dt is None else dt, int(local),
+                          int(gather), r.ctypes.data, ctypes.byref(h), ctypes.byref(inv))
+        return u, r, h.value, (None if inv.value < 0 else inv.value)
