@@ -76,7 +76,29 @@ struct Dev {
   C* lam;            // nc
   double* dt;        // nc
   Small* small;
+  // Block-local faces: the faces of the cells of logical block b are sface[sbeg[b] .. sbeg[b + 1])
+  // (ascending); incidence k of a cell reads its face's flux from LDS slot islot[k] & 0x3fff, bit
+  // 14 set where the cell is the face's neighbour, bit 15 on boundary faces.
+  const int* sbeg;
+  const int* sface;
+  const unsigned short* islot;
+  int max_slots;
+  long long* nslots;  // kNormSlots x kSums x kWords exact-sum words, spread to avoid contention
 };
+
+constexpr int kNormSlots = 64;
+
+// Workgroups are handed to the eight XCDs in turn (workgroup b to XCD b mod 8), each XCD with its
+// own L2. Remapped so that XCD k takes a contiguous run of logical blocks: the cells an XCD works
+// on at once are neighbours, and their stencil and face neighbours share its L2. The grid is
+// launched with xcd_grid(blocks) workgroups; the logical blocks past `blocks` do nothing.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_grid(int blocks) { return kXcds * ((blocks + kXcds - 1) / kXcds); }
+__device__ __forceinline__ int xcd_block() {
+  const int per = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+__device__ __forceinline__ int xcd_index() { return xcd_block() * kBlock + threadIdx.x; }
 
 template <class C>
 __device__ __forceinline__ W5<C> load_w(const C* __restrict__ p, long long i) {
@@ -166,7 +188,7 @@ __device__ __forceinline__ W5<C> primitive_of(const double* __restrict__ u, int 
 
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_prim(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_index();
   if (c >= d.nc) return;
   store_w(d.prim, c, primitive_of(u, d.nc, c, pa));
 }
@@ -193,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d, Params<C> pa) {
 // ties keeping the first entry. Without the limiter phi = 1 (second order unlimited).
 template <class C, bool LIMITER>
 __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d, Params<C> pa) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_index();
   if (c >= d.nc) return;
   const W5<C> wc = load_w(d.prim, c);
   const C wcv[5] = {wc.r, wc.u, wc.v, wc.w, wc.p};
@@ -274,11 +296,9 @@ __device__ __forceinline__ void load_rec(const C* __restrict__ rec, int c, C* g)
 }
 
 // {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for face f: the
-// area-weighted flux in C, stored for the per-cell gather (the store half of store_gather).
+// area-weighted flux in C.
 template <class C, int FLUX, bool PLANAR, bool SECOND>
-__global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
-  const int f = blockIdx.x * kBlock + threadIdx.x;
-  if (f >= d.nf) return;
+__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f) {
   C geo[8];
   load_n<C, 8>(d.fgeo + 8LL * f, geo);
   const V3<C> n{geo[0], geo[1], geo[2]};
@@ -356,6 +376,16 @@ __global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
       for (int k = 0; k < 5; ++k) out.f[k] = area * fx.f[k];
     }
   }
+  return out;
+}
+
+// The face loops' fluxes stored per face for the per-cell gather (the store half of the
+// reference's store_gather), for meshes whose blocks have too many faces for the fused kernel.
+template <class C, int FLUX, bool PLANAR, bool SECOND>
+__global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
+  const int f = xcd_index();
+  if (f >= d.nf) return;
+  const F5<C> out = face_flux<C, FLUX, PLANAR, SECOND>(d, pa, f);
   C o8[8] = {out.f[0], out.f[1], out.f[2], out.f[3], out.f[4], C(0), C(0), C(0)};
   store_n<C, 8>(d.fflux + 8LL * f, o8);
 }
@@ -365,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
 // from zero in FP64 and added to res (codegen/src/autan_codegen/emit_cpu.py:506-564).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_gather(Dev<C> d) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_index();
   if (c >= d.nc) return;
   const C nv = -d.vol[c];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, accb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -430,10 +460,14 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // spectral_radius (kernels/euler.py:1091-1112), then local_time_step (dt_c = CFL V / lambda,
-// physics/src/device_stepper.cuh:83-89) and the least dt_c over the cells, exactly.
+// physics/src/device_stepper.cuh:83-89) and each block's least dt_c as an ordered key in
+// partial[block]; k_min_key reduces those exactly (no same-address atomics from every wave).
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, Params<C> pa, double cfl, int store_dt) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, Params<C> pa, double cfl, int store_dt,
+                                                     unsigned long long* partial) {
+  __shared__ unsigned long long wave_key[kBlock / kWave];
+  const int blk = xcd_block();
+  const int c = blk * kBlock + threadIdx.x;
   unsigned long long key = ~0ULL;
   if (c < d.nc) {
     const W5<C> wc = load_w(d.prim, c);
@@ -462,7 +496,29 @@ __global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, Params<C> pa, dou
     if (!__builtin_isnan(dtc)) key = order_key(dtc);
   }
   key = wave_min(key);
-  if ((threadIdx.x % kWave) == 0 && key != ~0ULL) atomicMin(&d.small->hkey, key);
+  if ((threadIdx.x % kWave) == 0) wave_key[threadIdx.x / kWave] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wave_key[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; ++w) m = wave_key[w] < m ? wave_key[w] : m;
+    partial[blk] = m;
+  }
+}
+
+// The least of n keys (one block), into small->hkey if it is less.
+__global__ __launch_bounds__(1024) void k_min_key(const unsigned long long* __restrict__ partial, int n, Small* small) {
+  __shared__ unsigned long long wave_key[1024 / kWave];
+  unsigned long long m = ~0ULL;
+  for (int i = threadIdx.x; i < n; i += 1024) m = partial[i] < m ? partial[i] : m;
+  m = wave_min(m);
+  if ((threadIdx.x % kWave) == 0) wave_key[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 1024 / kWave; ++w) m = wave_key[w] < m ? wave_key[w] : m;
+    m = wave_key[0] < m ? wave_key[0] : m;
+    if (m < small->hkey) small->hkey = m;
+  }
 }
 
 // The validity checks of physics/src/device_stepper.cuh:107-145 for cell c of state u and
@@ -502,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, Params<C> pa, double a,
                                                const double* __restrict__ u, const double* stage_in,
                                                double* stage_out, const unsigned long long* hkey, double hvalue,
                                                int local) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_index();
   bool bad_state = false, bad_res = false;
   if (c < d.nc) {
     const double h = local ? d.dt[c] : (hkey != nullptr ? key_value(*hkey) : hvalue);
@@ -527,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, Params<C> pa, double a,
 // DeviceStepper::evaluate's validity checks alone.
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_check(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
+  const int c = xcd_index();
   bool bad_state = false, bad_res = false;
   if (c < d.nc) {
     bad_state = invalid_state(u, d.nc, c, pa);
@@ -615,12 +671,164 @@ __global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ res
   }
 }
 
-__global__ void k_clear(Small* small) {
+// Clears the step's results: block 0 the Small record, block b > 0 exact-sum slot b - 1.
+__global__ void k_clear(Small* small, long long* nslots) {
+  if (blockIdx.x > 0) {
+    long long* w = nslots + static_cast<long long>(blockIdx.x - 1) * kSums * kWords;
+    for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) w[t] = 0;
+    return;
+  }
   for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) (&small->words[0][0])[t] = 0;
   if (threadIdx.x == 0) {
     small->hkey = 0xfff0000000000000ULL;  // order_key(+inf): no cell, an infinite step
     small->invalid_state = INT32_MAX;
     small->invalid_residual = INT32_MAX;
+  }
+}
+
+// The convective spectral radius of face f for each side (kernels/euler.py:1091-1112): the
+// larger of the two sides' wave speeds times the area, taken as the owner takes it (own first)
+// and as the neighbour takes it, so that the max's tie and NaN rules stay those of each cell.
+template <class C>
+__device__ __forceinline__ void face_lambda(const Dev<C>& d, const Params<C>& pa, int f, C& for_owner,
+                                            C& for_neighbour) {
+  C geo[4];
+  load_n<C, 4>(d.fgeo + 8LL * f, geo);
+  const V3<C> n{geo[0], geo[1], geo[2]};
+  W5<C> wl, wr;
+  if (f < d.ni) {
+    const int2 lr = d.lr[f];
+    wl = load_w(d.prim, lr.x);
+    wr = load_w(d.prim, lr.y);
+  } else {
+    wl = load_w(d.prim, d.bowner[f - d.ni]);
+    wr = load_w(d.prim_b, f - d.ni);
+  }
+  const C ll = wave_speed(wl, n, pa.g, pa.pref);
+  const C lr = wave_speed(wr, n, pa.g, pa.pref);
+  for_owner = smax(ll, lr) * geo[3];
+  for_neighbour = smax(lr, ll) * geo[3];
+}
+
+// The faces and boundaries loops fused with their gather, one logical block of 256 cells per
+// workgroup. Phase 1: every face of the block's cells (its interior faces once, the faces it
+// shares with another block here and there too) evaluates its flux into LDS. Phase 2: each cell
+// adds its faces' fluxes in its cell_faces order, interior faces then boundary faces, each set
+// from zero in FP64, onto -V s (kernels/euler.py:611-616): the store_gather order of the
+// reference's device path, with no face array in HBM. FIRST (the step's R(u)) also gathers the
+// spectral radius, the cell's time step and the block's least one, and the exact sums of R^2.
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double cfl, int store_dt,
+                                                  unsigned long long* partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int kComp = FIRST ? 7 : 5;
+  const int ms = d.max_slots;
+  C* lf = reinterpret_cast<C*>(smem);
+  long long* acc = reinterpret_cast<long long*>(smem + ((kComp * ms * static_cast<int>(sizeof(C)) + 15) & ~15));
+  __shared__ unsigned long long wave_key[kBlock / kWave];
+  const int blk = xcd_block();
+  const int s0 = d.sbeg[blk], ns = d.sbeg[blk + 1] - s0;
+  if constexpr (FIRST) {
+    for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) acc[t] = 0;
+  }
+  for (int i = threadIdx.x; i < ns; i += kBlock) {
+    const int f = d.sface[s0 + i];
+    const F5<C> fx = face_flux<C, FLUX, PLANAR, SECOND>(d, pa, f);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) lf[q * ms + i] = fx.f[q];
+    if constexpr (FIRST) {
+      C lo, ln;
+      face_lambda(d, pa, f, lo, ln);
+      lf[5 * ms + i] = lo;
+      lf[6 * ms + i] = ln;
+    }
+  }
+  __syncthreads();
+  const int c = blk * kBlock + threadIdx.x;
+  const bool active = c < d.nc;
+  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double lam_acc = 0.0;
+  if (active) {
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+      const unsigned v = d.islot[k];
+      const int slot = v & 0x3fff;
+      if (v & 0x8000u) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) b[q] += static_cast<double>(lf[q * ms + slot]);
+      } else if (v & 0x4000u) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a[q] += -static_cast<double>(lf[q * ms + slot]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a[q] += static_cast<double>(lf[q * ms + slot]);
+      }
+      if constexpr (FIRST) lam_acc += static_cast<double>(lf[((v & 0x4000u) ? 6 : 5) * ms + slot]);
+    }
+    const C nv = -d.vol[c];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const C sq = d.source != nullptr ? d.source[static_cast<long long>(q) * d.nc + c] : C(0);
+      r[q] = (static_cast<double>(nv * sq) + a[q]) + b[q];
+      d.res[static_cast<long long>(q) * d.nc + c] = r[q];
+    }
+  }
+  if constexpr (FIRST) {
+    unsigned long long key = ~0ULL;
+    if (active) {
+      const C lam = static_cast<C>(lam_acc);
+      d.lam[c] = lam;
+      const double dtc = (cfl * d.vol64[c]) / static_cast<double>(lam);
+      if (store_dt) d.dt[c] = dtc;
+      if (!__builtin_isnan(dtc)) key = order_key(dtc);
+    }
+    key = wave_min(key);
+    if ((threadIdx.x % kWave) == 0) wave_key[threadIdx.x / kWave] = key;
+#pragma unroll
+    for (int q = 0; q < kSums; ++q) exact_add_wave(acc + q * kWords, r[q] * r[q], active);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long m = wave_key[0];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; ++w) m = wave_key[w] < m ? wave_key[w] : m;
+      partial[blk] = m;
+    }
+    if (threadIdx.x < kSums) {
+      long long* w = acc + threadIdx.x * kWords;
+      for (int l = 0; l + 1 < kLimbs; ++l) {
+        const long long carry = w[l] >> 32;
+        w[l] -= carry * (1LL << 32);
+        w[l + 1] += carry;
+      }
+    }
+    __syncthreads();
+    long long* out = d.nslots + static_cast<long long>(blk % kNormSlots) * kSums * kWords;
+    for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) {
+      const long long v = acc[t];
+      if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(out + t), static_cast<unsigned long long>(v));
+    }
+  }
+}
+
+// The step's reductions, one block: the exact-sum slots added into small->words (integer sums:
+// any order gives the same words), and the least of the blocks' time-step keys.
+__global__ __launch_bounds__(1024) void k_step_reduce(const long long* __restrict__ nslots,
+                                                      const unsigned long long* __restrict__ partial, int n,
+                                                      Small* small) {
+  __shared__ unsigned long long wave_key[1024 / kWave];
+  for (int t = threadIdx.x; t < kSums * kWords; t += 1024) {
+    long long v = 0;
+    for (int q = 0; q < kNormSlots; ++q) v += nslots[static_cast<long long>(q) * kSums * kWords + t];
+    (&small->words[0][0])[t] += v;
+  }
+  unsigned long long m = ~0ULL;
+  for (int i = threadIdx.x; i < n; i += 1024) m = partial[i] < m ? partial[i] : m;
+  m = wave_min(m);
+  if ((threadIdx.x % kWave) == 0) wave_key[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < 1024 / kWave; ++w) m = wave_key[w] < m ? wave_key[w] : m;
+    if (m < small->hkey) small->hkey = m;
   }
 }
 

@@ -19,6 +19,7 @@
 
 #include "../../../include/mxeuler.h"
 #include "kernels.hpp"
+#include "stream.hpp"
 
 namespace mxe {
 
@@ -29,6 +30,8 @@ void check(hipError_t e, const char* what) {
 }
 
 int blocks(long long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
+// Workgroups for n items under the XCD-aware block mapping.
+int grid(long long n) { return xcd_grid(blocks(n)); }
 
 template <class T>
 class Buffer {
@@ -137,6 +140,8 @@ class Stepper final : public StepperBase {
     lam_ = Buffer<C>(std::size_t(nc_));
     dt_ = Buffer<double>(std::size_t(nc_));
     small_ = Buffer<Small>(1);
+    partial_ = Buffer<unsigned long long>(std::size_t(grid(nc_)));
+    nslots_ = Buffer<long long>(std::size_t(kNormSlots) * kSums * kWords);
     check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small), hipHostMallocDefault), "hipHostMalloc");
     check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "create");
@@ -176,9 +181,8 @@ class Stepper final : public StepperBase {
   void evaluate(double* residuals, int* invalid) override {
     clear();
     launch_prim(u_.get());
-    residual_from_prim();
-    launch("norms", [&] { k_norms<<<norm_blocks(), kBlock, 0, stream_>>>(res_.get(), nc_, small_.get()); });
-    launch("check", [&] { k_check<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, u_.get()); });
+    residual_from_prim(true, 1.0, false);
+    launch("check", [&] { k_check<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), pa_, u_.get()); });
     finish(residuals, invalid);
   }
 
@@ -210,7 +214,7 @@ class Stepper final : public StepperBase {
 
   void spectral_radius(double* lam) override {
     clear();
-    launch("spectral", [&] { k_spectral<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, 1.0, 0); });
+    launch_spectral(1.0, false);
     std::vector<C> l(static_cast<std::size_t>(nc_));
     lam_.download(l.data(), l.size(), stream_);
     check(hipStreamSynchronize(stream_), "spectral_radius");
@@ -321,6 +325,11 @@ class Stepper final : public StepperBase {
     d.lam = lam_.get();
     d.dt = dt_.get();
     d.small = small_.get();
+    d.sbeg = sbeg_.get();
+    d.sface = sface_.get();
+    d.islot = islot_.get();
+    d.max_slots = max_slots_;
+    d.nslots = nslots_.get();
     return d;
   }
 
@@ -418,7 +427,55 @@ class Stepper final : public StepperBase {
     bowner_.upload(bowner, stream_);
     cfo_.upload(cfo, stream_);
     inc_.upload(inc, stream_);
+    build_slots(t, cfo);
     check(hipStreamSynchronize(stream_), "topology");
+  }
+
+  // The block-local face lists of the fused face kernel: for each logical block of kBlock cells,
+  // its cells' faces ascending, and for every incidence the LDS slot of its face with the cell's
+  // side and whether the face is a boundary face.
+  void build_slots(const mxeuler_topology& t, const std::vector<int>& cfo) {
+    const int nblk = grid(nc_);
+    std::vector<int> sbeg(std::size_t(nblk) + 1, 0);
+    std::vector<int> sface;
+    sface.reserve(std::size_t(cfo.back()) / 2 + std::size_t(nc_));
+    std::vector<unsigned short> islot(std::size_t(std::max(cfo.back(), 1)), 0);
+    std::vector<int> faces;
+    int max_slots = 0;
+    bool fits = true;
+    for (int b = 0; b < nblk; ++b) {
+      const int c0 = std::min(nc_, b * kBlock), c1 = std::min(nc_, (b + 1) * kBlock);
+      faces.assign(t.cell_faces + cfo[std::size_t(c0)], t.cell_faces + cfo[std::size_t(c1)]);
+      std::sort(faces.begin(), faces.end());
+      faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+      max_slots = std::max(max_slots, static_cast<int>(faces.size()));
+      if (faces.size() > 0x3fff) fits = false;
+      for (int c = c0; c < c1 && fits; ++c) {
+        for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) {
+          const int f = t.cell_faces[k];
+          const auto slot = static_cast<unsigned>(std::lower_bound(faces.begin(), faces.end(), f) - faces.begin());
+          const bool right = f < ni_ && t.owner[f] != c;
+          const bool boundary = f >= ni_;
+          islot[std::size_t(k)] = static_cast<unsigned short>(slot | (right ? 0x4000u : 0u) | (boundary ? 0x8000u : 0u));
+        }
+      }
+      sface.insert(sface.end(), faces.begin(), faces.end());
+      sbeg[std::size_t(b) + 1] = static_cast<int>(sface.size());
+    }
+    max_slots_ = max_slots;
+    fused_ = fits && faces_lds(true) <= 65536;
+    if (!fused_) return;
+    if (sface.empty()) sface.push_back(0);
+    sbeg_.upload(sbeg, stream_);
+    sface_.upload(sface, stream_);
+    islot_.upload(islot, stream_);
+  }
+
+  // Dynamic LDS of the fused face kernel.
+  std::size_t faces_lds(bool first) const {
+    const std::size_t comp = first ? 7 : 5;
+    const std::size_t fl = (comp * std::size_t(max_slots_) * sizeof(C) + 15) & ~std::size_t(15);
+    return fl + (first ? std::size_t(kSums) * kWords * sizeof(long long) : 0);
   }
 
   void build_stencil(const mxeuler_topology& t, const mxeuler_stencil& st) {
@@ -514,34 +571,99 @@ class Stepper final : public StepperBase {
     }
   }
 
-  int norm_blocks() const { return std::max(1, std::min(blocks(nc_), 2048)); }
+  int norm_blocks() const { return std::max(1, std::min(blocks(nc_), 512)); }
+
+  // Spectral radius, dt and its least value: per-block minima, then one block reduces them.
+  void launch_spectral(double cfl, bool local) {
+    const int g = grid(nc_);
+    launch("spectral", [&] {
+      k_spectral<C><<<g, kBlock, 0, stream_>>>(dev(), pa_, cfl, local ? 1 : 0, partial_.get());
+    });
+    launch("min_dt", [&] { k_min_key<<<1, 1024, 0, stream_>>>(partial_.get(), g, small_.get()); });
+  }
 
   void clear() {
-    launch("clear", [&] { k_clear<<<1, kBlock, 0, stream_>>>(small_.get()); });
+    launch("clear", [&] { k_clear<<<1 + kNormSlots, kBlock, 0, stream_>>>(small_.get(), nslots_.get()); });
   }
 
   void launch_prim(const double* u) {
-    launch("prim", [&] { k_prim<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, u); });
+    launch("prim", [&] { k_prim<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), pa_, u); });
   }
 
-  // Impl::cell_loops and Impl::faces from the primitives already stored.
-  void residual_from_prim() {
+  // Impl::cell_loops and Impl::faces from the primitives already stored. `first` (the step's
+  // R(u)) also leaves the spectral radius, the time steps and the exact norms' words.
+  void residual_from_prim(bool first = false, double cfl = 0.0, bool local = false) {
     const Dev<C> d = dev();
     if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d, pa_); });
     if (second_) {
       if (sc_.limiter) {
-        launch("grad_limiter", [&] { k_grad<C, true><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_); });
+        launch("grad_limiter", [&] { k_grad<C, true><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
       } else {
-        launch("grad", [&] { k_grad<C, false><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_); });
+        launch("grad", [&] { k_grad<C, false><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
       }
     }
+    if (fused_) {
+      launch(first ? "faces_first" : "faces", [&] { launch_faces(d, first, cfl, local); });
+      if (first) {
+        launch("step_reduce", [&] {
+          k_step_reduce<<<1, 1024, 0, stream_>>>(nslots_.get(), partial_.get(), grid(nc_), small_.get());
+        });
+      }
+      return;
+    }
     launch("flux", [&] { launch_flux(d); });
-    launch("gather", [&] { k_gather<C><<<blocks(nc_), kBlock, 0, stream_>>>(d); });
+    launch("gather", [&] { k_gather<C><<<grid(nc_), kBlock, 0, stream_>>>(d); });
+    if (first) {
+      launch("norms", [&] { k_norms<<<norm_blocks(), kBlock, 0, stream_>>>(res_.get(), nc_, small_.get()); });
+      launch_spectral(cfl, local);
+    }
+  }
+
+  template <int FLUX, bool PLANAR, bool SECOND>
+  void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
+    const int g = grid(nc_);
+    if (first) {
+      k_faces<C, FLUX, PLANAR, SECOND, true><<<g, kBlock, faces_lds(true), stream_>>>(d, pa_, cfl, local ? 1 : 0,
+                                                                                   partial_.get());
+    } else {
+      k_faces<C, FLUX, PLANAR, SECOND, false><<<g, kBlock, faces_lds(false), stream_>>>(d, pa_, cfl, 0, nullptr);
+    }
+  }
+
+  template <int FLUX>
+  void launch_faces_f(const Dev<C>& d, bool first, double cfl, bool local) {
+    if (planar_) {
+      if (second_) {
+        launch_faces_t<FLUX, true, true>(d, first, cfl, local);
+      } else {
+        launch_faces_t<FLUX, true, false>(d, first, cfl, local);
+      }
+    } else {
+      if (second_) {
+        launch_faces_t<FLUX, false, true>(d, first, cfl, local);
+      } else {
+        launch_faces_t<FLUX, false, false>(d, first, cfl, local);
+      }
+    }
+  }
+
+  void launch_faces(const Dev<C>& d, bool first, double cfl, bool local) {
+    switch (sc_.flux) {
+      case MXEULER_ROE:
+        launch_faces_f<0>(d, first, cfl, local);
+        break;
+      case MXEULER_HLLC:
+        launch_faces_f<1>(d, first, cfl, local);
+        break;
+      default:
+        launch_faces_f<2>(d, first, cfl, local);
+        break;
+    }
   }
 
   template <int FLUX>
   void launch_flux_t(const Dev<C>& d) {
-    const int g = blocks(nf_);
+    const int g = grid(nf_);
     if (planar_) {
       if (second_) {
         k_flux<C, FLUX, true, true><<<g, kBlock, 0, stream_>>>(d, pa_);
@@ -575,14 +697,8 @@ class Stepper final : public StepperBase {
   void enqueue_step(double cfl, bool has_dt, double dt, bool local) {
     clear();
     launch_prim(u_.get());
-    residual_from_prim();
-    launch("norms", [&] { k_norms<<<norm_blocks(), kBlock, 0, stream_>>>(res_.get(), nc_, small_.get()); });
     const bool use_local = !has_dt && local;
-    if (!has_dt) {
-      launch("spectral", [&] {
-        k_spectral<C><<<blocks(nc_), kBlock, 0, stream_>>>(dev(), pa_, cfl, use_local ? 1 : 0);
-      });
-    }
+    residual_from_prim(true, has_dt ? 1.0 : cfl, use_local);
     const unsigned long long* hkey = has_dt ? nullptr : &small_.get()->hkey;
     // Shu and Osher (1988), eq. 2.18.
     static constexpr double w[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
@@ -592,12 +708,12 @@ class Stepper final : public StepperBase {
       const Dev<C> d = dev();
       if (s < 2) {
         launch("rk_prim", [&] {
-          k_rk<C, true, false><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
+          k_rk<C, true, false><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
                                                                      stage_.get(), hkey, dt, use_local ? 1 : 0);
         });
       } else {
         launch("rk_check", [&] {
-          k_rk<C, false, true><<<blocks(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
+          k_rk<C, false, true><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
                                                                      stage_.get(), hkey, dt, use_local ? 1 : 0);
         });
       }
@@ -673,6 +789,12 @@ class Stepper final : public StepperBase {
   Buffer<double> u_, stage_, res_, dt_;
   Buffer<C> prim_, prim_b_, rec_, fflux_, lam_;
   Buffer<Small> small_;
+  Buffer<unsigned long long> partial_;
+  Buffer<int> sbeg_, sface_;
+  Buffer<unsigned short> islot_;
+  Buffer<long long> nslots_;
+  int max_slots_ = 0;
+  bool fused_ = false;
   Small* host_small_ = nullptr;
   bool prof_ = false;
   std::vector<std::string> prof_names_;
@@ -825,6 +947,48 @@ int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t
     e.download(out_exp, static_cast<std::size_t>(n), nullptr);
     l.download(out_log, static_cast<std::size_t>(n), nullptr);
     mxe::check(hipDeviceSynchronize(), "probe");
+  });
+}
+
+int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    using mxe::Buffer;
+    if (n < 2 || n % 2 != 0 || repeats < 1) throw std::invalid_argument("stream: n even and >= 2, repeats >= 1");
+    const long long m = n / 2;  // 16 B elements
+    Buffer<double2> a(static_cast<std::size_t>(m)), b(static_cast<std::size_t>(m)), c(static_cast<std::size_t>(m));
+    const int g = static_cast<int>((m + 255) / 256);
+    hipStream_t s;
+    mxe::check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    mxe::k_stream_init<<<g, 256, 0, s>>>(a.get(), b.get(), c.get(), m);
+    hipEvent_t e0, e1;
+    mxe::check(hipEventCreate(&e0), "hipEventCreate");
+    mxe::check(hipEventCreate(&e1), "hipEventCreate");
+    const double bytes[4] = {32.0 * m, 32.0 * m, 48.0 * m, 48.0 * m};  // arrays read + written
+    for (int k = 0; k < 4; ++k) {
+      gbps[k] = 0.0;
+      ms[k] = 1e30;
+    }
+    for (int r = 0; r < repeats; ++r) {
+      for (int k = 0; k < 4; ++k) {
+        mxe::check(hipEventRecord(e0, s), "hipEventRecord");
+        if (k == 0) mxe::k_stream_copy<<<g, 256, 0, s>>>(a.get(), c.get(), m);
+        if (k == 1) mxe::k_stream_scale<<<g, 256, 0, s>>>(c.get(), b.get(), 3.0, m);
+        if (k == 2) mxe::k_stream_add<<<g, 256, 0, s>>>(a.get(), b.get(), c.get(), m);
+        if (k == 3) mxe::k_stream_triad<<<g, 256, 0, s>>>(b.get(), c.get(), a.get(), 3.0, m);
+        mxe::check(hipGetLastError(), "stream kernel");
+        mxe::check(hipEventRecord(e1, s), "hipEventRecord");
+        mxe::check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float t = 0.0f;
+        mxe::check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        if (t < ms[k]) {
+          ms[k] = t;
+          gbps[k] = bytes[k] / (t * 1e-3) / 1e9;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
   });
 }
 

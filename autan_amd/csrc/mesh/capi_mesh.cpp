@@ -72,6 +72,13 @@ int mxmesh_renumber_rcm(mxmesh* m, char* err, size_t err_len) {
   });
 }
 
+int mxmesh_renumber_morton(mxmesh* m, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (m->prepared) throw std::logic_error("renumber before prepare");
+    m->mesh = mx::renumber(m->mesh, mx::morton_order(m->mesh));
+  });
+}
+
 int mxmesh_prepare(mxmesh* m, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
     const mx::Geometry g = mx::compute_geometry(m->mesh);

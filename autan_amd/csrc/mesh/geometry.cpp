@@ -234,8 +234,8 @@ Stencil face2_stencil(const Mesh& m, const Packed& p) {
   const std::size_t nc = at(m.n_cells), nf = at(m.n_faces), ni = at(m.n_interior);
   Stencil st;
   st.offsets.assign(nc + 1, 0);
-  st.entries.reserve(nc * 20);
-  st.position.reserve(3 * nc * 20);
+  st.entries.reserve(nc * 16);
+  std::vector<double> shift;  // entry-major while building
   // Per cell under construction: the translation each listed cell or boundary face was reached
   // with (a periodic path), to check that one index never stands for two images.
   std::vector<index_t> listed(nc, -1), bound_listed(nf - ni, -1);
@@ -309,33 +309,28 @@ Stencil face2_stencil(const Mesh& m, const Packed& p) {
     std::sort(bfaces.begin(), bfaces.end(), [](const Item& a, const Item& b) { return a.id < b.id; });
     for (const Item& it : cells) {
       st.entries.push_back(it.id);
-      for (std::size_t u = 0; u < 3; ++u) {
-        const double x = p.cell_centroid[u * nc + at(it.id)];
-        st.position.push_back(it.t[u] != 0.0 ? x + it.t[u] : x);
-      }
+      if (m.periodic()) shift.insert(shift.end(), it.t.begin(), it.t.end());
     }
     for (const Item& it : bfaces) {
       st.entries.push_back(-it.id - 1);
-      for (std::size_t u = 0; u < 3; ++u) {
-        const double x = p.face_centroid[u * nf + at(it.id)];
-        st.position.push_back(it.t[u] != 0.0 ? x + it.t[u] : x);
-      }
+      if (m.periodic()) shift.insert(shift.end(), it.t.begin(), it.t.end());
     }
     st.offsets[c + 1] = static_cast<offset_t>(st.entries.size());
   }
-  // position was filled entry-major (3 per entry): transpose to 3 x entries.
-  const std::size_t ns = st.entries.size();
-  std::vector<double> pos(3 * ns);
-  for (std::size_t e = 0; e < ns; ++e) {
-    for (std::size_t u = 0; u < 3; ++u) pos[u * ns + e] = st.position[3 * e + u];
+  (void)p;
+  if (m.periodic()) {
+    const std::size_t ns = st.entries.size();
+    st.translation.resize(3 * ns);
+    for (std::size_t e = 0; e < ns; ++e) {
+      for (std::size_t u = 0; u < 3; ++u) st.translation[u * ns + e] = shift[3 * e + u];
+    }
   }
-  st.position.swap(pos);
   return st;
 }
 
 std::size_t gradient_weights(const Mesh& m, const Packed& p, const Stencil& s, const std::vector<int>& kind,
                              std::vector<double>& w) {
-  const std::size_t nc = at(m.n_cells), ni = at(m.n_interior);
+  const std::size_t nc = at(m.n_cells), ni = at(m.n_interior), nf = at(m.n_faces);
   const std::size_t ns = s.entries.size();
   if (kind.size() != at(m.n_boundary())) throw std::invalid_argument("gradient_weights: one kind per boundary face");
   w.assign(3 * ns, 0.0);
@@ -356,9 +351,11 @@ std::size_t gradient_weights(const Mesh& m, const Packed& p, const Stencil& s, c
           used = all_boundaries || kind[f - ni] == bc_state;
         }
         // Offsets within 64 ulps of the coordinates are round-off (physics/src/euler.cpp:371-380).
+        // The entry's centroid: a cell's or a boundary face's, moved to the image the cell sees.
         std::array<double, 4> dk{};
         for (std::size_t j = 0; j < 3; ++j) {
-          const double x = s.position[j * ns + k];
+          double x = e >= 0 ? p.cell_centroid[j * nc + at(e)] : p.face_centroid[j * nf + at(-e - 1)];
+          if (!s.translation.empty() && s.translation[j * ns + k] != 0.0) x += s.translation[j * ns + k];
           const double xc = p.cell_centroid[j * nc + c];
           const double round_off = 64 * coordinate_eps * std::max(std::abs(x), std::abs(xc));
           dk[j] = x - xc;

@@ -607,6 +607,74 @@ std::vector<index_t> rcm_order(const Mesh& m) {
   return order;
 }
 
+namespace {
+
+// The 21 low bits of v spread to every third bit.
+std::uint64_t spread3(std::uint64_t v) {
+  v &= 0x1fffffULL;
+  v = (v | (v << 32)) & 0x1f00000000ffffULL;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+  v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+  v = (v | (v << 2)) & 0x1249249249249249ULL;
+  return v;
+}
+
+}  // namespace
+
+std::vector<index_t> morton_order(const Mesh& m) {
+  const std::size_t nc = at(m.n_cells), ni = at(m.n_interior);
+  std::vector<double> cx(nc, 0.0), cy(nc, 0.0), cz(nc, 0.0), cnt(nc, 0.0);
+  for (std::size_t f = 0; f < at(m.n_faces); ++f) {
+    const auto a = static_cast<std::size_t>(m.face_node_offsets[f]), b = static_cast<std::size_t>(m.face_node_offsets[f + 1]);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (std::size_t k = a; k < b; ++k) {
+      const auto p = at(m.face_nodes[k]);
+      sx += m.x[p];
+      sy += m.y[p];
+      sz += m.z[p];
+    }
+    const double nn = static_cast<double>(b - a);
+    const std::size_t o = at(m.owner[f]);
+    cx[o] += sx;
+    cy[o] += sy;
+    cz[o] += sz;
+    cnt[o] += nn;
+    if (f < ni) {
+      const std::size_t r = at(m.neighbour[f]);
+      const auto s = m.face_shift(static_cast<index_t>(f));
+      cx[r] += sx - nn * s[0];
+      cy[r] += sy - nn * s[1];
+      cz[r] += sz - nn * s[2];
+      cnt[r] += nn;
+    }
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (std::size_t c = 0; c < nc; ++c) {
+    cx[c] /= cnt[c];
+    cy[c] /= cnt[c];
+    cz[c] /= cnt[c];
+    const double v[3] = {cx[c], cy[c], cz[c]};
+    for (int u = 0; u < 3; ++u) {
+      lo[u] = std::min(lo[u], v[u]);
+      hi[u] = std::max(hi[u], v[u]);
+    }
+  }
+  const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+  const double q = static_cast<double>((1u << 21) - 1) / span;
+  std::vector<std::pair<std::uint64_t, index_t>> key(nc);
+  for (std::size_t c = 0; c < nc; ++c) {
+    const auto ix = static_cast<std::uint64_t>((cx[c] - lo[0]) * q);
+    const auto iy = static_cast<std::uint64_t>((cy[c] - lo[1]) * q);
+    const auto iz = static_cast<std::uint64_t>((cz[c] - lo[2]) * q);
+    key[c] = {spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2), static_cast<index_t>(c)};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<index_t> order(nc);
+  for (std::size_t c = 0; c < nc; ++c) order[c] = key[c].second;
+  return order;
+}
+
 Mesh renumber(const Mesh& m, const std::vector<index_t>& new_to_old) {
   const std::size_t nc = at(m.n_cells), nf = at(m.n_faces), ni = at(m.n_interior);
   if (new_to_old.size() != nc) throw std::invalid_argument("renumber: permutation size");

@@ -106,12 +106,12 @@ Packed pack(const Mesh& m, const Geometry& g);
 
 /// The face2 gradient stencil of cell_stencil (codegen/cpp/geometry.cpp:432-617): per cell its
 /// face neighbours and their face neighbours in increasing index, then the boundary faces of both
-/// as -(f + 1) in increasing index. `position` holds each entry's FP64 centroid (a cell's or a
-/// boundary face's) with periodic translations applied, which the weights need.
+/// as -(f + 1) in increasing index. On a periodic mesh `translation` holds, per entry, the
+/// periodic shift of the image the cell sees (3 x entries); it is empty otherwise.
 struct Stencil {
   std::vector<offset_t> offsets;  // nc + 1
   std::vector<index_t> entries;
-  std::vector<double> position;   // 3 x entries: the entry's centroid as seen from the cell
+  std::vector<double> translation;  // 3 x entries, periodic meshes only
 };
 Stencil face2_stencil(const Mesh& m, const Packed& p);
 
@@ -124,6 +124,10 @@ std::size_t gradient_weights(const Mesh& m, const Packed& p, const Stencil& s,
 
 /// Reverse Cuthill-McKee order of the cell graph (core/src/renumber.cpp:109-148): new_to_old.
 std::vector<index_t> rcm_order(const Mesh& m);
+/// Morton (Z-curve) order of the cell centres (mean of the cells' face nodes) quantised to 2^21
+/// per axis over the bounding box: new_to_old. A run of consecutive cells is then a compact 3D
+/// region, so the cells a GPU works on together share their stencil neighbours in cache.
+std::vector<index_t> morton_order(const Mesh& m);
 /// The mesh with cells renumbered (new_to_old), re-assembled in canonical order.
 Mesh renumber(const Mesh& m, const std::vector<index_t>& new_to_old);
 

@@ -81,13 +81,19 @@ def _call(fn, *args) -> None:
         raise RuntimeError(err.value.decode())
 
 
-def _build(create, *args, rcm: bool = True, kinds: np.ndarray | None = None) -> Mesh:
+def _build(create, *args, order: str | None = "rcm", kinds: np.ndarray | None = None) -> Mesh:
+    """order: "rcm" (the reference's renumbering), "morton" (a Z-curve of the cell centres) or
+    None (the generator's order)."""
     lib = _lib.mesh()
     ptr = ctypes.c_void_p()
     _call(create, *args, ctypes.byref(ptr))
     h = _Handle(ptr)
-    if rcm:
+    if order == "rcm":
         _call(lib.mxmesh_renumber_rcm, h.ptr)
+    elif order == "morton":
+        _call(lib.mxmesh_renumber_morton, h.ptr)
+    elif order is not None:
+        raise ValueError(f"unknown cell order {order!r}")
     _call(lib.mxmesh_prepare, h.ptr)
     sizes = (ctypes.c_int64 * 8)()
     lib.mxmesh_sizes(h.ptr, sizes)
@@ -129,8 +135,9 @@ def _fetch(h: _Handle) -> dict[str, np.ndarray]:
         if n == 0:
             out[name] = np.zeros(0, dtype=dt)
             continue
+        # A view of the native array: valid while the handle (kept with the Mesh) lives.
         buf = (ctypes.c_char * (n * dt.itemsize)).from_address(data.value)
-        out[name] = np.frombuffer(buf, dtype=dt).copy()
+        out[name] = np.frombuffer(buf, dtype=dt)
     return out
 
 
@@ -141,25 +148,25 @@ def set_boundary_kinds(mesh: Mesh, kinds: np.ndarray) -> None:
     mesh.arrays["stencil_weights"] = _fetch(h)["stencil_weights"]
 
 
-def periodic_hex(n: int, lo: float = 0.0, hi: float = 1.0, rcm: bool = True) -> Mesh:
+def periodic_hex(n: int, lo: float = 0.0, hi: float = 1.0, order: str | None = "rcm") -> Mesh:
     lib = _lib.mesh()
-    return _build(lib.mxmesh_periodic_hex, ctypes.c_int(n), ctypes.c_double(lo), ctypes.c_double(hi), rcm=rcm)
+    return _build(lib.mxmesh_periodic_hex, ctypes.c_int(n), ctypes.c_double(lo), ctypes.c_double(hi), order=order)
 
 
-def kuhn_tets(n: int, jitter: float = 0.1, seed: int = 0, rcm: bool = True, kinds=None) -> Mesh:
+def kuhn_tets(n: int, jitter: float = 0.1, seed: int = 0, order: str | None = "rcm", kinds=None) -> Mesh:
     lib = _lib.mesh()
     return _build(
-        lib.mxmesh_kuhn_tets, ctypes.c_int(n), ctypes.c_double(jitter), ctypes.c_uint64(seed), rcm=rcm, kinds=kinds
+        lib.mxmesh_kuhn_tets, ctypes.c_int(n), ctypes.c_double(jitter), ctypes.c_uint64(seed), order=order, kinds=kinds
     )
 
 
-def bump_hex(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, h=0.1, w=0.2, x0=1.5, rcm=True, kinds=None) -> Mesh:
+def bump_hex(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, h=0.1, w=0.2, x0=1.5, order="rcm", kinds=None) -> Mesh:
     lib = _lib.mesh()
     args = [ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz)]
     args += [ctypes.c_double(v) for v in (lx, ly, lz, h, w, x0)]
-    return _build(lib.mxmesh_bump_hex, *args, rcm=rcm, kinds=kinds)
+    return _build(lib.mxmesh_bump_hex, *args, order=order, kinds=kinds)
 
 
-def read_polymesh(path: str | os.PathLike, rcm: bool = False, kinds=None) -> Mesh:
+def read_polymesh(path: str | os.PathLike, order: str | None = None, kinds=None) -> Mesh:
     lib = _lib.mesh()
-    return _build(lib.mxmesh_read_polymesh, os.fsencode(str(path)), rcm=rcm, kinds=kinds)
+    return _build(lib.mxmesh_read_polymesh, os.fsencode(str(path)), order=order, kinds=kinds)

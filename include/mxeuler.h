@@ -140,6 +140,10 @@ int mxeuler_profile(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local
 int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t n, double* out_sqrt,
                        double* out_div, double* out_exp, double* out_log, char* err, size_t err_len);
 
+/* HIP STREAM (bench/stream/stream.cu's measure of peak bandwidth): n FP64 values per array, the
+ * best of `repeats` runs of copy, scale, add and triad; gbps and ms receive 4 values each. */
+int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* err, size_t err_len);
+
 /* Version string of the library. */
 const char* mxeuler_version(void);
 

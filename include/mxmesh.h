@@ -37,6 +37,9 @@ void mxmesh_destroy(mxmesh* m);
 
 /* Renumbers the cells by reverse Cuthill-McKee (before mxmesh_prepare). */
 int mxmesh_renumber_rcm(mxmesh* m, char* err, size_t err_len);
+/* Renumbers the cells along a Morton curve of their centres (before mxmesh_prepare): runs of
+ * consecutive cells are compact regions, which keeps stencil gathers in cache. */
+int mxmesh_renumber_morton(mxmesh* m, char* err, size_t err_len);
 /* FP64 geometry, packed device geometry and the face2 stencil. */
 int mxmesh_prepare(mxmesh* m, char* err, size_t err_len);
 /* One condition code per boundary face (BoundaryKind values 0..5): recomputes the gradient

@@ -94,20 +94,20 @@ def _prim(u: np.ndarray, gamma=GAMMA) -> np.ndarray:
     return np.stack([rho, vel[0], vel[1], vel[2], p])
 
 
-def vortex_case(n=16, flux="roe", second_order=False, limiter=False, rcm=True) -> Case:
+def vortex_case(n=16, flux="roe", second_order=False, limiter=False, order="rcm") -> Case:
     """The periodic isentropic vortex: n^3 hexahedra of [-10, 10]^3 (the unit cube mapped by
     x' = 20 x - 10), freestream Mach 0.5 along x, strength 5."""
-    m = meshes.periodic_hex(n, -10.0, 10.0, rcm=rcm)
+    m = meshes.periodic_hex(n, -10.0, 10.0, order=order)
     sc = freestream_scheme(1.0, (0.5, 0.0, 0.0), 1.0 / GAMMA, flux=flux, second_order=second_order, limiter=limiter)
     sc.gas_constant = 1.0 / GAMMA
     u = vortex_state(m.centroids())
     return Case("vortex", m, sc, u.reshape(-1), np.zeros(0), np.zeros((5, 0)))
 
 
-def tets_case(n=6, flux="hllc", second_order=True, limiter=True, venkat_k=5.0, rcm=True, source=True) -> Case:
+def tets_case(n=6, flux="hllc", second_order=True, limiter=True, venkat_k=5.0, order="rcm", source=True) -> Case:
     """Kuhn tetrahedra of the jittered unit cube, every boundary face a `state` face holding the
     exact field, with a volume source."""
-    m = meshes.kuhn_tets(n, 0.1, 0, rcm=rcm)
+    m = meshes.kuhn_tets(n, 0.1, 0, order=order)
     w_free = np.array([1.0, 0.3 * math.sqrt(GAMMA * (1 / GAMMA) / 1.0), 0.0, 0.0, 1.0 / GAMMA])
     sc = freestream_scheme(1.0, w_free[1:4], w_free[4], flux=flux, second_order=second_order, limiter=limiter)
     sc.venkat_k = venkat_k
@@ -122,10 +122,10 @@ def tets_case(n=6, flux="hllc", second_order=True, limiter=True, venkat_k=5.0, r
     return Case("tets", m, sc, u.reshape(-1), kinds.astype(np.float64), bc, src)
 
 
-def fixture_case(name: str, flux="ausm+up", second_order=True, limiter=True, kinds_by_zone=None, rcm=False) -> Case:
+def fixture_case(name: str, flux="ausm+up", second_order=True, limiter=True, kinds_by_zone=None, order=None) -> Case:
     """A reference mesh fixture (OpenFOAM polyMesh under tests/golden/meshes) with a condition per
     boundary zone."""
-    m = meshes.read_polymesh(GOLDEN / "meshes" / name, rcm=rcm)
+    m = meshes.read_polymesh(GOLDEN / "meshes" / name, order=order)
     w_free = np.array([1.0, 0.4, 0.1, -0.05, 1.0 / GAMMA])
     sc = freestream_scheme(1.0, w_free[1:4], w_free[4], flux=flux, second_order=second_order, limiter=limiter)
     if kinds_by_zone is None:
