@@ -79,14 +79,27 @@ struct Dev {
   // Block-local faces: the faces of the cells of logical block b are sface[sbeg[b] .. sbeg[b + 1])
   // (ascending); incidence k of a cell reads its face's flux from LDS slot islot[k] & 0x3fff, bit
   // 14 set where the cell is the face's neighbour, bit 15 on boundary faces.
+  const int* cbeg;  // cells of face block b: [cbeg[b], cbeg[b + 1])
   const int* sbeg;
   const int* sface;
   const unsigned short* islot;
   int max_slots;
   long long* nslots;  // kNormSlots x kSums x kWords exact-sum words, spread to avoid contention
+  // Staged gradient: block b's table of cells then boundary faces tent[tbeg[b] .. tbeg[b + 1]),
+  // sloc the table position of every sliced stencil entry, lens = stencil length | incidences
+  // << 16, and the limiter's offsets of each incidence (ir0, ir1, ir2) sliced from ioff.
+  const int* tbeg;
+  const int* tent;
+  const unsigned short* sloc;
+  const int* lens;
+  const long long* ioff;
+  const C* ir0;
+  const C* ir1;
+  const C* ir2;
 };
 
 constexpr int kNormSlots = 64;
+constexpr int kMaxSlots = 2 * kBlock;  // faces per face block
 
 // Workgroups are handed to the eight XCDs in turn (workgroup b to XCD b mod 8), each XCD with its
 // own L2. Remapped so that XCD k takes a contiguous run of logical blocks: the cells an XCD works
@@ -288,6 +301,94 @@ __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d, Params<C> pa) {
   store_n<C, 20>(d.rec + 20LL * c, out);
 }
 
+// The same gradient and limiter with the block's stencil values staged in LDS: the block's
+// table of cells and boundary faces is read once, in ascending order (runs of consecutive
+// records), then every stencil entry gathers its primitives from LDS through a 16-bit position.
+// Weights, positions and the limiter's face offsets stream in the wave-sliced layout. The sums,
+// bounds and minima run in the same order as k_grad.
+template <class C, bool LIMITER>
+__global__ __launch_bounds__(kBlock) void k_grad_lds(Dev<C> d, Params<C> pa) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* tab = reinterpret_cast<C*>(smem);
+  const int blk = xcd_block();
+  const int t0 = d.tbeg[blk], tn = d.tbeg[blk + 1] - t0;
+  for (int j = threadIdx.x; j < tn; j += kBlock) {
+    const int e = d.tent[t0 + j];
+    const W5<C> w = e >= 0 ? load_w(d.prim, e) : load_w(d.prim_b, -e - 1);
+    tab[j] = w.r;
+    tab[tn + j] = w.u;
+    tab[2 * tn + j] = w.v;
+    tab[3 * tn + j] = w.w;
+    tab[4 * tn + j] = w.p;
+  }
+  __syncthreads();
+  const int c = blk * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  const W5<C> wc = load_w(d.prim, c);
+  const C wcv[5] = {wc.r, wc.u, wc.v, wc.w, wc.p};
+  double acc[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  C lo[5], hi[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    lo[q] = infinity<C>();
+    hi[q] = -infinity<C>();
+  }
+  const int lens = d.lens[c];
+  const int len = lens & 0xffff;
+  const long long base = d.soff[c / kWave] + (c % kWave);
+  for (int j = 0; j < len; ++j) {
+    const long long k = base + static_cast<long long>(j) * kWave;
+    const int l = d.sloc[k];
+    const C w0 = d.sw0[k], w1 = d.sw1[k], w2 = d.sw2[k];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const C wn = tab[i * tn + l];
+      const C dd = wn - wcv[i];
+      acc[3 * i] += static_cast<double>(dd * w0);
+      acc[3 * i + 1] += static_cast<double>(dd * w1);
+      acc[3 * i + 2] += static_cast<double>(dd * w2);
+      if (wn < lo[i]) lo[i] = wn;
+      if (wn > hi[i]) hi[i] = wn;
+    }
+  }
+  C out[20];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) out[q] = static_cast<C>(acc[q]);
+  if constexpr (LIMITER) {
+    C phi[5], eps2[5];
+    const C vol = d.vol[c];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      lo[i] = smin(lo[i], wcv[i]);
+      hi[i] = smax(hi[i], wcv[i]);
+      phi[i] = infinity<C>();
+      eps2[i] = pa.scale[i] * vol;
+    }
+    const int ilen = lens >> 16;
+    const long long ib = d.ioff[c / kWave] + (c % kWave);
+    for (int j = 0; j < ilen; ++j) {
+      const long long k = ib + static_cast<long long>(j) * kWave;
+      const C r0 = d.ir0[k], r1 = d.ir1[k], r2 = d.ir2[k];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const C dd = out[3 * i] * r0 + out[3 * i + 1] * r1 + out[3 * i + 2] * r2;
+        const C dmax = hi[i] - wcv[i], dmin = lo[i] - wcv[i];
+        const C dp = dd > C(0.0) ? dmax : dmin;
+        const C v = (dp * dp + eps2[i] + C(2.0) * dd * dp) / (dp * dp + C(2.0) * dd * dd + dd * dp + eps2[i]);
+        if (v < phi[i]) phi[i] = v;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[15 + i] = smin(phi[i], C(1.0));
+  } else {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) out[15 + i] = C(1.0);
+  }
+  store_n<C, 20>(d.rec + 20LL * c, out);
+}
+
 // ------------------------------------------------------------------------ face fluxes
 
 template <class C>
@@ -297,8 +398,12 @@ __device__ __forceinline__ void load_rec(const C* __restrict__ rec, int c, C* g)
 
 // {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for face f: the
 // area-weighted flux in C.
-template <class C, int FLUX, bool PLANAR, bool SECOND>
-__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f) {
+// With LAMBDA it also leaves the face's convective spectral radius (kernels/euler.py:1091-1112)
+// as each side takes it: lam[0] = max(s_owner, s_other) A for the owner, lam[1] = max(s_other,
+// s_owner) A for the neighbour, s = |v . n| + c of the cells' (not reconstructed) primitives and
+// of the boundary state.
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool LAMBDA = false>
+__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f, C* lam = nullptr) {
   C geo[8];
   load_n<C, 8>(d.fgeo + 8LL * f, geo);
   const V3<C> n{geo[0], geo[1], geo[2]};
@@ -307,6 +412,11 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
   if (f < d.ni) {
     const int2 lr = d.lr[f];
     W5<C> wl = load_w(d.prim, lr.x), wr = load_w(d.prim, lr.y);
+    if constexpr (LAMBDA) {
+      const C sl = wave_speed(wl, n, pa.g, pa.pref), sr = wave_speed(wr, n, pa.g, pa.pref);
+      lam[0] = smax(sl, sr) * area;
+      lam[1] = smax(sr, sl) * area;
+    }
     C gl[20], gr[20];
     W5<C> fl = wl, fr = wr;
     if constexpr (SECOND) {
@@ -337,6 +447,11 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
   } else {
     const int b = f - d.ni;
     const W5<C> wl = load_w(d.prim, d.bowner[b]);
+    if constexpr (LAMBDA) {
+      const C sl = wave_speed(wl, n, pa.g, pa.pref), sb = wave_speed(load_w(d.prim_b, b), n, pa.g, pa.pref);
+      lam[0] = smax(sl, sb) * area;
+      lam[1] = lam[0];
+    }
     C gl[20];
     W5<C> wf = wl;
     if constexpr (SECOND || !PLANAR) {
@@ -686,32 +801,9 @@ __global__ void k_clear(Small* small, long long* nslots) {
   }
 }
 
-// The convective spectral radius of face f for each side (kernels/euler.py:1091-1112): the
-// larger of the two sides' wave speeds times the area, taken as the owner takes it (own first)
-// and as the neighbour takes it, so that the max's tie and NaN rules stay those of each cell.
-template <class C>
-__device__ __forceinline__ void face_lambda(const Dev<C>& d, const Params<C>& pa, int f, C& for_owner,
-                                            C& for_neighbour) {
-  C geo[4];
-  load_n<C, 4>(d.fgeo + 8LL * f, geo);
-  const V3<C> n{geo[0], geo[1], geo[2]};
-  W5<C> wl, wr;
-  if (f < d.ni) {
-    const int2 lr = d.lr[f];
-    wl = load_w(d.prim, lr.x);
-    wr = load_w(d.prim, lr.y);
-  } else {
-    wl = load_w(d.prim, d.bowner[f - d.ni]);
-    wr = load_w(d.prim_b, f - d.ni);
-  }
-  const C ll = wave_speed(wl, n, pa.g, pa.pref);
-  const C lr = wave_speed(wr, n, pa.g, pa.pref);
-  for_owner = smax(ll, lr) * geo[3];
-  for_neighbour = smax(lr, ll) * geo[3];
-}
-
-// The faces and boundaries loops fused with their gather, one logical block of 256 cells per
-// workgroup. Phase 1: every face of the block's cells (its interior faces once, the faces it
+// The faces and boundaries loops fused with their gather, one block of at most 256 consecutive
+// cells and at most kMaxSlots faces per workgroup (two full rounds of its threads). Phase 1:
+// every face of the block's cells (its interior faces once, the faces it
 // shares with another block here and there too) evaluates its flux into LDS. Phase 2: each cell
 // adds its faces' fluxes in its cell_faces order, interior faces then boundary faces, each set
 // from zero in FP64, onto -V s (kernels/euler.py:611-616): the store_gather order of the
@@ -733,19 +825,18 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double
   }
   for (int i = threadIdx.x; i < ns; i += kBlock) {
     const int f = d.sface[s0 + i];
-    const F5<C> fx = face_flux<C, FLUX, PLANAR, SECOND>(d, pa, f);
+    C lam[2];
+    const F5<C> fx = face_flux<C, FLUX, PLANAR, SECOND, FIRST>(d, pa, f, lam);
 #pragma unroll
     for (int q = 0; q < 5; ++q) lf[q * ms + i] = fx.f[q];
     if constexpr (FIRST) {
-      C lo, ln;
-      face_lambda(d, pa, f, lo, ln);
-      lf[5 * ms + i] = lo;
-      lf[6 * ms + i] = ln;
+      lf[5 * ms + i] = lam[0];
+      lf[6 * ms + i] = lam[1];
     }
   }
   __syncthreads();
-  const int c = blk * kBlock + threadIdx.x;
-  const bool active = c < d.nc;
+  const int c = d.cbeg[blk] + threadIdx.x;
+  const bool active = c < d.cbeg[blk + 1];
   double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double lam_acc = 0.0;
   if (active) {

@@ -140,7 +140,7 @@ class Stepper final : public StepperBase {
     lam_ = Buffer<C>(std::size_t(nc_));
     dt_ = Buffer<double>(std::size_t(nc_));
     small_ = Buffer<Small>(1);
-    partial_ = Buffer<unsigned long long>(std::size_t(grid(nc_)));
+    partial_ = Buffer<unsigned long long>(std::size_t(std::max(grid(nc_), xcd_grid(face_blocks_))));
     nslots_ = Buffer<long long>(std::size_t(kNormSlots) * kSums * kWords);
     check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small), hipHostMallocDefault), "hipHostMalloc");
     check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
@@ -325,11 +325,20 @@ class Stepper final : public StepperBase {
     d.lam = lam_.get();
     d.dt = dt_.get();
     d.small = small_.get();
+    d.cbeg = cbeg_.get();
     d.sbeg = sbeg_.get();
     d.sface = sface_.get();
     d.islot = islot_.get();
     d.max_slots = max_slots_;
     d.nslots = nslots_.get();
+    d.tbeg = tbeg_.get();
+    d.tent = tent_.get();
+    d.sloc = sloc_.get();
+    d.lens = lens_.get();
+    d.ioff = ioff_.get();
+    d.ir0 = ir0_.get();
+    d.ir1 = ir1_.get();
+    d.ir2 = ir2_.get();
     return d;
   }
 
@@ -431,41 +440,62 @@ class Stepper final : public StepperBase {
     check(hipStreamSynchronize(stream_), "topology");
   }
 
-  // The block-local face lists of the fused face kernel: for each logical block of kBlock cells,
-  // its cells' faces ascending, and for every incidence the LDS slot of its face with the cell's
+  // The face blocks of the fused face kernel: consecutive cells, at most kBlock of them and at
+  // most kMaxSlots distinct faces, so that phase 1 runs two full rounds of the workgroup. For each
+  // block its faces ascending, and for every incidence the LDS slot of its face with the cell's
   // side and whether the face is a boundary face.
   void build_slots(const mxeuler_topology& t, const std::vector<int>& cfo) {
-    const int nblk = grid(nc_);
-    std::vector<int> sbeg(std::size_t(nblk) + 1, 0);
-    std::vector<int> sface;
+    std::vector<int> cbeg{0}, sbeg{0}, sface;
     sface.reserve(std::size_t(cfo.back()) / 2 + std::size_t(nc_));
     std::vector<unsigned short> islot(std::size_t(std::max(cfo.back(), 1)), 0);
-    std::vector<int> faces;
+    std::vector<int> mark(std::size_t(nf_), -1), faces;
     int max_slots = 0;
     bool fits = true;
-    for (int b = 0; b < nblk; ++b) {
-      const int c0 = std::min(nc_, b * kBlock), c1 = std::min(nc_, (b + 1) * kBlock);
-      faces.assign(t.cell_faces + cfo[std::size_t(c0)], t.cell_faces + cfo[std::size_t(c1)]);
-      std::sort(faces.begin(), faces.end());
-      faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
-      max_slots = std::max(max_slots, static_cast<int>(faces.size()));
-      if (faces.size() > 0x3fff) fits = false;
-      for (int c = c0; c < c1 && fits; ++c) {
+    int c = 0;
+    while (c < nc_) {
+      const int b = static_cast<int>(cbeg.size()) - 1;
+      faces.clear();
+      const int c0 = c;
+      while (c < nc_ && c - c0 < kBlock) {
+        int fresh = 0;
+        for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) fresh += mark[std::size_t(t.cell_faces[k])] != b;
+        if (static_cast<int>(faces.size()) + fresh > kMaxSlots && c > c0) break;
         for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) {
           const int f = t.cell_faces[k];
+          if (mark[std::size_t(f)] != b) {
+            mark[std::size_t(f)] = b;
+            faces.push_back(f);
+          }
+        }
+        ++c;
+      }
+      std::sort(faces.begin(), faces.end());
+      max_slots = std::max(max_slots, static_cast<int>(faces.size()));
+      if (faces.size() > 0x3fff) fits = false;
+      for (int cc = c0; cc < c && fits; ++cc) {
+        for (int k = cfo[std::size_t(cc)]; k < cfo[std::size_t(cc) + 1]; ++k) {
+          const int f = t.cell_faces[k];
           const auto slot = static_cast<unsigned>(std::lower_bound(faces.begin(), faces.end(), f) - faces.begin());
-          const bool right = f < ni_ && t.owner[f] != c;
+          const bool right = f < ni_ && t.owner[f] != cc;
           const bool boundary = f >= ni_;
           islot[std::size_t(k)] = static_cast<unsigned short>(slot | (right ? 0x4000u : 0u) | (boundary ? 0x8000u : 0u));
         }
       }
       sface.insert(sface.end(), faces.begin(), faces.end());
-      sbeg[std::size_t(b) + 1] = static_cast<int>(sface.size());
+      sbeg.push_back(static_cast<int>(sface.size()));
+      cbeg.push_back(c);
+    }
+    face_blocks_ = static_cast<int>(cbeg.size()) - 1;
+    // Pad to the XCD-aware grid: the extra blocks hold no cell and no face.
+    while (static_cast<int>(cbeg.size()) < xcd_grid(face_blocks_) + 1) {
+      cbeg.push_back(nc_);
+      sbeg.push_back(sbeg.back());
     }
     max_slots_ = max_slots;
     fused_ = fits && faces_lds(true) <= 65536;
     if (!fused_) return;
     if (sface.empty()) sface.push_back(0);
+    cbeg_.upload(cbeg, stream_);
     sbeg_.upload(sbeg, stream_);
     sface_.upload(sface, stream_);
     islot_.upload(islot, stream_);
@@ -509,6 +539,8 @@ class Stepper final : public StepperBase {
         w2[at] = static_cast<C>(st.weights[2 * ns + k]);
       }
     }
+    build_tables(soff, slen, idx);
+    slen_host_ = slen;
     soff_.upload(soff, stream_);
     slen_.upload(slen, stream_);
     sidx_.upload(idx, stream_);
@@ -516,6 +548,96 @@ class Stepper final : public StepperBase {
     sw1_.upload(w1, stream_);
     sw2_.upload(w2, stream_);
     check(hipStreamSynchronize(stream_), "stencil");
+  }
+
+  // The staged gradient's tables: logical block b's cells read, through their stencils, the
+  // cells tent[tbeg[b] .. ) (ascending) then the boundary faces -(b + 1) (ascending); every sliced
+  // stencil entry holds its position in that list (16 bits), so the kernel loads the list once
+  // into LDS, in order, and gathers from LDS.
+  void build_tables(const std::vector<long long>& soff, const std::vector<int>& slen, const std::vector<int>& idx) {
+    const int nblk = grid(nc_);
+    std::vector<int> tbeg(std::size_t(nblk) + 1, 0), tent;
+    tent.reserve(std::size_t(nc_) * 3);
+    std::vector<unsigned short> sloc(idx.size(), 0);
+    std::vector<int> cells, bnd;
+    int max_table = 0;
+    for (int b = 0; b < nblk; ++b) {
+      const int c0 = std::min(nc_, b * kBlock), c1 = std::min(nc_, (b + 1) * kBlock);
+      cells.clear();
+      bnd.clear();
+      for (int c = c0; c < c1; ++c) {
+        const long long base = soff[std::size_t(c / kWave)] + c % kWave;
+        for (int j = 0; j < slen[std::size_t(c)]; ++j) {
+          const int e = idx[std::size_t(base + static_cast<long long>(j) * kWave)];
+          (e >= 0 ? cells : bnd).push_back(e >= 0 ? e : -e - 1);
+        }
+      }
+      std::sort(cells.begin(), cells.end());
+      cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+      std::sort(bnd.begin(), bnd.end());
+      bnd.erase(std::unique(bnd.begin(), bnd.end()), bnd.end());
+      const int n = static_cast<int>(cells.size() + bnd.size());
+      max_table = std::max(max_table, n);
+      for (int c = c0; c < c1 && n < 65536; ++c) {
+        const long long base = soff[std::size_t(c / kWave)] + c % kWave;
+        for (int j = 0; j < slen[std::size_t(c)]; ++j) {
+          const std::size_t at = std::size_t(base + static_cast<long long>(j) * kWave);
+          const int e = idx[at];
+          const long long pos = e >= 0 ? std::lower_bound(cells.begin(), cells.end(), e) - cells.begin()
+                                       : static_cast<long long>(cells.size()) +
+                                             (std::lower_bound(bnd.begin(), bnd.end(), -e - 1) - bnd.begin());
+          sloc[at] = static_cast<unsigned short>(pos);
+        }
+      }
+      tent.insert(tent.end(), cells.begin(), cells.end());
+      for (const int bi : bnd) tent.push_back(-bi - 1);
+      tbeg[std::size_t(b) + 1] = static_cast<int>(tent.size());
+    }
+    max_table_ = max_table;
+    staged_ = max_table < 65536 && std::size_t(max_table) * 5 * sizeof(C) <= 65536;
+    if (!staged_) return;
+    if (tent.empty()) tent.push_back(0);
+    tbeg_.upload(tbeg, stream_);
+    tent_.upload(tent, stream_);
+    sloc_.upload(sloc, stream_);
+  }
+
+  // The limiter's per-incidence offsets r = rel(x_f, x_c) of the cell's own side, sliced like the
+  // stencil (incidence j of cell 64 s + l at ioff[s] + 64 j + l), and the per-cell counts
+  // lens = stencil length | incidences << 16.
+  void build_incidence_slices(const mxeuler_topology& t, const mxeuler_geometry& g, const std::vector<int>& slen) {
+    const int slices = (nc_ + kWave - 1) / kWave;
+    const std::size_t nf = std::size_t(nf_);
+    std::vector<long long> ioff(std::size_t(slices) + 1, 0);
+    std::vector<int> lens(std::size_t(nc_), 0);
+    for (int s = 0; s < slices; ++s) {
+      int longest = 0;
+      for (int l = 0; l < kWave && s * kWave + l < nc_; ++l) {
+        const int c = s * kWave + l;
+        const int n = static_cast<int>(t.cell_face_offsets[c + 1] - t.cell_face_offsets[c]);
+        lens[std::size_t(c)] = slen[std::size_t(c)] | (n << 16);
+        longest = std::max(longest, n);
+      }
+      ioff[std::size_t(s) + 1] = ioff[std::size_t(s)] + static_cast<long long>(longest) * kWave;
+    }
+    const std::size_t total = std::max<std::size_t>(std::size_t(ioff.back()), 1);
+    std::vector<C> r0(total, C(0)), r1(total, C(0)), r2(total, C(0));
+    for (int c = 0; c < nc_; ++c) {
+      const long long base = ioff[std::size_t(c / kWave)] + c % kWave;
+      for (long long j = 0; j < t.cell_face_offsets[c + 1] - t.cell_face_offsets[c]; ++j) {
+        const std::size_t f = std::size_t(t.cell_faces[t.cell_face_offsets[c] + j]);
+        const double* r = t.owner[f] == c ? g.r_left : g.r_right;
+        const std::size_t at = std::size_t(base + j * kWave);
+        r0[at] = static_cast<C>(r[f]);
+        r1[at] = static_cast<C>(r[nf + f]);
+        r2[at] = static_cast<C>(r[2 * nf + f]);
+      }
+    }
+    ioff_.upload(ioff, stream_);
+    lens_.upload(lens, stream_);
+    ir0_.upload(r0, stream_);
+    ir1_.upload(r1, stream_);
+    ir2_.upload(r2, stream_);
   }
 
   void build_geometry(const mxeuler_topology& t, const mxeuler_geometry& g) {
@@ -542,7 +664,9 @@ class Stepper final : public StepperBase {
     for (std::size_t c = 0; c < nc; ++c) vol[c] = static_cast<C>(g.cell_volume[c]);
     vol_.upload(vol, stream_);
     vol64_.upload(vol64, stream_);
-    (void)t;
+    build_incidence_slices(t, g, slen_host_);
+    slen_host_.clear();
+    slen_host_.shrink_to_fit();
     check(hipStreamSynchronize(stream_), "geometry");
   }
 
@@ -596,7 +720,12 @@ class Stepper final : public StepperBase {
     const Dev<C> d = dev();
     if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d, pa_); });
     if (second_) {
-      if (sc_.limiter) {
+      const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
+      if (staged_ && sc_.limiter) {
+        launch("grad_limiter", [&] { k_grad_lds<C, true><<<grid(nc_), kBlock, lds, stream_>>>(d, pa_); });
+      } else if (staged_) {
+        launch("grad", [&] { k_grad_lds<C, false><<<grid(nc_), kBlock, lds, stream_>>>(d, pa_); });
+      } else if (sc_.limiter) {
         launch("grad_limiter", [&] { k_grad<C, true><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
       } else {
         launch("grad", [&] { k_grad<C, false><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
@@ -606,7 +735,7 @@ class Stepper final : public StepperBase {
       launch(first ? "faces_first" : "faces", [&] { launch_faces(d, first, cfl, local); });
       if (first) {
         launch("step_reduce", [&] {
-          k_step_reduce<<<1, 1024, 0, stream_>>>(nslots_.get(), partial_.get(), grid(nc_), small_.get());
+          k_step_reduce<<<1, 1024, 0, stream_>>>(nslots_.get(), partial_.get(), xcd_grid(face_blocks_), small_.get());
         });
       }
       return;
@@ -621,7 +750,7 @@ class Stepper final : public StepperBase {
 
   template <int FLUX, bool PLANAR, bool SECOND>
   void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
-    const int g = grid(nc_);
+    const int g = xcd_grid(face_blocks_);
     if (first) {
       k_faces<C, FLUX, PLANAR, SECOND, true><<<g, kBlock, faces_lds(true), stream_>>>(d, pa_, cfl, local ? 1 : 0,
                                                                                    partial_.get());
@@ -790,7 +919,15 @@ class Stepper final : public StepperBase {
   Buffer<C> prim_, prim_b_, rec_, fflux_, lam_;
   Buffer<Small> small_;
   Buffer<unsigned long long> partial_;
-  Buffer<int> sbeg_, sface_;
+  Buffer<int> cbeg_, sbeg_, sface_;
+  int face_blocks_ = 0;
+  Buffer<int> tbeg_, tent_, lens_;
+  Buffer<unsigned short> sloc_;
+  Buffer<long long> ioff_;
+  Buffer<C> ir0_, ir1_, ir2_;
+  std::vector<int> slen_host_;
+  int max_table_ = 0;
+  bool staged_ = false;
   Buffer<unsigned short> islot_;
   Buffer<long long> nslots_;
   int max_slots_ = 0;
