@@ -59,6 +59,13 @@ int mxmesh_bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, dou
   return guarded(err, err_len, [&] { make(mx::bump_hex(nx, ny, nz, lx, ly, lz, h, w, x0), out); });
 }
 
+int mxmesh_bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
+                       int prism_layers, int band_lo, int band_hi, mxmesh** out, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    make(mx::bump_hybrid(nx, ny, nz, lx, ly, lz, h, w, x0, prism_layers, band_lo, band_hi), out);
+  });
+}
+
 int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_len) {
   return guarded(err, err_len, [&] { make(mx::read_polymesh(dir), out); });
 }

@@ -205,35 +205,52 @@ struct CellFaces {
 template <class ZoneOf>
 RawFaces match_faces(const CellFaces& cf, const ZoneOf& zone_of) {
   const std::size_t n = cf.cell.size();
+  // A face is known by its three smallest nodes, its largest node and its node count; two sides
+  // with equal keys must also hold equal node sets, which the pairing below checks.
   struct Key {
-    index_t a, b, c;
+    index_t a, b, c, z, count;
     std::uint32_t i;
   };
   std::vector<Key> keys(n);
   for (std::size_t i = 0; i < n; ++i) {
     std::array<index_t, 3> s{INT32_MAX, INT32_MAX, INT32_MAX};
+    index_t z = -1;
     for (auto k = cf.off[i]; k < cf.off[i + 1]; ++k) {
       index_t v = cf.nodes[static_cast<std::size_t>(k)];
+      z = std::max(z, v);
       for (int q = 0; q < 3; ++q) {
         if (v < s[q]) std::swap(v, s[q]);
       }
     }
-    keys[i] = {s[0], s[1], s[2], static_cast<std::uint32_t>(i)};
+    keys[i] = {s[0], s[1], s[2], z, static_cast<index_t>(cf.off[i + 1] - cf.off[i]), static_cast<std::uint32_t>(i)};
   }
+  auto same = [](const Key& p, const Key& q) {
+    return p.a == q.a && p.b == q.b && p.c == q.c && p.z == q.z && p.count == q.count;
+  };
   std::sort(keys.begin(), keys.end(), [](const Key& p, const Key& q) {
     if (p.a != q.a) return p.a < q.a;
     if (p.b != q.b) return p.b < q.b;
     if (p.c != q.c) return p.c < q.c;
+    if (p.z != q.z) return p.z < q.z;
+    if (p.count != q.count) return p.count < q.count;
     return p.i < q.i;
   });
+  auto sorted_nodes = [&](std::size_t i) {
+    std::vector<index_t> v(cf.nodes.begin() + cf.off[i], cf.nodes.begin() + cf.off[i + 1]);
+    std::sort(v.begin(), v.end());
+    return v;
+  };
   RawFaces out;
   out.off.reserve(n / 2 + 16);
   for (std::size_t k = 0; k < n;) {
     const Key& p = keys[k];
     const std::size_t i = p.i;
-    const bool pair = k + 1 < n && keys[k + 1].a == p.a && keys[k + 1].b == p.b && keys[k + 1].c == p.c;
-    if (pair && k + 2 < n && keys[k + 2].a == p.a && keys[k + 2].b == p.b && keys[k + 2].c == p.c) {
+    const bool pair = k + 1 < n && same(keys[k + 1], p);
+    if (pair && k + 2 < n && same(keys[k + 2], p)) {
       throw std::runtime_error("match_faces: a face shared by more than two cells");
+    }
+    if (pair && p.count > 3 && sorted_nodes(i) != sorted_nodes(keys[k + 1].i)) {
+      throw std::runtime_error("match_faces: two distinct faces share a key");
     }
     out.nodes.insert(out.nodes.end(), cf.nodes.begin() + cf.off[i], cf.nodes.begin() + cf.off[i + 1]);
     out.off.push_back(static_cast<offset_t>(out.nodes.size()));
@@ -376,6 +393,159 @@ Mesh bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, double h,
     }
   }
   return assemble(static_cast<index_t>(nx) * ny * nz, std::move(x), std::move(y), std::move(z), std::move(f));
+}
+
+Mesh bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
+                 int prism_layers, int band_lo, int band_hi) {
+  if (nx < 2 || ny < 2 || nz < 2 || nx % 2 || nz % 2 || (band_hi - band_lo) % 2 || band_lo < prism_layers ||
+      band_hi > ny || prism_layers < 0 || band_lo > band_hi) {
+    throw std::invalid_argument("bump_hybrid: nx, nz and the band height even, prisms below the band");
+  }
+  const int px = nx + 1, py = ny + 1, pz = nz + 1;
+  const std::size_t np = static_cast<std::size_t>(px) * py * pz;
+  std::vector<double> x(np), y(np), z(np);
+  for (int k = 0; k < pz; ++k) {
+    for (int j = 0; j < py; ++j) {
+      for (int i = 0; i < px; ++i) {
+        const std::size_t v = static_cast<std::size_t>(i + px * (j + py * k));
+        const double xi = lx * i / nx;
+        const double t = (xi - x0) / w;
+        const double wall = h * std::exp(-t * t);
+        x[v] = xi;
+        y[v] = wall + (ly - wall) * j / ny;
+        z[v] = lz * k / nz;
+      }
+    }
+  }
+  auto node = [&](int i, int j, int k) { return static_cast<index_t>(i + px * (j + py * k)); };
+  auto in_band = [&](int j) { return j >= band_lo && j < band_hi; };
+  // Cell ids: prisms (two per hex of the prism rows), merged cells, then hexes.
+  CellFaces cf;
+  index_t nc = 0;
+  // Outward orientation by Newell's normal against the vector from the cell centre.
+  std::vector<index_t> poly;
+  double ccx = 0, ccy = 0, ccz = 0;
+  auto emit = [&](index_t c) {
+    double nx_ = 0, ny_ = 0, nz_ = 0, fx = 0, fy = 0, fz = 0;
+    const std::size_t n = poly.size();
+    for (std::size_t a = 0; a < n; ++a) {
+      const auto p = static_cast<std::size_t>(poly[a]), q = static_cast<std::size_t>(poly[(a + 1) % n]);
+      nx_ += (y[p] - y[q]) * (z[p] + z[q]);
+      ny_ += (z[p] - z[q]) * (x[p] + x[q]);
+      nz_ += (x[p] - x[q]) * (y[p] + y[q]);
+      fx += x[p];
+      fy += y[p];
+      fz += z[p];
+    }
+    const double d = nx_ * (fx / n - ccx) + ny_ * (fy / n - ccy) + nz_ * (fz / n - ccz);
+    if (d < 0) std::reverse(poly.begin(), poly.end());
+    cf.nodes.insert(cf.nodes.end(), poly.begin(), poly.end());
+    cf.off.push_back(static_cast<offset_t>(cf.nodes.size()));
+    cf.cell.push_back(c);
+  };
+  auto centre = [&](std::initializer_list<index_t> nodes) {
+    ccx = ccy = ccz = 0;
+    for (const index_t v : nodes) {
+      ccx += x[at(v)];
+      ccy += y[at(v)];
+      ccz += z[at(v)];
+    }
+    ccx /= static_cast<double>(nodes.size());
+    ccy /= static_cast<double>(nodes.size());
+    ccz /= static_cast<double>(nodes.size());
+  };
+  for (int k = 0; k < nz; ++k) {
+    for (int j = 0; j < ny; ++j) {
+      for (int i = 0; i < nx; ++i) {
+        const index_t a000 = node(i, j, k), a100 = node(i + 1, j, k), a010 = node(i, j + 1, k),
+                      a110 = node(i + 1, j + 1, k), a001 = node(i, j, k + 1), a101 = node(i + 1, j, k + 1),
+                      a011 = node(i, j + 1, k + 1), a111 = node(i + 1, j + 1, k + 1);
+        if (j < prism_layers) {
+          // Two prisms: the x-y quad split along (i, j) - (i+1, j+1), extruded over [k, k+1].
+          const index_t c0 = nc++, c1 = nc++;
+          centre({a000, a100, a110, a001, a101, a111});
+          poly = {a000, a100, a110}; emit(c0);
+          poly = {a001, a101, a111}; emit(c0);
+          poly = {a000, a100, a101, a001}; emit(c0);
+          poly = {a100, a110, a111, a101}; emit(c0);
+          poly = {a000, a110, a111, a001}; emit(c0);
+          centre({a000, a110, a010, a001, a111, a011});
+          poly = {a000, a110, a010}; emit(c1);
+          poly = {a001, a111, a011}; emit(c1);
+          poly = {a000, a110, a111, a001}; emit(c1);
+          poly = {a110, a010, a011, a111}; emit(c1);
+          poly = {a010, a000, a001, a011}; emit(c1);
+          continue;
+        }
+        if (in_band(j)) {
+          if (i % 2 || (j - band_lo) % 2 || k % 2) continue;
+          // One merged cell over [i, i+2] x [j, j+2] x [k, k+2]; a side facing another merged cell
+          // is the 8-node ring of its 2 x 2 quads, else its 4 quads.
+          const index_t c = nc++;
+          centre({node(i, j, k), node(i + 2, j, k), node(i, j + 2, k), node(i + 2, j + 2, k), node(i, j, k + 2),
+                  node(i + 2, j, k + 2), node(i, j + 2, k + 2), node(i + 2, j + 2, k + 2)});
+          // side: axis (0 x, 1 y, 2 z), at the low (0) or high (2) end.
+          for (int axis = 0; axis < 3; ++axis) {
+            for (int end = 0; end <= 2; end += 2) {
+              auto at3 = [&](int u, int v) {
+                int q[3];
+                q[axis] = end;
+                q[(axis + 1) % 3] = u;
+                q[(axis + 2) % 3] = v;
+                return node(i + q[0], j + q[1], k + q[2]);
+              };
+              bool merged_nb;
+              if (axis == 0) {
+                merged_nb = end == 0 ? i > 0 : i + 2 < nx;
+              } else if (axis == 1) {
+                merged_nb = end == 0 ? j > band_lo : j + 2 < band_hi;
+              } else {
+                merged_nb = end == 0 ? k > 0 : k + 2 < nz;
+              }
+              if (merged_nb) {
+                poly = {at3(0, 0), at3(1, 0), at3(2, 0), at3(2, 1), at3(2, 2), at3(1, 2), at3(0, 2), at3(0, 1)};
+                emit(c);
+              } else {
+                for (int u = 0; u < 2; ++u) {
+                  for (int v = 0; v < 2; ++v) {
+                    poly = {at3(u, v), at3(u + 1, v), at3(u + 1, v + 1), at3(u, v + 1)};
+                    emit(c);
+                  }
+                }
+              }
+            }
+          }
+          continue;
+        }
+        const index_t c = nc++;
+        centre({a000, a100, a010, a110, a001, a101, a011, a111});
+        poly = {a000, a001, a011, a010}; emit(c);
+        poly = {a100, a110, a111, a101}; emit(c);
+        poly = {a000, a100, a101, a001}; emit(c);
+        poly = {a010, a011, a111, a110}; emit(c);
+        poly = {a000, a010, a110, a100}; emit(c);
+        poly = {a001, a101, a111, a011}; emit(c);
+      }
+    }
+  }
+  auto zone_of = [&](const index_t* v, std::size_t count) {
+    auto all = [&](auto pred) {
+      for (std::size_t q = 0; q < count; ++q) {
+        const int vi = v[q] % px, vj = (v[q] / px) % py, vk = v[q] / (px * py);
+        if (!pred(vi, vj, vk)) return false;
+      }
+      return true;
+    };
+    if (all([](int i, int, int) { return i == 0; })) return 0;
+    if (all([&](int i, int, int) { return i == nx; })) return 1;
+    if (all([](int, int j, int) { return j == 0; })) return 2;
+    if (all([&](int, int j, int) { return j == ny; })) return 3;
+    if (all([](int, int, int k) { return k == 0; })) return 4;
+    return 5;
+  };
+  RawFaces faces = match_faces(cf, zone_of);
+  cf = CellFaces();
+  return assemble(nc, std::move(x), std::move(y), std::move(z), std::move(faces));
 }
 
 // polyMesh ---------------------------------------------------------------------------------------

@@ -143,6 +143,13 @@ Mesh kuhn_tets(int n, double jitter, std::uint64_t seed);
 /// Gaussian bump h exp(-((x - x0)/w)^2) blended linearly to zero at the top. Zones 0..5 = inflow
 /// (xmin), outflow (xmax), lower wall (ymin), upper wall (ymax), zmin, zmax.
 Mesh bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0);
+/// The same channel, hybrid: rows j < prism_layers above the lower wall split into two
+/// triangular prisms each (the x-y quad cut along its diagonal, extruded in z), rows band_lo <= j < band_hi merged 2 x 2 x 2
+/// into polyhedra (nx, nz and the band height even). A face between two merged cells is one
+/// 8-node polygon (its quads' outer ring); a merged cell shows a fine neighbour its 4 quads. Zones
+/// as bump_hex.
+Mesh bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
+                 int prism_layers, int band_lo, int band_hi);
 /// OpenFOAM polyMesh (ASCII) directory: points, faces, owner, neighbour, boundary.
 Mesh read_polymesh(const std::string& dir);
 

@@ -167,6 +167,20 @@ def bump_hex(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, h=0.1, w=0.2, x0=1.5, order="rc
     return _build(lib.mxmesh_bump_hex, *args, order=order, kinds=kinds)
 
 
+def bump_hybrid(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, h=0.1, w=0.2, x0=1.5, prism_layers=2, band=None,
+                order="morton", kinds=None) -> Mesh:
+    """The bump channel with a prism layer on the lower wall and a band of merged polyhedra."""
+    lib = _lib.mesh()
+    if band is None:
+        lo = max(prism_layers, ny // 4)
+        band = (lo, lo + 2 * ((ny - lo) // 4))
+    lo, hi = band
+    args = [ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz)]
+    args += [ctypes.c_double(v) for v in (lx, ly, lz, h, w, x0)]
+    args += [ctypes.c_int(prism_layers), ctypes.c_int(lo), ctypes.c_int(hi)]
+    return _build(lib.mxmesh_bump_hybrid, *args, order=order, kinds=kinds)
+
+
 def read_polymesh(path: str | os.PathLike, order: str | None = None, kinds=None) -> Mesh:
     lib = _lib.mesh()
     return _build(lib.mxmesh_read_polymesh, os.fsencode(str(path)), order=order, kinds=kinds)

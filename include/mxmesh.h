@@ -31,6 +31,10 @@ int mxmesh_kuhn_tets(int n, double jitter, uint64_t seed, mxmesh** out, char* er
 /* nx x ny x nz hexahedra of a channel with a Gaussian bump h exp(-((x - x0) / w)^2) on y = 0. */
 int mxmesh_bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
                     mxmesh** out, char* err, size_t err_len);
+/* The bump channel, hybrid: prism_layers rows of prisms on the lower wall, rows [band_lo, band_hi)
+ * merged 2 x 2 x 2 into polyhedra, hexahedra elsewhere. */
+int mxmesh_bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
+                       int prism_layers, int band_lo, int band_hi, mxmesh** out, char* err, size_t err_len);
 /* An ASCII OpenFOAM polyMesh directory. */
 int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_len);
 void mxmesh_destroy(mxmesh* m);

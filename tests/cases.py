@@ -139,10 +139,10 @@ def fixture_case(name: str, flux="ausm+up", second_order=True, limiter=True, kin
     return Case(name, m, sc, u.reshape(-1), kinds.astype(np.float64), bc, None, uses)
 
 
-def bump_case(nx=24, ny=8, nz=4, flux="ausm+up", second_order=True, limiter=True) -> Case:
-    """The bump channel in hexahedra: farfield inflow and outflow, slip walls top and bottom,
-    slip side planes."""
-    m = meshes.bump_hex(nx, ny, nz)
+def bump_case(nx=24, ny=8, nz=4, flux="ausm+up", second_order=True, limiter=True, hybrid=False) -> Case:
+    """The bump channel, in hexahedra or hybrid (prism wall layer, band of merged polyhedra):
+    farfield inflow and outflow, slip walls top and bottom, slip side planes."""
+    m = meshes.bump_hybrid(nx, ny, nz) if hybrid else meshes.bump_hex(nx, ny, nz)
     mach = 0.5
     c = 1.0
     w_free = np.array([1.0, mach * c, 0.0, 0.0, 1.0 / GAMMA])
@@ -152,7 +152,7 @@ def bump_case(nx=24, ny=8, nz=4, flux="ausm+up", second_order=True, limiter=True
     meshes.set_boundary_kinds(m, kinds)
     bc = boundary_data(kinds, w_free)
     u = smooth_state(m.centroids(), vel=tuple(w_free[1:4]), amp=0.02)
-    return Case("bump", m, sc, u.reshape(-1), kinds.astype(np.float64), bc, None, True)
+    return Case("bump-hybrid" if hybrid else "bump", m, sc, u.reshape(-1), kinds.astype(np.float64), bc, None, True)
 
 
 def normwise_units(case: Case, policy_eps: float) -> np.ndarray:

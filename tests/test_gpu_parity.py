@@ -45,6 +45,8 @@ CASES = {
     "poly-ausm2-warped": lambda: cases.fixture_case("poly", "ausm+up"),
     "poly-roe2-state": lambda: cases.fixture_case("poly", "roe", kinds_by_zone=[meshes.STATE]),
     "bump-ausm2": lambda: cases.bump_case(),
+    "bump-hybrid-ausm2": lambda: cases.bump_case(16, 12, 8, hybrid=True),
+    "bump-hybrid-hllc2": lambda: cases.bump_case(16, 12, 8, "hllc", hybrid=True),
 }
 
 

@@ -33,6 +33,7 @@ def all_meshes():
     yield "kuhn", meshes.kuhn_tets(4)
     yield "kuhn-morton", meshes.kuhn_tets(4, order="morton")
     yield "bump", meshes.bump_hex(12, 4, 3)
+    yield "bump-hybrid", meshes.bump_hybrid(12, 10, 4)
 
 
 @pytest.mark.parametrize("name,m", list(all_meshes()))
@@ -84,6 +85,32 @@ def test_planarity_flags():
     assert fixture("hex").all_planar and fixture("tet-gmsh").all_planar and fixture("hybrid").all_planar
     assert not fixture("poly").all_planar
     assert meshes.kuhn_tets(3).all_planar and meshes.periodic_hex(5).all_planar
+
+
+def test_bump_hybrid_cells():
+    """Prisms on the wall, merged 2 x 2 x 2 polyhedra in the band, hexahedra elsewhere; the cells
+    fill the channel less the bump, and the band's faces between merged cells are 8-node rings,
+    non-planar over the bump."""
+    nx, ny, nz = 12, 10, 4
+    m = meshes.bump_hybrid(nx, ny, nz, prism_layers=2, band=(4, 8), order=None)
+    prisms = 2 * nx * 2 * nz
+    merged = (nx // 2) * 2 * (nz // 2)
+    hexes = nx * (ny - 2 - 4) * nz
+    assert m.n_cells == prisms + merged + hexes
+    nfc = np.diff(m.cell_face_offsets)
+    assert np.sum(nfc == 5) == prisms and np.sum(nfc == 6) == hexes and np.sum(nfc >= 9) == merged
+    nodes = np.diff(m.face_node_offsets)
+    assert set(np.unique(nodes)) == {3, 4, 8}
+    # channel 3 x 1 x 1 less the bump's area, h w sqrt(pi) with h = 0.1, w = 0.2 (sampled)
+    x = np.linspace(0.0, 3.0, nx + 1)
+    wall = 0.1 * np.exp(-(((x - 1.5) / 0.2) ** 2))
+    area = np.sum(0.5 * (wall[1:] + wall[:-1]) * np.diff(x))
+    assert abs(m.cell_volume.sum() - (3.0 - area)) <= 1e-12
+    assert not m.all_planar
+    bad = [(5, 10, 4, 2, (4, 8)), (12, 10, 4, 2, (4, 7)), (12, 10, 4, 5, (4, 8))]
+    for nx_, ny_, nz_, pl, band in bad:
+        with pytest.raises(RuntimeError):
+            meshes.bump_hybrid(nx_, ny_, nz_, prism_layers=pl, band=band)
 
 
 def test_periodic_hex():
