@@ -923,6 +923,47 @@ __global__ __launch_bounds__(1024) void k_step_reduce(const long long* __restric
   }
 }
 
+// ------------------------------------------------------------------------ wall loads
+
+// Euler::boundary_pressure (physics/src/euler.cpp:1187-1223) on the listed boundary faces, in FP64
+// from the stored values: the face pressure of the boundary flux's reconstruction (the cell value
+// where the reconstructed density or pressure is not positive) and the pressure force
+// p_f S + phi_p M^T grad p. rb: FP64 x_f - x_c of every boundary face (3 per face); out: 3 x n SoA.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_face_force(Dev<C> d, Params<C> pa, const int* __restrict__ faces, int n,
+                                                      const double* __restrict__ rb, double* __restrict__ out) {
+  const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
+  if (i >= n) return;
+  const int f = faces[i];
+  const int b = f - d.ni;
+  const long long c = d.bowner[b];
+  const C* g = d.rec + 20 * c;
+  double drho = 0.0, dp = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    const double r = rb[3 * static_cast<long long>(b) + j];
+    drho += static_cast<double>(g[j]) * r;
+    dp += static_cast<double>(g[12 + j]) * r;
+  }
+  const double rho_c = static_cast<double>(d.prim[8 * c]);
+  const double p_c = pa.p_ref + static_cast<double>(d.prim[8 * c + 4]);
+  const double rho_f = rho_c + static_cast<double>(g[15]) * drho;
+  double p_f = p_c + static_cast<double>(g[19]) * dp;
+  if (!(rho_f > 0.0 && p_f > 0.0)) p_f = p_c;
+  const double phi_p = static_cast<double>(g[19]);
+  const C* fg = d.fgeo + 8 * static_cast<long long>(f);
+  for (int k = 0; k < 3; ++k) {
+    double warp = 0.0;
+    if (d.moment != nullptr) {
+      for (int j = 0; j < 3; ++j) {
+        warp += static_cast<double>(d.moment[(3 * j + k) * static_cast<long long>(d.nf) + f]) *
+                static_cast<double>(g[12 + j]);
+      }
+    }
+    const double area_k = static_cast<double>(fg[k]) * static_cast<double>(fg[3]);
+    out[k * static_cast<long long>(n) + i] = p_f * area_k + phi_p * warp;
+  }
+}
+
 // ------------------------------------------------------------------------ math probe
 
 template <class C>

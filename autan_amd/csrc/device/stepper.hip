@@ -18,7 +18,9 @@
 #include <vector>
 
 #include "../../../include/mxeuler.h"
+#include "exact.hpp"
 #include "kernels.hpp"
+#include "steady.hpp"
 #include "stream.hpp"
 
 namespace mxe {
@@ -100,6 +102,9 @@ class StepperBase {
   virtual void set_source(const double* source) = 0;
   virtual void spectral_radius(double* lam) = 0;
   virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
+  virtual void face_force(const int* faces, long long n, double* force) = 0;
+  virtual double area_scale() const = 0;
+  virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
                       int max_kernels, std::string* names) = 0;
 };
@@ -239,6 +244,31 @@ class Stepper final : public StepperBase {
       for (std::size_t k = 0; k < 5; ++k) prim_b[k * std::size_t(nb_) + b] = pb[8 * b + k];
     }
   }
+
+  // The pressure force on the listed boundary faces for the current state: the residual of u
+  // first (prim, grad, phi), as EulerPhysics::face_pressure evaluates it before reading them.
+  void face_force(const int* faces, long long n, double* force) override {
+    for (long long i = 0; i < n; ++i) {
+      if (faces[i] < ni_ || faces[i] >= nf_) throw std::invalid_argument("face_force: not a boundary face");
+    }
+    if (n == 0) return;
+    if (n > INT32_MAX / 3) throw std::invalid_argument("face_force: too many faces");
+    Buffer<int> list(static_cast<std::size_t>(n));
+    Buffer<double> out(3 * static_cast<std::size_t>(n));
+    list.upload(faces, static_cast<std::size_t>(n), stream_);
+    launch_prim(u_.get());
+    residual_from_prim();
+    const int ni = static_cast<int>(n);
+    launch("face_force", [&] {
+      k_face_force<C><<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(dev(), pa_, list.get(), ni, rb_.get(),
+                                                                          out.get());
+    });
+    out.download(force, 3 * static_cast<std::size_t>(n), stream_);
+    check(hipStreamSynchronize(stream_), "face_force");
+  }
+
+  double area_scale() const override { return area_scale_; }
+  double gamma() const override { return sc_.gamma; }
 
   int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches, int max_kernels,
               std::string* names) override {
@@ -664,6 +694,27 @@ class Stepper final : public StepperBase {
     for (std::size_t c = 0; c < nc; ++c) vol[c] = static_cast<C>(g.cell_volume[c]);
     vol_.upload(vol, stream_);
     vol64_.upload(vol64, stream_);
+    // FP64 x_f - x_c of the boundary faces, for the wall loads.
+    std::vector<double> rb(3 * std::max<std::size_t>(nf - ni, 1), 0.0);
+    for (std::size_t f = ni; f < nf; ++f) {
+      for (std::size_t u = 0; u < 3; ++u) rb[3 * (f - ni) + u] = g.r_left[u * nf + f];
+    }
+    rb_.upload(rb, stream_);
+    // sqrt(sum_i A_i^2 / N), A_i the surface of cell i summed in face order, as
+    // EulerPhysics::residual_scales does on one rank (physics/src/euler_physics.cpp:440-466).
+    {
+      std::vector<double> surface(nc, 0.0);
+      for (std::size_t f = 0; f < nf; ++f) {
+        surface[std::size_t(t.owner[f])] += g.face_area[f];
+        if (f < ni) surface[std::size_t(t.neighbour[f])] += g.face_area[f];
+      }
+      ExactSum squares, count;
+      for (std::size_t c = 0; c < nc; ++c) {
+        squares.add(surface[c] * surface[c]);
+        count.add(1.0);
+      }
+      area_scale_ = std::sqrt(squares.value() / count.value());
+    }
     build_incidence_slices(t, g, slen_host_);
     slen_host_.clear();
     slen_host_.shrink_to_fit();
@@ -859,52 +910,9 @@ class Stepper final : public StepperBase {
     *invalid = a != INT32_MAX ? a : (b != INT32_MAX ? b : -1);
   }
 
-  // The sum the words hold, rounded once (runtime/src/exact_sum.cpp:63-100).
-  static double exact_value(const long long* in) {
-    constexpr double inf = std::numeric_limits<double>::infinity();
-    if (in[kLimbs + 2] > 0 || (in[kLimbs] > 0 && in[kLimbs + 1] > 0)) return std::numeric_limits<double>::quiet_NaN();
-    if (in[kLimbs] > 0) return inf;
-    if (in[kLimbs + 1] > 0) return -inf;
-    long long w[kLimbs];
-    for (int i = 0; i < kLimbs; ++i) w[i] = in[i];
-    auto normalize = [&w] {
-      for (int i = 0; i + 1 < kLimbs; ++i) {
-        const long long carry = w[i] >> 32;
-        w[i] -= carry * (1LL << 32);
-        w[i + 1] += carry;
-      }
-    };
-    normalize();
-    const bool negative = w[kLimbs - 1] < 0;
-    if (negative) {
-      for (long long& v : w) v = -v;
-      normalize();
-    }
-    int t = kLimbs - 1;
-    while (t >= 0 && w[t] == 0) --t;
-    if (t < 0) return 0.0;
-    auto limb = [&w](int k) { return k < 0 ? 0ULL : static_cast<unsigned long long>(w[k]); };
-    const unsigned long long top = limb(t);
-    if (top > 0xffffffffULL) return negative ? -inf : inf;
-    int b = 0;
-    while ((top >> b) != 0) ++b;
-    const unsigned long long below = (limb(t - 1) << 32) | limb(t - 2);
-    const unsigned long long lead = (top << (64 - b)) | (below >> b);
-    bool sticky = (below & ((1ULL << b) - 1)) != 0;
-    for (int k = t - 3; k >= 0 && !sticky; --k) sticky = limb(k) != 0;
-    unsigned long long mantissa = lead >> 11;
-    const unsigned long long rest = lead & 0x7ff;
-    if (rest > 0x400 || (rest == 0x400 && (sticky || (mantissa & 1) != 0))) ++mantissa;
-    int exponent = 32 * (t - 2) - 1088 + b + 11;
-    if (mantissa == (1ULL << 53)) {
-      mantissa >>= 1;
-      ++exponent;
-    }
-    const double magnitude = std::ldexp(static_cast<double>(mantissa), exponent);
-    return negative ? -magnitude : magnitude;
-  }
-
   mxeuler_scheme sc_;
+  Buffer<double> rb_;
+  double area_scale_ = 0.0;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   int nc_ = 0, nf_ = 0, ni_ = 0, nb_ = 0;
@@ -1126,6 +1134,118 @@ int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* e
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
+  });
+}
+
+int mxeuler_face_force(mxeuler_stepper* s, const int32_t* faces, int64_t n, double* force, char* err,
+                       size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (n < 0 || (n > 0 && (faces == nullptr || force == nullptr))) throw std::invalid_argument("null argument");
+    s->impl->face_force(faces, n, force);
+  });
+}
+
+int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const double* area_vector,
+                  const double* face_centroid, const double center[3], double p_inf, double force[3],
+                  double moment[3], char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (n < 0 || !center || !force || !moment || (n > 0 && (!faces || !area_vector || !face_centroid))) {
+      throw std::invalid_argument("null argument");
+    }
+    const auto m = static_cast<std::size_t>(n);
+    std::vector<double> ff(3 * m);
+    s->impl->face_force(faces, n, ff.data());
+    // EulerPhysics::loads and wall_force (physics/src/euler_physics.cpp:527-562, 930-948): the
+    // pressure force minus p_inf S, and its moment about `center`, as exact sums.
+    mxe::ExactSum sums[6];
+    for (std::size_t i = 0; i < m; ++i) {
+      const double f[3] = {ff[i] - p_inf * area_vector[i], ff[m + i] - p_inf * area_vector[m + i],
+                           ff[2 * m + i] - p_inf * area_vector[2 * m + i]};
+      const double arm[3] = {face_centroid[i] - center[0], face_centroid[m + i] - center[1],
+                             face_centroid[2 * m + i] - center[2]};
+      for (int k = 0; k < 3; ++k) sums[k].add(f[k]);
+      sums[3].add(arm[1] * f[2] - arm[2] * f[1]);
+      sums[4].add(arm[2] * f[0] - arm[0] * f[2]);
+      sums[5].add(arm[0] * f[1] - arm[1] * f[0]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      force[k] = sums[k].value();
+      moment[k] = sums[3 + k].value();
+    }
+  });
+}
+
+int mxeuler_residual_scales(mxeuler_stepper* s, double density, double speed, double pressure, double scales[5],
+                            char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!scales) throw std::invalid_argument("null argument");
+    if (!(density > 0.0) || !(pressure > 0.0) || !(speed >= 0.0)) {
+      throw std::invalid_argument("residual_scales: density and pressure must be positive, speed non-negative");
+    }
+    // EulerPhysics::residual_scales (physics/src/euler_physics.cpp:467-476).
+    const double g = s->impl->gamma();
+    const double area = s->impl->area_scale();
+    const double c = std::sqrt(g * pressure / density);
+    const double q = speed + c;
+    const double h = c * c / (g - 1.0) + 0.5 * speed * speed;
+    const double mass = density * q * area;
+    const double momentum = mass * q;
+    scales[0] = mass;
+    scales[1] = scales[2] = scales[3] = momentum;
+    scales[4] = mass * h;
+  });
+}
+
+int mxsteady_run(const mxsteady_ops* ops, const mxsteady_options* options, mxsteady_result* result,
+                 double* history, int32_t* provisional, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!ops || !options || !result) throw std::invalid_argument("null argument");
+    mxe::steady::run(*ops, *options, *result, history, provisional);
+  });
+}
+
+namespace {
+
+int steady_step(void* ctx, double cfl, double residuals[5], int32_t* invalid) {
+  try {
+    double h = 0.0;
+    int inv = -1;
+    static_cast<mxeuler_stepper*>(ctx)->impl->step(cfl, false, 0.0, true, residuals, &h, &inv);
+    *invalid = inv;
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
+int steady_evaluate(void* ctx, double residuals[5], int32_t* invalid) {
+  try {
+    int inv = -1;
+    static_cast<mxeuler_stepper*>(ctx)->impl->evaluate(residuals, &inv);
+    *invalid = inv;
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
+int steady_order(void* ctx, int32_t on) {
+  try {
+    static_cast<mxeuler_stepper*>(ctx)->impl->set_second_order(on != 0);
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
+}
+
+}  // namespace
+
+int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result, double* history,
+                   int32_t* provisional, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !options || !result) throw std::invalid_argument("null argument");
+    const mxsteady_ops ops{s, steady_step, steady_evaluate, steady_order};
+    mxe::steady::run(ops, *options, *result, history, provisional);
   });
 }
 

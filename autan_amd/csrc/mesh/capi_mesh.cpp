@@ -13,6 +13,7 @@ struct mxmesh {
   mx::Packed packed;
   mx::Stencil stencil;
   std::vector<double> weights;
+  std::vector<double> area_vector;  // 3 x nf, FP64 area vectors (owner -> neighbour)
 };
 
 namespace {
@@ -90,6 +91,7 @@ int mxmesh_prepare(mxmesh* m, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
     const mx::Geometry g = mx::compute_geometry(m->mesh);
     m->packed = mx::pack(m->mesh, g);
+    m->area_vector = g.face_area;
     m->stencil = mx::face2_stencil(m->mesh, m->packed);
     m->prepared = true;
     // Every boundary face a `state` face until the caller says otherwise.
@@ -160,6 +162,7 @@ int mxmesh_array(const mxmesh* m, const char* name, const void** data, int64_t* 
     if (n == "cell_volume") return f64(p.cell_volume);
     if (n == "cell_centroid") return f64(p.cell_centroid);
     if (n == "face_centroid") return f64(p.face_centroid);
+    if (n == "face_area_vector") return f64(m->area_vector);
     if (n == "stencil_offsets") return i64(m->stencil.offsets);
     if (n == "stencil") return i32(m->stencil.entries);
     if (n == "stencil_weights") return f64(m->weights);

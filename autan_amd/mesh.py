@@ -22,7 +22,7 @@ STATE, FARFIELD, SLIP, INLET, OUTLET, WALL = range(6)
 
 _ARRAYS = (
     "owner neighbour cell_face_offsets cell_faces face_zone shift face_node_offsets face_nodes x y z "
-    "face_normal face_area r_left r_right face_moment cell_volume cell_centroid face_centroid "
+    "face_normal face_area face_area_vector r_left r_right face_moment cell_volume cell_centroid face_centroid "
     "stencil_offsets stencil stencil_weights"
 ).split()
 _DTYPES = {0: np.int32, 1: np.int64, 2: np.float64}

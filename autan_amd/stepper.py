@@ -296,6 +296,43 @@ class DeviceStepper:
         out["prim_b"] = out["prim_b"][: 5 * nb]
         return out
 
+    def face_force(self, faces: np.ndarray) -> np.ndarray:
+        """Euler::boundary_pressure's pressure force on the listed boundary faces for the current
+        state, shape (3, n)."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        out = np.empty(3 * max(f.size, 1))
+        self._call(self.lib.mxeuler_face_force, _p(f.ctypes.data), ctypes.c_int64(f.size), _p(out.ctypes.data))
+        return out[: 3 * f.size].reshape(3, f.size)
+
+    def loads(self, faces: np.ndarray, center=(0.0, 0.0, 0.0), p_inf: float = 0.0) -> tuple[np.ndarray, np.ndarray]:
+        """EulerPhysics::loads over the listed boundary faces: force and moment about `center`
+        (exact sums), the pressure force minus p_inf S."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        nf = self.mesh.n_faces
+        av = np.ascontiguousarray(self.mesh.face_area_vector.reshape(3, nf)[:, f])
+        fc = np.ascontiguousarray(self.mesh.face_centroid.reshape(3, nf)[:, f])
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        force = (ctypes.c_double * 3)()
+        moment = (ctypes.c_double * 3)()
+        self._call(
+            self.lib.mxeuler_loads,
+            _p(f.ctypes.data),
+            ctypes.c_int64(f.size),
+            _p(av.ctypes.data),
+            _p(fc.ctypes.data),
+            _p(c.ctypes.data),
+            _d(p_inf),
+            force,
+            moment,
+        )
+        return np.array(force[:]), np.array(moment[:])
+
+    def residual_scales(self, density: float, speed: float, pressure: float) -> np.ndarray:
+        """EulerPhysics::residual_scales: Phi_k = f_k sqrt(sum A_i^2 / N)."""
+        out = (ctypes.c_double * 5)()
+        self._call(self.lib.mxeuler_residual_scales, _d(density), _d(speed), _d(pressure), out)
+        return np.array(out[:])
+
     def profile(self, steps: int, cfl: float, local: bool = False, max_kernels: int = 32) -> dict:
         """Milliseconds per step and per launch of each kernel over `steps` steps (HIP events)."""
         ms_step = ctypes.c_double()

@@ -11,8 +11,8 @@ spectral radius and time step, validity checks).
 
 Prints one JSON line: ns per cell per step (lower is better), the roofline of the dominant kernel
 (algorithmic bytes per launch over its event-timed duration, against the 8 TB/s HBM peak), and a
-CPU baseline: the oracle (this repo's serial C++ restatement of the reference's CPU loops) timed on
-a bounded sample on this host.
+CPU baseline: the oracle (this repo's C++ restatement of the reference's CPU loops) timed on a
+bounded sample on this host, serial and on every host thread the process may use.
 """
 
 from __future__ import annotations
@@ -57,27 +57,60 @@ def build_case(n: int, policy: str, flux: str, order: str = "morton"):
     return m, sc, u, kinds.astype(np.float64), bc, src
 
 
+def host_threads() -> int:
+    """The host threads this process may use: OMP_NUM_THREADS where set, else its CPU affinity,
+    at most 16 (one GPU's share of the host)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(policy: str, flux: str, n: int, steps: int) -> dict:
-    """The oracle (serial, the reference CPU path's semantics) on a bounded sample."""
+    """The oracle (the reference CPU path's semantics: serial gather loops) on a bounded sample,
+    then the same loops on every host thread this process may use."""
     import numpy as np
 
+    from oracle import oracle as orc
     from oracle.oracle import Oracle
 
-    m, sc, u, kinds, bc, src = build_case(n, policy, flux)
+    m, sc, u0, kinds, bc, src = build_case(n, policy, flux)
     ora = Oracle(m, sc, policy, kinds, bc, src)
-    ora.step(u, cfl=0.5)  # warm
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        u, norms, h, inv = ora.step(u, cfl=0.5)
-    t = time.perf_counter() - t0
-    assert inv is None and np.all(np.isfinite(norms))
+    timings = {}
+    for threads in (1, host_threads()):
+        orc.set_threads(threads)
+        u = u0.copy()
+        ora.step(u.copy(), cfl=0.5)  # warm
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u, norms, h, inv = ora.step(u, cfl=0.5)
+        timings[threads] = time.perf_counter() - t0
+        assert inv is None and np.all(np.isfinite(norms))
+    orc.set_threads(1)
+    t1 = timings[1]
+    tn_threads = max(timings)
     return {
-        "value": t / steps / m.n_cells * 1e9,
+        "value": round(t1 / steps / m.n_cells * 1e9, 2),
         "unit": "ns/cell/iter",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} SSP-RK3 steps of the same workload at {n}^3 x 6 = {m.n_cells} tets, serial oracle, "
-        f"{t:.1f} s",
+        "sample": f"{steps} SSP-RK3 steps of the same workload at {n}^3 x 6 = {m.n_cells} tets, serial oracle "
+        f"(gather loops), {t1:.1f} s",
+        "threaded": {
+            "value": round(timings[tn_threads] / steps / m.n_cells * 1e9, 2),
+            "cores": tn_threads,
+            "cpu": cpu_model(),
+            "seconds": round(timings[tn_threads], 2),
+        },
     }
 
 
@@ -102,8 +135,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=128, help="cubes per side (6 n^3 tetrahedra)")
     ap.add_argument("--policy", default="fp32+acc64", choices=["fp32+acc64", "fp64"])
     ap.add_argument("--flux", default="hllc", choices=["roe", "hllc", "ausm+up"])
-    ap.add_argument("--cpu-n", type=int, default=24, help="cubes per side of the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-n", type=int, default=48, help="cubes per side of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
     args = ap.parse_args()

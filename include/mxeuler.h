@@ -145,6 +145,70 @@ int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t
 int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* err, size_t err_len);
 
 /* Version string of the library. */
+/* Euler::boundary_pressure (physics/src/euler.cpp:1187-1223) for the current state: the residual of
+ * u is evaluated first, then for each listed boundary face (global face index) the pressure force
+ * p_f S + phi_p M^T grad p in FP64, force 3 x n SoA. */
+int mxeuler_face_force(mxeuler_stepper* s, const int32_t* faces, int64_t n, double* force, char* err,
+                       size_t err_len);
+/* EulerPhysics::loads (physics/src/euler_physics.cpp:527-562; wall_force :930-948): exact sums over
+ * the listed boundary faces of the face force minus p_inf S_f, and of its moment about `center`.
+ * area_vector and face_centroid (3 x n SoA, FP64) are the caller's geometry of the listed faces. */
+int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const double* area_vector,
+                  const double* face_centroid, const double center[3], double p_inf, double force[3],
+                  double moment[3], char* err, size_t err_len);
+/* EulerPhysics::residual_scales (physics/src/euler_physics.cpp:440-477): Phi_k = f_k sqrt(sum_i
+ * A_i^2 / N) with the freestream's density, speed and pressure. */
+int mxeuler_residual_scales(mxeuler_stepper* s, double density, double speed, double pressure, double scales[5],
+                            char* err, size_t err_len);
+
+/* ---- Steady runs (EulerPhysics::device_step and the runtime's steady loop) -------------------
+ * A steady run on local time steps: an optional first-order start (physics/include/autan/physics/
+ * first_order_start.hpp:22-137; physics/src/euler_physics.cpp:1495-1571) and the convergence rule
+ * of case-schema.md#time (runtime/src/run.cpp:2262-2389; runtime/include/autan/runtime/
+ * round_off.hpp:14-160). The stepper is reached through callbacks, so the same run drives the device
+ * stepper (mxeuler_steady) or any other with the same three calls. */
+enum { MXSTEADY_CONVERGED = 0, MXSTEADY_BUDGET = 1, MXSTEADY_DIVERGED = 2 };
+enum { MXSTEADY_END_NONE = 0, MXSTEADY_END_DROP = 1, MXSTEADY_END_STALL = 2, MXSTEADY_END_CAP = 3,
+       MXSTEADY_END_LEVEL = 4 };
+
+typedef struct mxsteady_ops {
+  void* ctx;
+  /* DeviceStepper::step(cfl, nullopt, local = true): the norms of R(u) before the update. */
+  int (*step)(void* ctx, double cfl, double residuals[5], int32_t* invalid_cell);
+  /* DeviceStepper::evaluate(): the norms of R(u) and the first invalid cell (-1: none). */
+  int (*evaluate)(void* ctx, double residuals[5], int32_t* invalid_cell);
+  int (*set_second_order)(void* ctx, int32_t on);
+} mxsteady_ops;
+
+typedef struct mxsteady_options {
+  int64_t max_iterations;  /* >= 1 (case default 10000) */
+  double residual_orders;  /* > 0 (case default 8) */
+  double cfl;              /* the device path's steady CFL, 0.8 (euler_physics.cpp:94) */
+  int32_t level_rule;      /* 1 for fp32+acc64: level-crossing start, round-off plateau convergence */
+  int32_t start_order;     /* 1: first-order start, 2: second order from the first step */
+  const double* scales;    /* Phi_k (mxeuler_residual_scales) for the fp32+acc64 floors, or NULL */
+  const double* reference; /* 5 residuals to drop from (a warm start's source), or NULL: iteration 1 */
+} mxsteady_options;
+
+typedef struct mxsteady_result {
+  int32_t status;            /* MXSTEADY_* */
+  int32_t first_order_end;   /* MXSTEADY_END_* */
+  int32_t first_order_steps; /* provisional iterations */
+  int32_t diverged_cell;     /* -1: none */
+  int32_t plateau_mask;      /* bit e: equation e converged on its round-off plateau */
+  int32_t pad;
+  int64_t iterations;        /* the last completed iteration */
+  double reference[5];       /* the residuals the run dropped from */
+  double residuals[5];       /* the last reported residuals */
+} mxsteady_result;
+
+/* history: 5 reported residuals per iteration (or NULL); provisional: 1 per iteration (or NULL). */
+int mxsteady_run(const mxsteady_ops* ops, const mxsteady_options* options, mxsteady_result* result,
+                 double* history, int32_t* provisional, char* err, size_t err_len);
+/* mxsteady_run over the device stepper's state (upload it first). */
+int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result, double* history,
+                   int32_t* provisional, char* err, size_t err_len);
+
 const char* mxeuler_version(void);
 
 #ifdef __cplusplus

@@ -455,6 +455,11 @@ void numerical_flux(int flux, const C* wl, const C* wr, const C* n, C g, C minf,
   }
 }
 
+// Threads of the gather loops: 1 is the reference's serial CPU path; more runs every per-cell
+// (and per-boundary-face) loop of the gather variant in parallel. Each cell's arithmetic and order
+// stay those of the serial loop, so the results do not depend on the count.
+inline int g_threads = 1;
+
 // ------------------------------------------------------------------------------ exact sums
 // runtime/src/exact_sum.cpp:22-103: 68 limbs of 32 bits, limb i weighs 2^(32 i - 1088).
 class ExactSum {
@@ -573,6 +578,7 @@ class Base {
   virtual void arrays(double* prim, double* grad, double* phi, double* wmin, double* wmax, double* prim_b) = 0;
   virtual int step(double* u, double cfl, int has_dt, double dt, int local, int gather, double* residuals,
                    double* time_step, index_t* invalid) = 0;
+  virtual void face_force(const double* u, const index_t* faces, std::size_t n, double* out) = 0;
   virtual int first_invalid(const double* u, const double* res) = 0;
   virtual void set_second_order(int on) = 0;
 };
@@ -637,16 +643,19 @@ class Solver final : public Base {
   // Euler::state then Euler::residual (physics/src/euler.cpp:925-941).
   void residual(const double* u, double* res, bool gather) override {
     for (std::size_t k = 0; k < 5; ++k) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
       for (std::size_t c = 0; c < nc_; ++c) state_[k * nc_ + c] = S(u[k * nc_ + c] - sc_.reference[k]);
     }
     cell_loops();
     faces(gather);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t i = 0; i < 5 * nc_; ++i) res[i] = res_[i];
   }
 
   // kernels/euler.py:1091-1112 over the incidences of each cell, summed in FP64, stored as State.
   void spectral_radius(double* lam) override {
     const C g = C(sc_.gamma), pref = C(p_ref_);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       A acc = 0;
       for (auto k = cfo_[c]; k < cfo_[c + 1]; ++k) {
@@ -691,6 +700,40 @@ class Solver final : public Base {
     }
     for (std::size_t i = 0; i < 15 * nc_; ++i) grad[i] = grad_[i];
     for (std::size_t i = 0; i < 5 * nf_; ++i) prim_b[i] = prim_b_[i];
+  }
+
+  // Euler::boundary_pressure (physics/src/euler.cpp:1187-1223) after the residual of u: the face
+  // pressure of the boundary flux's reconstruction and the force p_f S + phi_p M^T grad p, in FP64
+  // from the stored arrays, on the listed boundary faces; out 3 x n.
+  void face_force(const double* u, const index_t* faces, std::size_t n, double* out) override {
+    std::vector<double> res(5 * nc_);
+    residual(u, res.data(), true);
+    for (std::size_t i = 0; i < n; ++i) {
+      const auto f = static_cast<std::size_t>(faces[i]);
+      const auto c = static_cast<std::size_t>(owner_[f]);
+      auto g = [&](std::size_t a, std::size_t j) { return static_cast<double>(grad_[(3 * a + j) * nc_ + c]); };
+      double drho = 0.0, dp = 0.0;
+      for (std::size_t j = 0; j < 3; ++j) {
+        const double r = rl_[j * nf_ + f];
+        drho += g(0, j) * r;
+        dp += g(4, j) * r;
+      }
+      const double rho_c = static_cast<double>(prim_[c]);
+      const double p_c = p_ref_ + static_cast<double>(prim_[4 * nc_ + c]);
+      const double rho_f = rho_c + static_cast<double>(phi_[c]) * drho;
+      double p_f = p_c + static_cast<double>(phi_[4 * nc_ + c]) * dp;
+      if (!(rho_f > 0.0 && p_f > 0.0)) p_f = p_c;
+      const double phi_p = static_cast<double>(phi_[4 * nc_ + c]);
+      for (std::size_t k = 0; k < 3; ++k) {
+        double warp = 0.0;
+        for (std::size_t j = 0; j < 3; ++j) {
+          const double mjk = moment_.empty() ? 0.0 : static_cast<double>(moment_[(3 * j + k) * nf_ + f]);
+          warp += mjk * g(4, j);
+        }
+        const double area_k = static_cast<double>(normal_[k * nf_ + f]) * static_cast<double>(area_[f]);
+        out[k * n + i] = p_f * area_k + phi_p * warp;
+      }
+    }
   }
 
   // EulerPhysics::first_invalid_cell (physics/src/euler_physics.cpp:1399-1433): the first cell
@@ -741,15 +784,15 @@ class Solver final : public Base {
       spectral_radius(lam.data());
       h = std::numeric_limits<double>::infinity();
       dtc.resize(nc_);
-      for (std::size_t c = 0; c < nc_; ++c) {
-        dtc[c] = cfl * volume64_[c] / static_cast<double>(S(lam[c]));
-        h = std::min(h, dtc[c]);
-      }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+      for (std::size_t c = 0; c < nc_; ++c) dtc[c] = cfl * volume64_[c] / static_cast<double>(S(lam[c]));
+      for (std::size_t c = 0; c < nc_; ++c) h = std::min(h, dtc[c]);
     }
     const double weights[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
     for (int s = 0; s < 3; ++s) {
       if (s > 0) residual(stage.data(), res.data(), gather != 0);
       for (std::size_t k = 0; k < 5; ++k) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
         for (std::size_t c = 0; c < nc_; ++c) {
           const std::size_t i = k * nc_ + c;
           const double hc = (!has_dt && local) ? dtc[c] : h;
@@ -809,6 +852,7 @@ class Solver final : public Base {
     // primitives (kernels/euler.py:428-433)
     C ur[5];
     for (int k = 0; k < 5; ++k) ur[k] = C(sc_.reference[k]);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       C dq[5], w[5];
       for (int k = 0; k < 5; ++k) dq[k] = C(state_[k * nc_ + c]);
@@ -816,6 +860,7 @@ class Solver final : public Base {
       for (int k = 0; k < 5; ++k) prim_[k * nc_ + c] = S(w[k]);
     }
     // boundary_states (kernels/euler.py:442-466)
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t f = ni_; f < nf_; ++f) {
       C wi[5], data[5], n[3], out[5];
       load_prim(static_cast<std::size_t>(owner_[f]), wi);
@@ -829,6 +874,7 @@ class Solver final : public Base {
       if (sc_.limiter) limiter();
     }
     // sources (kernels/euler.py:611-616): res = -V s
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       const C nv = -C(volume_[c]);
       for (int k = 0; k < 5; ++k) res_[k * nc_ + c] = A(nv * C(source_[k * nc_ + c]));
@@ -838,6 +884,7 @@ class Solver final : public Base {
   // gradients (kernels/euler.py:530-549; reduction body emit_cpu.py:126-182).
   void gradients() {
     const C inf = std::numeric_limits<C>::infinity();
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       A acc[15];
       C lo[5], hi[5], wc[5];
@@ -880,6 +927,7 @@ class Solver final : public Base {
     const C inf = std::numeric_limits<C>::infinity();
     C scale[5];
     for (int k = 0; k < 5; ++k) scale[k] = C(limiter_scale_[k]);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       C wc[5], gc[15], lo[5], hi[5], acc[5];
       load_prim(c, wc);
@@ -988,6 +1036,7 @@ class Solver final : public Base {
       }
       return;
     }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       A acc[5] = {};
       for (auto k = cfo_[c]; k < cfo_[c + 1]; ++k) {
@@ -1000,6 +1049,7 @@ class Solver final : public Base {
       }
       for (int q = 0; q < 5; ++q) res_[q * nc_ + c] += acc[q];
     }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (std::size_t c = 0; c < nc_; ++c) {
       A acc[5] = {};
       for (auto k = cfo_[c]; k < cfo_[c + 1]; ++k) {
@@ -1073,6 +1123,14 @@ int oracle_step(void* h, double* u, double cfl, int has_dt, double dt, int local
                 double* time_step, int* invalid) {
   return static_cast<oracle::Base*>(h)->step(u, cfl, has_dt, dt, local, gather, residuals, time_step, invalid);
 }
+
+void oracle_face_force(void* h, const double* u, const std::int32_t* faces, std::int64_t n, double* out) {
+  static_cast<oracle::Base*>(h)->face_force(u, faces, static_cast<std::size_t>(n), out);
+}
+
+void oracle_set_threads(int n) { oracle::g_threads = n < 1 ? 1 : n; }
+
+int oracle_threads() { return oracle::g_threads; }
 
 int oracle_first_invalid(void* h, const double* u, const double* res) {
   return static_cast<oracle::Base*>(h)->first_invalid(u, res);

@@ -81,7 +81,20 @@ def lib() -> ctypes.CDLL:
     L.oracle_boundary_value.argtypes = [_p, _d, _p, _p, _d, _d, _d, _p]
     L.oracle_to_primitive.argtypes = [_p, _p, _d, _p]
     L.oracle_warp.argtypes = [_p] * 6 + [_d, _d, _p]
+    L.oracle_set_threads.argtypes = [ctypes.c_int]
+    L.oracle_face_force.argtypes = [_p, _p, _p, ctypes.c_int64, _p]
+    L.oracle_threads.restype = ctypes.c_int
     return L
+
+
+def set_threads(n: int) -> None:
+    """Threads of the gather loops (1: the serial loops of the reference's CPU path). Results do
+    not depend on the count."""
+    lib().oracle_set_threads(int(n))
+
+
+def threads() -> int:
+    return lib().oracle_threads()
 
 
 def _a(x) -> np.ndarray:
@@ -209,6 +222,15 @@ class Oracle:
 
     def set_second_order(self, on: bool) -> None:
         lib().oracle_set_second_order(self.h, int(on))
+
+    def face_force(self, u, faces) -> np.ndarray:
+        """Euler::boundary_pressure's force on the listed boundary faces after the residual of u,
+        shape (3, n)."""
+        u = _a(u).reshape(-1)
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        out = np.empty(3 * max(f.size, 1))
+        lib().oracle_face_force(self.h, u.ctypes.data, f.ctypes.data, f.size, out.ctypes.data)
+        return out[: 3 * f.size].reshape(3, f.size)
 
     def first_invalid(self, u, res=None) -> int | None:
         u = _a(u).reshape(-1)

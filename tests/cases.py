@@ -173,3 +173,30 @@ def normwise_units(case: Case, policy_eps: float) -> np.ndarray:
     np.add.at(area, m.owner, m.face_area)
     np.add.at(area, m.neighbour, m.face_area[: m.n_interior])
     return policy_eps * f[:, None] * area[None, :]
+
+
+class OracleTarget:
+    """The oracle as the stepper of a steady run (step / evaluate / set_second_order), holding the
+    state u: the CPU side of the steady-run comparisons."""
+
+    def __init__(self, ora, u) -> None:
+        self.ora = ora
+        self.u = np.array(u, dtype=np.float64).reshape(-1)
+        self.n = self.u.size // 5
+
+    def norms(self, res: np.ndarray) -> np.ndarray:
+        from oracle import oracle as orc
+
+        r = res.reshape(5, self.n)
+        return np.array([math.sqrt(orc.exact_sum(r[k] * r[k]) / self.n) for k in range(5)])
+
+    def step(self, cfl):
+        self.u, norms, _h, inv = self.ora.step(self.u, cfl=cfl, local=True)
+        return norms, inv
+
+    def evaluate(self):
+        res = self.ora.residual(self.u)
+        return self.norms(res), self.ora.first_invalid(self.u, res)
+
+    def set_second_order(self, on: bool) -> None:
+        self.ora.set_second_order(on)

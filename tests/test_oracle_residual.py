@@ -313,3 +313,24 @@ def test_renumbering_moves_the_residual_with_the_cells():
     rb = Oracle(b.mesh, b.scheme, "fp64", b.bc_kind, b.bc, b.source).residual(b.u).reshape(5, -1)[:, ib]
     units = cases.normwise_units(a, EPS["fp64"])[:, ia]
     assert np.max(np.abs(ra - rb) / units) <= 500
+
+
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+def test_threaded_gather_loops_give_the_serial_bits(policy):
+    """The gather variant on several threads keeps each cell's arithmetic and order, so the
+    state, norms and time step are those of one thread, bit for bit."""
+    case = cases.fixture_case("poly", "hllc")
+    ora = Oracle(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
+    out = []
+    for t in (1, 4):
+        orc.set_threads(t)
+        try:
+            u, norms, h, inv = ora.step(case.u.copy(), cfl=0.5)
+            u, norms2, h2, inv = ora.step(u, cfl=0.5, local=True)
+        finally:
+            orc.set_threads(1)
+        out.append((u, norms, h, norms2))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][3], out[1][3])
+    assert out[0][2] == out[1][2]
