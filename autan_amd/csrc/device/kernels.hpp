@@ -923,6 +923,57 @@ __global__ __launch_bounds__(1024) void k_step_reduce(const long long* __restric
   }
 }
 
+// ------------------------------------------------------------------------ conservation check
+
+// EulerPhysics::check_conservation's sums on one rank (physics/src/euler_physics.cpp:1750-1784), per
+// equation k as exact sums, every term on its own: words[k] E = sum_c R^F_c - sum_b F_b,
+// words[5 + k] D = sum |F| over the incidences (an interior face twice), words[10 + k] the boundary
+// fluxes, words[15 + k] sum_c R^F_c. res holds R^F (the residual without its source), fflux every
+// face's flux. Grid-stride with block-uniform trip counts (the wave-wide exact adds need every lane).
+constexpr int kConsSums = 20;
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_conservation(Dev<C> d, long long* __restrict__ words) {
+  __shared__ long long acc[kConsSums][kWords];
+  for (int i = threadIdx.x; i < kConsSums * kWords; i += kBlock) (&acc[0][0])[i] = 0;
+  __syncthreads();
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock; base < d.nc; base += stride) {
+    const long long c = base + threadIdx.x;
+    const bool active = c < d.nc;
+    for (int k = 0; k < kSums; ++k) {
+      const double r = active ? d.res[k * static_cast<long long>(d.nc) + c] : 0.0;
+      exact_add_wave(acc[k], r, active);
+      exact_add_wave(acc[15 + k], r, active);
+    }
+  }
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock; base < d.nf; base += stride) {
+    const long long f = base + threadIdx.x;
+    const bool active = f < d.nf;
+    const bool boundary = active && f >= d.ni;
+    for (int k = 0; k < kSums; ++k) {
+      const double v = active ? static_cast<double>(d.fflux[8 * f + k]) : 0.0;
+      exact_add_wave(acc[5 + k], boundary ? fabs(v) : 2.0 * fabs(v), active);
+      exact_add_wave(acc[k], -v, boundary);
+      exact_add_wave(acc[10 + k], v, boundary);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kConsSums) {
+    long long* a = acc[threadIdx.x];
+    for (int l = 0; l + 1 < kLimbs; ++l) {
+      const long long carry = a[l] >> 32;
+      a[l] -= carry * (1LL << 32);
+      a[l + 1] += carry;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kConsSums * kWords; i += kBlock) {
+    const long long v = (&acc[0][0])[i];
+    if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(words + i), static_cast<unsigned long long>(v));
+  }
+}
+
 // ------------------------------------------------------------------------ wall loads
 
 // Euler::boundary_pressure (physics/src/euler.cpp:1187-1223) on the listed boundary faces, in FP64

@@ -103,6 +103,7 @@ class StepperBase {
   virtual void spectral_radius(double* lam) = 0;
   virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
   virtual void face_force(const int* faces, long long n, double* force) = 0;
+  virtual void conservation(double* out) = 0;
   virtual double area_scale() const = 0;
   virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
@@ -265,6 +266,29 @@ class Stepper final : public StepperBase {
     });
     out.download(force, 3 * static_cast<std::size_t>(n), stream_);
     check(hipStreamSynchronize(stream_), "face_force");
+  }
+
+  // EulerPhysics::check_conservation on the current state: R^F (the residual without its
+  // source), every face's flux from the store loop, and their exact sums.
+  void conservation(double* out) override {
+    Buffer<C> saved = std::move(source_);
+    try {
+      launch_prim(u_.get());
+      residual_from_prim();
+    } catch (...) {
+      source_ = std::move(saved);
+      throw;
+    }
+    source_ = std::move(saved);
+    const Dev<C> d = dev();
+    launch("flux", [&] { launch_flux(d); });
+    Buffer<long long> words(std::size_t(kConsSums) * kWords);
+    std::vector<long long> zero(words.size(), 0), host(words.size());
+    words.upload(zero.data(), zero.size(), stream_);
+    launch("conservation", [&] { k_conservation<C><<<1024, kBlock, 0, stream_>>>(d, words.get()); });
+    words.download(host.data(), host.size(), stream_);
+    check(hipStreamSynchronize(stream_), "conservation");
+    for (int i = 0; i < kConsSums; ++i) out[i] = exact_value(host.data() + std::size_t(i) * kWords);
   }
 
   double area_scale() const override { return area_scale_; }
@@ -1172,6 +1196,13 @@ int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const dou
       force[k] = sums[k].value();
       moment[k] = sums[3 + k].value();
     }
+  });
+}
+
+int mxeuler_conservation(mxeuler_stepper* s, double sums[20], char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!sums) throw std::invalid_argument("null argument");
+    s->impl->conservation(sums);
   });
 }
 

@@ -327,6 +327,14 @@ class DeviceStepper:
         )
         return np.array(force[:]), np.array(moment[:])
 
+    def conservation(self) -> dict[str, np.ndarray]:
+        """EulerPhysics::check_conservation's exact sums on the current state: E = sum R^F - sum
+        F_b, D = sum |F| over the incidences, the boundary fluxes and sum R^F, 5 each."""
+        out = (ctypes.c_double * 20)()
+        self._call(self.lib.mxeuler_conservation, out)
+        a = np.array(out[:])
+        return {"E": a[0:5], "D": a[5:10], "boundary": a[10:15], "faces": a[15:20]}
+
     def residual_scales(self, density: float, speed: float, pressure: float) -> np.ndarray:
         """EulerPhysics::residual_scales: Phi_k = f_k sqrt(sum A_i^2 / N)."""
         out = (ctypes.c_double * 5)()

@@ -156,6 +156,11 @@ int mxeuler_face_force(mxeuler_stepper* s, const int32_t* faces, int64_t n, doub
 int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const double* area_vector,
                   const double* face_centroid, const double center[3], double p_inf, double force[3],
                   double moment[3], char* err, size_t err_len);
+/* EulerPhysics::check_conservation (physics/src/euler_physics.cpp:1750-1784; docs/design/euler.md,
+ * Conservation check) on the current state: per equation k, exact sums over the cells and faces,
+ * sums[k] E = sum_c R^F_c - sum_b F_b, sums[5 + k] D = sum |F| over the incidences, sums[10 + k] the
+ * boundary fluxes, sums[15 + k] sum_c R^F_c, with R^F the residual without its source. */
+int mxeuler_conservation(mxeuler_stepper* s, double sums[20], char* err, size_t err_len);
 /* EulerPhysics::residual_scales (physics/src/euler_physics.cpp:440-477): Phi_k = f_k sqrt(sum_i
  * A_i^2 / N) with the freestream's density, speed and pressure. */
 int mxeuler_residual_scales(mxeuler_stepper* s, double density, double speed, double pressure, double scales[5],

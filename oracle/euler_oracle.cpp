@@ -579,6 +579,7 @@ class Base {
   virtual int step(double* u, double cfl, int has_dt, double dt, int local, int gather, double* residuals,
                    double* time_step, index_t* invalid) = 0;
   virtual void face_force(const double* u, const index_t* faces, std::size_t n, double* out) = 0;
+  virtual void conservation(const double* u, double* out) = 0;
   virtual int first_invalid(const double* u, const double* res) = 0;
   virtual void set_second_order(int on) = 0;
 };
@@ -688,6 +689,39 @@ class Solver final : public Base {
         boundary_body(f, out);
       }
       for (int k = 0; k < 5; ++k) flux[k * nf_ + f] = A(out[k]);
+    }
+  }
+
+  // EulerPhysics::check_conservation (physics/src/euler_physics.cpp:1750-1784) on one rank: R^F is
+  // the residual of u without its source, the fluxes those of the store loops; per equation k
+  // out[k] E, out[5 + k] D, out[10 + k] boundary, out[15 + k] faces, each an exact sum of its terms.
+  void conservation(const double* u, double* out) override {
+    std::vector<S> saved(5 * nc_, S(0));
+    std::swap(saved, source_);
+    std::vector<double> res(5 * nc_), flux(5 * nf_);
+    residual(u, res.data(), true);
+    std::swap(saved, source_);
+    face_fluxes(flux.data());
+    for (std::size_t k = 0; k < 5; ++k) {
+      ExactSum e, d, b, faces;
+      for (std::size_t c = 0; c < nc_; ++c) {
+        e.add(res[k * nc_ + c]);
+        faces.add(res[k * nc_ + c]);
+      }
+      for (std::size_t f = ni_; f < nf_; ++f) {
+        const double v = flux[k * nf_ + f];
+        e.add(-v);
+        d.add(std::abs(v));
+        b.add(v);
+      }
+      for (std::size_t f = 0; f < ni_; ++f) {
+        d.add(std::abs(flux[k * nf_ + f]));
+        d.add(std::abs(flux[k * nf_ + f]));
+      }
+      out[k] = e.value();
+      out[5 + k] = d.value();
+      out[10 + k] = b.value();
+      out[15 + k] = faces.value();
     }
   }
 
@@ -1126,6 +1160,10 @@ int oracle_step(void* h, double* u, double cfl, int has_dt, double dt, int local
 
 void oracle_face_force(void* h, const double* u, const std::int32_t* faces, std::int64_t n, double* out) {
   static_cast<oracle::Base*>(h)->face_force(u, faces, static_cast<std::size_t>(n), out);
+}
+
+void oracle_conservation(void* h, const double* u, double* out) {
+  static_cast<oracle::Base*>(h)->conservation(u, out);
 }
 
 void oracle_set_threads(int n) { oracle::g_threads = n < 1 ? 1 : n; }

@@ -83,6 +83,7 @@ def lib() -> ctypes.CDLL:
     L.oracle_warp.argtypes = [_p] * 6 + [_d, _d, _p]
     L.oracle_set_threads.argtypes = [ctypes.c_int]
     L.oracle_face_force.argtypes = [_p, _p, _p, ctypes.c_int64, _p]
+    L.oracle_conservation.argtypes = [_p, _p, _p]
     L.oracle_threads.restype = ctypes.c_int
     return L
 
@@ -231,6 +232,13 @@ class Oracle:
         out = np.empty(3 * max(f.size, 1))
         lib().oracle_face_force(self.h, u.ctypes.data, f.ctypes.data, f.size, out.ctypes.data)
         return out[: 3 * f.size].reshape(3, f.size)
+
+    def conservation(self, u) -> dict[str, np.ndarray]:
+        """check_conservation's exact sums: E, D, boundary and faces, 5 each."""
+        u = _a(u).reshape(-1)
+        out = np.empty(20)
+        lib().oracle_conservation(self.h, u.ctypes.data, out.ctypes.data)
+        return {"E": out[0:5], "D": out[5:10], "boundary": out[10:15], "faces": out[15:20]}
 
     def first_invalid(self, u, res=None) -> int | None:
         u = _a(u).reshape(-1)

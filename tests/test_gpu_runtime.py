@@ -108,3 +108,22 @@ def test_steady_run_matches_the_oracle_run_bit_for_bit(policy):
     np.testing.assert_array_equal(got.provisional, want.provisional)
     assert got.first_order_steps > 0
     np.testing.assert_array_equal(dev.download(), target.u)
+
+
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+@pytest.mark.parametrize("name", ["tets-hllc2", "bump-hybrid-ausm2"])
+def test_conservation_sums_match_oracle(name, policy):
+    """check_conservation's exact sums on the device: bit for bit with the oracle where no face
+    calls exp/log, and E within its round-off bound everywhere."""
+    case = LOAD_CASES[name]()
+    dev, ora = build(case, policy)
+    dev.upload(case.u)
+    got = dev.conservation()
+    want = ora.conservation(case.u)
+    m = int(np.diff(case.mesh.cell_face_offsets).max())
+    assert np.all(np.abs(got["E"]) <= m * 2.0**-53 * got["D"])
+    if case.uses_exp_log:
+        np.testing.assert_allclose(got["D"], want["D"], rtol=1e-6)
+    else:
+        for key in ("E", "D", "boundary", "faces"):
+            np.testing.assert_array_equal(got[key], want[key])

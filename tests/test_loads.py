@@ -83,3 +83,26 @@ def test_exact_loads_sum():
     rng = np.random.default_rng(3)
     x = rng.standard_normal(1000) * 10.0 ** rng.integers(-5, 5, 1000)
     assert orc.exact_sum(x) == orc.exact_sum(x[::-1]) == orc.exact_sum(rng.permutation(x))
+
+
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+@pytest.mark.parametrize("name", ["bump-hybrid", "tets", "poly"])
+def test_conservation_holds_to_round_off(name, policy):
+    """Sum_c R^F_c = sum_b F_b where every interior flux enters its two cells with opposite signs:
+    E is the rounding of the per-cell FP64 sums, at most (faces per cell) u D, u = 2^-53; the sums
+    of the residual are those of the oracle's own residual without source."""
+    case = {
+        "bump-hybrid": lambda: cases.bump_case(16, 12, 8, hybrid=True),
+        "tets": lambda: cases.tets_case(5),
+        "poly": lambda: cases.fixture_case("poly", "hllc"),
+    }[name]()
+    ora = Oracle(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
+    r = ora.conservation(case.u)
+    m = int(np.diff(case.mesh.cell_face_offsets).max())
+    assert np.all(np.abs(r["E"]) <= m * 2.0**-53 * r["D"])
+    assert np.all(r["D"] > 0)
+    plain = Oracle(case.mesh, case.scheme, policy, case.bc_kind, case.bc, None)
+    res = plain.residual(case.u).reshape(5, -1)
+    assert list(r["faces"]) == [orc.exact_sum(res[k]) for k in range(5)]
+    flux = ora.face_fluxes().reshape(5, -1)[:, case.mesh.n_interior :]
+    assert list(r["boundary"]) == [orc.exact_sum(flux[k]) for k in range(5)]
