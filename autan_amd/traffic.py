@@ -92,6 +92,11 @@ def kernel_bytes(c: Counts, policy: str = "fp32+acc64", local: bool = False) -> 
     out["rk_prim"] = base + N * (5 * r / 2 + 5 * s)  # stage 1 reads no separate stage; averaged
     out["rk_check"] = base + N * 5 * r
     out["clear"] = 5 * 71 * 8
+    # The fused face kernel: the flux kernel's inputs and the gather's, the fluxes staying on chip
+    # (LDS); the first residual of a step also writes lambda (and the local dt) from the FP64 volume
+    # and adds the norms of the residual it writes.
+    out["faces"] = out["flux"] - F * 5 * s + out["gather"] - F * 5 * s
+    out["faces_first"] = out["faces"] + N * (r + s + (r if local else 0))
     return out
 
 

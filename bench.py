@@ -1,13 +1,19 @@
 """Benchmark: SSP-RK3 steps of the explicit Euler residual on one MI355X.
 
-Workload (the manufactured-solution tetrahedra, the configuration the metric is quoted on that
-fits one GPU): the unit cube in n^3 cubes of 6 Kuhn tetrahedra (n = 128: 12,582,912 cells),
-interior nodes jittered by +-10 % of the spacing (seed 0), cells renumbered along a Morton curve; a smooth manufactured
-Euler solution (rho, u, v, w, p sinusoids of amplitude 0.1 about rho = 1, p = 1/1.4, Mach 0.3)
-with its volume source and exact `state` boundary faces; second-order HLLC with face2
-least-squares gradients and the Venkatakrishnan limiter (K = 5), fp32+acc64, SSP-RK3 with the
+Default workload (the manufactured-solution tetrahedra, the configuration the metric is quoted on
+that fits one GPU): the unit cube in n^3 cubes of 6 Kuhn tetrahedra (n = 128: 12,582,912 cells),
+interior nodes jittered by +-10 % of the spacing (seed 0), cells renumbered along a Morton curve; a
+smooth manufactured Euler solution (rho, u, v, w, p sinusoids of amplitude 0.1 about rho = 1,
+p = 1/1.4, Mach 0.3) with its volume source and exact `state` boundary faces; second-order HLLC with
+face2 least-squares gradients and the Venkatakrishnan limiter (K = 5), fp32+acc64, SSP-RK3 with the
 global CFL 0.5 time step. One step = one SSP-RK3 step (three residuals, the norms of R(u), the
 spectral radius and time step, validity checks).
+
+Other workloads (--workload): `vortex`, the isentropic vortex on 32^3 periodic hexahedra
+(first-order Roe, fp64, 200 steps, compared with the CPU oracle's 200 steps); `bump`, the hybrid
+bump channel (about 8 M cells with a prism layer and a band of merged polyhedra; local time steps),
+which also reports a steady run's residual drop per second, the conservation check and the
+fp32+acc64 residual's distance from the fp64 one.
 
 Prints one JSON line: ns per cell per step (lower is better), the roofline of the dominant kernel
 (algorithmic bytes per launch over its event-timed duration, against the 8 TB/s HBM peak), and a
@@ -20,6 +26,7 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -33,28 +40,6 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def build_case(n: int, policy: str, flux: str, order: str = "morton"):
-    import numpy as np
-
-    from autan_amd import mesh as meshes
-    from autan_amd import mms
-    from autan_amd.stepper import freestream_scheme
-
-    t0 = time.time()
-    m = meshes.kuhn_tets(n, 0.1, 0, order=order)
-    kinds = np.zeros(m.n_boundary, dtype=np.int32)
-    meshes.set_boundary_kinds(m, kinds)
-    log(f"mesh {n}^3 x 6 tets: {m.n_cells} cells, {m.n_faces} faces, {m.n_stencil} stencil entries ({time.time() - t0:.1f} s)")
-    rho, vel, p = mms.freestream()
-    sc = freestream_scheme(rho, vel, p, flux=flux, second_order=True, limiter=True)
-    sc.venkat_k = 5.0
-    x = m.centroids()
-    u = mms.conserved(mms.primitive(x)).reshape(-1)
-    src = mms.source(x).reshape(-1)
-    bc = mms.primitive(m.face_centroids()[m.n_interior :])
-    return m, sc, u, kinds.astype(np.float64), bc, src
 
 
 def host_threads() -> int:
@@ -75,43 +60,61 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(policy: str, flux: str, n: int, steps: int) -> dict:
+def make_workload(name: str, args, size=None):
+    from autan_amd import workloads
+
+    t0 = time.time()
+    if name == "tets":
+        w = workloads.tets(size or args.n, args.policy, args.flux, args.order)
+    elif name == "vortex":
+        w = workloads.vortex(size or 32, args.policy, args.flux)
+    else:
+        nx, ny, nz = size or (462, 154, 154)
+        w = workloads.bump(nx, ny, nz, args.policy, args.flux, order=args.order)
+    m = w.mesh
+    log(f"{name}: {m.n_cells} cells, {m.n_faces} faces, {m.n_stencil} stencil entries ({time.time() - t0:.1f} s)")
+    return w
+
+
+def cpu_baseline(w, steps: int, sample: str) -> tuple[dict, object]:
     """The oracle (the reference CPU path's semantics: serial gather loops) on a bounded sample,
-    then the same loops on every host thread this process may use."""
+    then the same loops on every host thread this process may use. Returns the record and the
+    state after the serial steps."""
     import numpy as np
 
     from oracle import oracle as orc
     from oracle.oracle import Oracle
 
-    m, sc, u0, kinds, bc, src = build_case(n, policy, flux)
-    ora = Oracle(m, sc, policy, kinds, bc, src)
-    timings = {}
-    for threads in (1, host_threads()):
+    m = w.mesh
+    ora = Oracle(m, w.scheme, w.policy, w.bc_kind, w.bc, w.source)
+    timings, states = {}, {}
+    for threads in dict.fromkeys((1, host_threads())):
         orc.set_threads(threads)
-        u = u0.copy()
-        ora.step(u.copy(), cfl=0.5)  # warm
+        u = w.u.copy()
         t0 = time.perf_counter()
         for _ in range(steps):
-            u, norms, h, inv = ora.step(u, cfl=0.5)
+            u, norms, h, inv = ora.step(u, cfl=w.cfl, local=w.local)
         timings[threads] = time.perf_counter() - t0
-        assert inv is None and np.all(np.isfinite(norms))
+        states[threads] = u
+        if inv is not None or not np.all(np.isfinite(norms)):
+            raise RuntimeError(f"the oracle diverged: invalid cell {inv}")
     orc.set_threads(1)
     t1 = timings[1]
-    tn_threads = max(timings)
-    return {
+    tn = max(timings)
+    record = {
         "value": round(t1 / steps / m.n_cells * 1e9, 2),
         "unit": "ns/cell/iter",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} SSP-RK3 steps of the same workload at {n}^3 x 6 = {m.n_cells} tets, serial oracle "
-        f"(gather loops), {t1:.1f} s",
+        "sample": f"{steps} SSP-RK3 steps of {sample} ({m.n_cells} cells), serial oracle (gather loops), {t1:.1f} s",
         "threaded": {
-            "value": round(timings[tn_threads] / steps / m.n_cells * 1e9, 2),
-            "cores": tn_threads,
+            "value": round(timings[tn] / steps / m.n_cells * 1e9, 2),
+            "cores": tn,
             "cpu": cpu_model(),
-            "seconds": round(timings[tn_threads], 2),
+            "seconds": round(timings[tn], 2),
         },
     }
+    return record, states[1]
 
 
 def stream_peak() -> dict:
@@ -127,52 +130,30 @@ def stream_peak() -> dict:
     return {"copy": gbps[0], "scale": gbps[1], "add": gbps[2], "triad": gbps[3]}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=128, help="cubes per side (6 n^3 tetrahedra)")
-    ap.add_argument("--policy", default="fp32+acc64", choices=["fp32+acc64", "fp64"])
-    ap.add_argument("--flux", default="hllc", choices=["roe", "hllc", "ausm+up"])
-    ap.add_argument("--cpu-n", type=int, default=48, help="cubes per side of the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=4)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
-              "(the sharding design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
-        return 2
-
+def timed_steps(dev, w, warmup: int, steps: int):
     import numpy as np
 
-    from autan_amd import traffic
-    from autan_amd.stepper import DeviceStepper
-
-    m, sc, u, kinds, bc, src = build_case(args.n, args.policy, args.flux, args.order)
-    dev = DeviceStepper(m, sc, args.policy, kinds, bc, src)
-    dev.upload(u)
-    log("uploaded; warming up")
-    for _ in range(args.warmup):
-        r = dev.step(cfl=0.5)
+    for _ in range(warmup):
+        r = dev.step(cfl=w.cfl, local=w.local)
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        r = dev.step(cfl=0.5)
+    for _ in range(steps):
+        r = dev.step(cfl=w.cfl, local=w.local)
     elapsed = time.perf_counter() - t0
     if r.invalid is not None or not np.all(np.isfinite(r.residuals)):
         raise RuntimeError(f"the run diverged: invalid cell {r.invalid}, norms {r.residuals}")
-    ms_step = elapsed / args.steps * 1e3
-    ns_cell = ms_step * 1e6 / m.n_cells
-    log(f"{args.steps} steps: {ms_step:.3f} ms/step, {ns_cell:.3f} ns/cell/step, norms {r.residuals}")
+    return elapsed, r
 
-    prof = dev.profile(steps=5, cfl=0.5)
-    counts = traffic.Counts.of(m, source=True, limiter=True, second_order=True)
-    kb = traffic.kernel_bytes(counts, args.policy)
+
+def roofline(dev, w, ms_step: float, tag: str) -> tuple[dict, dict]:
+    from autan_amd import traffic
+
+    prof = dev.profile(steps=5, cfl=w.cfl, local=w.local)
+    counts = traffic.Counts.of(w.mesh, **w.counts)
+    kb = traffic.kernel_bytes(counts, w.policy, w.local)
     per_step = {k: v["ms"] * v["launches"] for k, v in prof["kernels"].items()}
     dominant = max(per_step, key=per_step.get)
     ms_launch = prof["kernels"][dominant]["ms"]
-    achieved = kb[dominant] / (ms_launch * 1e-3) / 1e9
+    achieved = kb[dominant] / (ms_launch * 1e-3) / 1e9 if dominant in kb else None
     kernels = {
         k: {
             "ms": round(v["ms"], 4),
@@ -181,64 +162,205 @@ def main() -> int:
         }
         for k, v in prof["kernels"].items()
     }
-    step_b = traffic.step_bytes(counts, args.policy)
-    ref_b = traffic.reference_step_bytes(counts, args.policy)
+    step_b = traffic.step_bytes(counts, w.policy, w.local)
+    ref_b = traffic.reference_step_bytes(counts, w.policy, w.local)
     peak = stream_peak()
     log(f"profile: {json.dumps(kernels)}")
     log(f"STREAM: {json.dumps(peak)}")
-    traffic_file = ROOT / "profiles" / f"traffic_{dominant}_tets{args.n}_{args.policy}.json"
+    traffic_file = ROOT / "profiles" / f"traffic_{dominant}_{tag}_{w.policy}.json"
     measured = json.loads(traffic_file.read_text())["bytes_per_launch"] if traffic_file.exists() else None
+    roof = {
+        "bound": "hbm",
+        "kernel": dominant,
+        "achieved": None if achieved is None else round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured,
+        "algorithmic_bytes_per_launch": kb.get(dominant),
+        "ms_per_launch": round(ms_launch, 4),
+    }
+    step = {
+        "event_ms": round(prof["ms_per_step"], 4),
+        "algorithmic_bytes": step_b,
+        "achieved_GBs": round(step_b / (ms_step * 1e-3) / 1e9, 1),
+        "frac_of_peak": round(step_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "reference_B_step": ref_b,
+        "reference_B_step_frac": round(ref_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stream_GBs": {k: round(v, 1) for k, v in peak.items()},
+        "kernels": kernels,
+    }
+    return roof, step
 
+
+def flux_scale(w):
+    """f_k A_i (5, n): the reference's per-cell error unit, f = (rho q, rho q^2 x 3, rho H q) of the
+    freestream with q = |v| + c, A_i the cell's surface."""
+    import numpy as np
+
+    ref = np.array(w.scheme.reference)
+    g = w.scheme.gamma
+    v = ref[1:4] / ref[0]
+    p = (g - 1.0) * (ref[4] - 0.5 * ref[0] * (v @ v))
+    c = math.sqrt(g * p / ref[0])
+    q = math.sqrt(v @ v) + c
+    h = c * c / (g - 1.0) + 0.5 * (v @ v)
+    f = np.array([ref[0] * q, ref[0] * q * q, ref[0] * q * q, ref[0] * q * q, ref[0] * h * q])
+    m = w.mesh
+    area = np.zeros(m.n_cells)
+    np.add.at(area, m.owner, m.face_area)
+    np.add.at(area, m.neighbour, m.face_area[: m.n_interior])
+    return f[:, None] * area[None, :]
+
+
+def bump_extras(dev, w, args) -> dict:
+    """The bump channel's steady run (orders of residual drop per second from the freestream
+    start), its conservation check and the fp32+acc64 residual against the fp64 one."""
+    import numpy as np
+
+    from autan_amd import steady
+    from autan_amd.stepper import DeviceStepper
+
+    out = {}
+    dev.upload(w.u)
+    ref = np.array(w.scheme.reference)
+    speed = float(np.linalg.norm(ref[1:4])) / ref[0]
+    p = (w.scheme.gamma - 1.0) * (ref[4] - 0.5 * ref[0] * speed * speed)
+    scales = dev.residual_scales(ref[0], speed, p)
+    t0 = time.perf_counter()
+    run = steady.solve(dev, max_iterations=args.steady_iterations, residual_orders=8.0, scales=scales)
+    t = time.perf_counter() - t0
+    drop = run.reference / run.residuals
+    out["steady"] = {
+        "iterations": run.iterations,
+        "seconds": round(t, 3),
+        "first_order_steps": run.first_order_steps,
+        "first_order_end": run.first_order_end,
+        "orders_dropped": [round(math.log10(d), 3) for d in drop],
+        "mass_orders_per_second": round(math.log10(drop[0]) / t, 3),
+        "status": ["converged", "budget", "diverged"][run.status],
+    }
+    c = dev.conservation()
+    out["conservation"] = {
+        "mass_E": c["E"][0],
+        "mass_E_over_D": c["E"][0] / c["D"][0],
+        "E": [float(v) for v in c["E"]],
+        "D": [float(v) for v in c["D"]],
+    }
+    if w.policy == "fp32+acc64" and not args.no_fp64_check:
+        u = dev.download()
+        r32 = dev.residual(u)
+        dev.close()
+        dev64 = DeviceStepper(w.mesh, w.scheme, "fp64", w.bc_kind, w.bc, w.source)
+        r64 = dev64.residual(u)
+        dev64.close()
+        err = np.abs(r32 - r64).reshape(5, -1) / flux_scale(w)
+        out["fp32_vs_fp64"] = {
+            "state": f"after the {run.iterations}-iteration steady run",
+            "max_err_over_f_A": [float(e) for e in err.max(axis=1)],
+            "max_in_eps32": round(float(err.max() / np.finfo(np.float32).eps), 2),
+        }
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (tets 20, bump 20, vortex 200)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump"])
+    ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
+    ap.add_argument("--bump-size", default="462,154,154", help="bump: lattice nx,ny,nz")
+    ap.add_argument("--policy", default=None, choices=["fp32+acc64", "fp64"])
+    ap.add_argument("--flux", default=None, choices=["roe", "hllc", "ausm+up"])
+    ap.add_argument("--cpu-n", type=int, default=48, help="tets: cubes per side of the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--steady-iterations", type=int, default=400)
+    ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
+              "(the sharding design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+    defaults = {
+        "tets": ("fp32+acc64", "hllc", 20),
+        "vortex": ("fp64", "roe", 200),
+        "bump": ("fp32+acc64", "hllc", 20),
+    }[args.workload]
+    args.policy = args.policy or defaults[0]
+    args.flux = args.flux or defaults[1]
+    steps = args.steps or defaults[2]
+
+    import numpy as np
+
+    from autan_amd.stepper import DeviceStepper
+
+    size = tuple(int(v) for v in args.bump_size.split(",")) if args.workload == "bump" else None
+    w = make_workload(args.workload, args, size)
+    m = w.mesh
+    dev = DeviceStepper(m, w.scheme, w.policy, w.bc_kind, w.bc, w.source)
+    dev.upload(w.u)
+    log("uploaded; warming up")
+    warmup = 0 if args.workload == "vortex" else args.warmup
+    elapsed, r = timed_steps(dev, w, warmup, steps)
+    ms_step = elapsed / steps * 1e3
+    ns_cell = ms_step * 1e6 / m.n_cells
+    log(f"{steps} steps: {ms_step:.3f} ms/step, {ns_cell:.3f} ns/cell/step, norms {r.residuals}")
+    u_gpu = dev.download()
+    tag = {"tets": f"tets{args.n}", "vortex": "vortex32", "bump": "bump" + "x".join(map(str, size or ()))}[args.workload]
+    roof, step = roofline(dev, w, ms_step, tag)
+
+    extras = {}
     cpu = None
-    if not args.no_cpu:
-        log("CPU baseline (oracle, one thread)")
-        cpu = cpu_baseline(args.policy, args.flux, args.cpu_n, args.cpu_steps)
+    if args.workload == "vortex":
+        # The CPU oracle runs the same steps from the same state: the GPU's state must be its bits.
+        if not args.no_cpu:
+            log("CPU oracle: the same steps")
+            cpu, u_cpu = cpu_baseline(w, steps, w.description)
+            d = np.abs(u_gpu - u_cpu)
+            extras["parity"] = {"steps": steps, "max_abs_diff": float(d.max()), "bitwise": bool(np.array_equal(u_gpu, u_cpu))}
+    elif args.workload == "bump":
+        extras = bump_extras(dev, w, args)
+        if not args.no_cpu:
+            log("CPU baseline (oracle) on a lattice of half the size per axis")
+            nx, ny, nz = size
+            small = make_workload("bump", args, (nx // 4 * 2, ny // 2, nz // 4 * 2))
+            cpu, _ = cpu_baseline(small, args.cpu_steps, "the same channel on a lattice of half the size per axis")
+    elif not args.no_cpu:
+        log("CPU baseline (oracle)")
+        small = make_workload("tets", args, args.cpu_n)
+        cpu, _ = cpu_baseline(small, args.cpu_steps, f"the same workload at {args.cpu_n}^3 x 6 tets")
 
     line = {
         "metric": "ns/cell/iter",
         "value": round(ns_cell, 4),
         "unit": "ns/cell/iter",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_step, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64acc" if args.policy == "fp32+acc64" else "f64",
+        "dtype": "f32+f64acc" if w.policy == "fp32+acc64" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"Kuhn tetrahedra {args.n}^3 x 6 (jitter 10 %, seed 0), manufactured solution, "
-            f"second-order {args.flux.upper()}, face2 LSQ + Venkatakrishnan K=5, {args.policy}, SSP-RK3 CFL 0.5, "
-            f"{args.order} cell order",
+            "workload": w.description,
             "cells": m.n_cells,
             "faces": m.n_faces,
             "stencil_entries": m.n_stencil,
         },
         "cells_iter_per_s": round(m.n_cells / (ms_step * 1e-3), 1),
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dominant,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured,
-            "algorithmic_bytes_per_launch": kb[dominant],
-            "ms_per_launch": round(ms_launch, 4),
-        },
-        "step": {
-            "event_ms": round(prof["ms_per_step"], 4),
-            "algorithmic_bytes": step_b,
-            "achieved_GBs": round(step_b / (ms_step * 1e-3) / 1e9, 1),
-            "frac_of_peak": round(step_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "reference_B_step": ref_b,
-            "reference_B_step_frac": round(ref_b / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "stream_GBs": {k: round(v, 1) for k, v in peak.items()},
-            "kernels": kernels,
-        },
+        "roofline": roof,
+        "step": step,
         "cpu_baseline": cpu,
     }
+    if cpu is not None:
+        line["gpu_vs_cpu"] = {"serial": round(cpu["value"] / ns_cell, 1),
+                              "threaded": round(cpu["threaded"]["value"] / ns_cell, 1)}
+    line.update(extras)
     print(json.dumps(line), flush=True)
     return 0
 
