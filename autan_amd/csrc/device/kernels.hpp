@@ -76,13 +76,15 @@ struct Dev {
   C* lam;            // nc
   double* dt;        // nc
   Small* small;
-  // Block-local faces: the faces of the cells of logical block b are sface[sbeg[b] .. sbeg[b + 1])
-  // (ascending); incidence k of a cell reads its face's flux from LDS slot islot[k] & 0x3fff, bit
-  // 14 set where the cell is the face's neighbour, bit 15 on boundary faces.
+  // Block-local faces: the faces of the cells of logical block b are the slots sbeg[b] ..
+  // sbeg[b + 1] (ascending face order; a face on two blocks has a slot in each); incidence k of a
+  // cell reads its face's flux from LDS slot islot[k] & 0x3fff, bit 14 set where the cell is the
+  // face's neighbour, bit 15 on boundary faces.
   const int* cbeg;  // cells of face block b: [cbeg[b], cbeg[b + 1])
   const int* sbeg;
-  const int* sface;
   const unsigned short* islot;
+  const C* srec;  // per slot: the face's 12 values (face_core)
+  const C* smom;  // per slot: the face's nine moments (non-planar meshes)
   int max_slots;
   long long* nslots;  // kNormSlots x kSums x kWords exact-sum words, spread to avoid contention
   // Staged gradient: block b's table of cells then boundary faces tent[tbeg[b] .. tbeg[b + 1]),
@@ -396,22 +398,73 @@ __device__ __forceinline__ void load_rec(const C* __restrict__ rec, int c, C* g)
   load_n<C, 20>(rec + 20LL * c, g);
 }
 
-// {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for face f: the
-// area-weighted flux in C.
-// With LAMBDA it also leaves the face's convective spectral radius (kernels/euler.py:1091-1112)
-// as each side takes it: lam[0] = max(s_owner, s_other) A for the owner, lam[1] = max(s_other,
-// s_owner) A for the neighbour, s = |v . n| + c of the cells' (not reconstructed) primitives and
-// of the boundary state.
-template <class C, int FLUX, bool PLANAR, bool SECOND, bool LAMBDA = false>
-__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f, C* lam = nullptr) {
-  C geo[8];
-  load_n<C, 8>(d.fgeo + 8LL * f, geo);
-  const V3<C> n{geo[0], geo[1], geo[2]};
-  const C area = geo[3];
+// A face as the flux bodies read it, 12 values: unit normal and area, x_f - x_c of the left cell
+// and the left cell's index, x_f - x_c of the right cell and its index (the indices' bits held in
+// the compute type; a right index -(b + 1) marks boundary face ni + b, whose right offsets are 0).
+template <class C>
+__device__ __forceinline__ C index_bits(int i) {
+  if constexpr (std::is_same_v<C, float>) {
+    return __int_as_float(i);
+  } else {
+    return __longlong_as_double(static_cast<long long>(i));
+  }
+}
+
+template <class C>
+__device__ __forceinline__ int bits_index(C v) {
+  if constexpr (std::is_same_v<C, float>) {
+    return __float_as_int(v);
+  } else {
+    return static_cast<int>(__double_as_longlong(v));
+  }
+}
+
+// Where face_core reads the cells' primitives and gradient records: global memory, or LDS for the
+// cells [cb, cb + ncb) of a face block staged there (others from global memory).
+template <class C>
+struct GlobalCells {
+  const C* prim;
+  const C* rec;
+  __device__ __forceinline__ W5<C> w(int c) const { return load_w(prim, c); }
+  __device__ __forceinline__ void r(int c, C* g) const { load_rec(rec, c, g); }
+};
+
+template <class C>
+struct StagedCells {
+  const C* prim;
+  const C* rec;
+  const C* sp;  // LDS: 8 per staged cell
+  const C* sr;  // LDS: 20 per staged cell
+  int cb, ncb;
+  __device__ __forceinline__ W5<C> w(int c) const {
+    const unsigned l = static_cast<unsigned>(c - cb);
+    return l < static_cast<unsigned>(ncb) ? load_w(sp, static_cast<long long>(l)) : load_w(prim, c);
+  }
+  __device__ __forceinline__ void r(int c, C* g) const {
+    const unsigned l = static_cast<unsigned>(c - cb);
+    if (l < static_cast<unsigned>(ncb)) {
+      load_rec(sr, static_cast<int>(l), g);
+    } else {
+      load_rec(rec, c, g);
+    }
+  }
+};
+
+// {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for the face `g`
+// (12 values, above): the area-weighted flux in C. The warp terms read the face's nine moments at
+// m[q * ms]. With LAMBDA it also leaves the face's convective spectral radius
+// (kernels/euler.py:1091-1112) as each side takes it: lam[0] = max(s_owner, s_other) A for the
+// owner, lam[1] = max(s_other, s_owner) A for the neighbour, s = |v . n| + c of the cells' (not
+// reconstructed) primitives and of the boundary state.
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool LAMBDA, class Src>
+__device__ __forceinline__ F5<C> face_core(const Dev<C>& d, const Params<C>& pa, const Src& src, const C* g,
+                                           const C* m9, long long ms, C* lam) {
+  const V3<C> n{g[0], g[1], g[2]};
+  const C area = g[3];
+  const int left = bits_index(g[7]), right = bits_index(g[11]);
   F5<C> out;
-  if (f < d.ni) {
-    const int2 lr = d.lr[f];
-    W5<C> wl = load_w(d.prim, lr.x), wr = load_w(d.prim, lr.y);
+  if (right >= 0) {
+    W5<C> wl = src.w(left), wr = src.w(right);
     if constexpr (LAMBDA) {
       const C sl = wave_speed(wl, n, pa.g, pa.pref), sr = wave_speed(wr, n, pa.g, pa.pref);
       lam[0] = smax(sl, sr) * area;
@@ -420,12 +473,10 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
     C gl[20], gr[20];
     W5<C> fl = wl, fr = wr;
     if constexpr (SECOND) {
-      C rr[4];
-      load_n<C, 4>(d.frr + 4LL * f, rr);
-      load_rec(d.rec, lr.x, gl);
-      load_rec(d.rec, lr.y, gr);
-      fl = reconstruct(wl, gl, gl + 15, V3<C>{geo[4], geo[5], geo[6]}, pa.pref);
-      fr = reconstruct(wr, gr, gr + 15, V3<C>{rr[0], rr[1], rr[2]}, pa.pref);
+      src.r(left, gl);
+      src.r(right, gr);
+      fl = reconstruct(wl, gl, gl + 15, V3<C>{g[4], g[5], g[6]}, pa.pref);
+      fr = reconstruct(wr, gr, gr + 15, V3<C>{g[8], g[9], g[10]}, pa.pref);
     }
     const F5<C> fx = flux<FLUX>(fl, fr, n, pa.g, pa.minf, pa.pref);
 #pragma unroll
@@ -433,7 +484,7 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
     if constexpr (!PLANAR) {
       C m[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) m[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+      for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
       if constexpr (!SECOND) {
 #pragma unroll
         for (int q = 0; q < 20; ++q) gl[q] = gr[q] = C(0);
@@ -445,8 +496,8 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
       for (int k = 0; k < 5; ++k) out.f[k] = out.f[k] + wp.f[k];
     }
   } else {
-    const int b = f - d.ni;
-    const W5<C> wl = load_w(d.prim, d.bowner[b]);
+    const int b = -right - 1;
+    const W5<C> wl = src.w(left);
     if constexpr (LAMBDA) {
       const C sl = wave_speed(wl, n, pa.g, pa.pref), sb = wave_speed(load_w(d.prim_b, b), n, pa.g, pa.pref);
       lam[0] = smax(sl, sb) * area;
@@ -456,13 +507,13 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
     W5<C> wf = wl;
     if constexpr (SECOND || !PLANAR) {
       if constexpr (SECOND) {
-        load_rec(d.rec, d.bowner[b], gl);
+        src.r(left, gl);
       } else {
 #pragma unroll
         for (int q = 0; q < 20; ++q) gl[q] = C(0);
       }
     }
-    if constexpr (SECOND) wf = reconstruct(wl, gl, gl + 15, V3<C>{geo[4], geo[5], geo[6]}, pa.pref);
+    if constexpr (SECOND) wf = reconstruct(wl, gl, gl + 15, V3<C>{g[4], g[5], g[6]}, pa.pref);
     C data[8];
     load_n<C, 8>(d.bc + 8LL * b, data);
     const int kind = d.bkind[b];
@@ -475,7 +526,7 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
       if constexpr (!PLANAR) {
         C m[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) m[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+        for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
         const C p4 = gl[15 + 4];
         const C g4[3] = {gl[12], gl[13], gl[14]};
         const C ww[5] = {C(0.0), p4 * (m[0] * g4[0] + m[3] * g4[1] + m[6] * g4[2]),
@@ -492,6 +543,30 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
     }
   }
   return out;
+}
+
+// face_core for face f from the per-face arrays (fgeo, frr, lr, bowner, the SoA moments).
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool LAMBDA = false>
+__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f, C* lam = nullptr) {
+  C g[12];
+  load_n<C, 8>(d.fgeo + 8LL * f, g);
+  if (f < d.ni) {
+    const int2 lr = d.lr[f];
+    C rr[4];
+    load_n<C, 4>(d.frr + 4LL * f, rr);
+    g[7] = index_bits<C>(lr.x);
+    g[8] = rr[0];
+    g[9] = rr[1];
+    g[10] = rr[2];
+    g[11] = index_bits<C>(lr.y);
+  } else {
+    const int b = f - d.ni;
+    g[7] = index_bits<C>(d.bowner[b]);
+    g[8] = g[9] = g[10] = C(0);
+    g[11] = index_bits<C>(-b - 1);
+  }
+  return face_core<C, FLUX, PLANAR, SECOND, LAMBDA>(d, pa, GlobalCells<C>{d.prim, d.rec}, g,
+                                                   PLANAR ? nullptr : d.moment + f, d.nf, lam);
 }
 
 // The face loops' fluxes stored per face for the per-cell gather (the store half of the
@@ -809,24 +884,50 @@ __global__ void k_clear(Small* small, long long* nslots) {
 // from zero in FP64, onto -V s (kernels/euler.py:611-616): the store_gather order of the
 // reference's device path, with no face array in HBM. FIRST (the step's R(u)) also gathers the
 // spectral radius, the cell's time step and the block's least one, and the exact sums of R^2.
-template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST>
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE>
 __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double cfl, int store_dt,
                                                   unsigned long long* partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kComp = FIRST ? 7 : 5;
   const int ms = d.max_slots;
   C* lf = reinterpret_cast<C*>(smem);
-  long long* acc = reinterpret_cast<long long*>(smem + ((kComp * ms * static_cast<int>(sizeof(C)) + 15) & ~15));
+  const int lf_bytes = (kComp * ms * static_cast<int>(sizeof(C)) + 15) & ~15;
+  long long* acc = reinterpret_cast<long long*>(smem + lf_bytes);
   __shared__ unsigned long long wave_key[kBlock / kWave];
   const int blk = xcd_block();
   const int s0 = d.sbeg[blk], ns = d.sbeg[blk + 1] - s0;
   if constexpr (FIRST) {
     for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) acc[t] = 0;
   }
+  const int cb = d.cbeg[blk], ncb = d.cbeg[blk + 1] - cb;
+  // STAGE: the block's own cells' primitives (8 per cell) and gradient records (20) copied to LDS
+  // first, as contiguous runs; faces read them there and only their other blocks' cells from
+  // global memory.
+  C* sp = reinterpret_cast<C*>(smem + lf_bytes + (FIRST ? kSums * kWords * 8 : 0));
+  C* sr = sp + 8 * kBlock;
+  if constexpr (STAGE) {
+    constexpr int kVec = 16 / sizeof(C);
+    using V = std::conditional_t<std::is_same_v<C, float>, float4, double2>;
+    const V* gp = reinterpret_cast<const V*>(d.prim + 8LL * cb);
+    const V* gr = reinterpret_cast<const V*>(d.rec + 20LL * cb);
+    V* lp = reinterpret_cast<V*>(sp);
+    V* lr = reinterpret_cast<V*>(sr);
+    for (int i = threadIdx.x; i < ncb * (8 / kVec); i += kBlock) lp[i] = gp[i];
+    for (int i = threadIdx.x; i < ncb * (20 / kVec); i += kBlock) lr[i] = gr[i];
+    __syncthreads();
+  }
   for (int i = threadIdx.x; i < ns; i += kBlock) {
-    const int f = d.sface[s0 + i];
+    const long long sl = s0 + i;
+    C g[12];
+    load_n<C, 12>(d.srec + 12 * sl, g);
     C lam[2];
-    const F5<C> fx = face_flux<C, FLUX, PLANAR, SECOND, FIRST>(d, pa, f, lam);
+    const C* m9 = PLANAR ? nullptr : d.smom + 9 * sl;
+    F5<C> fx;
+    if constexpr (STAGE) {
+      fx = face_core<C, FLUX, PLANAR, SECOND, FIRST>(d, pa, StagedCells<C>{d.prim, d.rec, sp, sr, cb, ncb}, g, m9, 1, lam);
+    } else {
+      fx = face_core<C, FLUX, PLANAR, SECOND, FIRST>(d, pa, GlobalCells<C>{d.prim, d.rec}, g, m9, 1, lam);
+    }
 #pragma unroll
     for (int q = 0; q < 5; ++q) lf[q * ms + i] = fx.f[q];
     if constexpr (FIRST) {
@@ -835,8 +936,8 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double
     }
   }
   __syncthreads();
-  const int c = d.cbeg[blk] + threadIdx.x;
-  const bool active = c < d.cbeg[blk + 1];
+  const int c = cb + threadIdx.x;
+  const bool active = threadIdx.x < ncb;
   double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double lam_acc = 0.0;
   if (active) {

@@ -104,6 +104,7 @@ class StepperBase {
   virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
   virtual void face_force(const int* faces, long long n, double* force) = 0;
   virtual void conservation(double* out) = 0;
+  virtual void tune(const std::string& key, int value) = 0;
   virtual double area_scale() const = 0;
   virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
@@ -291,6 +292,18 @@ class Stepper final : public StepperBase {
     for (int i = 0; i < kConsSums; ++i) out[i] = exact_value(host.data() + std::size_t(i) * kWords);
   }
 
+  // Kernel variants the autotuner chooses between; every variant gives the same bits.
+  void tune(const std::string& key, int value) override {
+    if (key == "face_stage") {
+      if (value != 0 && !(std::is_same_v<C, float> && fused_ && faces_lds(true, true) <= 65536)) {
+        throw std::invalid_argument("face_stage: needs fp32+acc64 and the fused face kernel");
+      }
+      stage_faces_ = value != 0;
+      return;
+    }
+    throw std::invalid_argument("unknown tuning key '" + key + "'");
+  }
+
   double area_scale() const override { return area_scale_; }
   double gamma() const override { return sc_.gamma; }
 
@@ -381,7 +394,8 @@ class Stepper final : public StepperBase {
     d.small = small_.get();
     d.cbeg = cbeg_.get();
     d.sbeg = sbeg_.get();
-    d.sface = sface_.get();
+    d.srec = srec_.get();
+    d.smom = smom_.get();
     d.islot = islot_.get();
     d.max_slots = max_slots_;
     d.nslots = nslots_.get();
@@ -548,18 +562,19 @@ class Stepper final : public StepperBase {
     max_slots_ = max_slots;
     fused_ = fits && faces_lds(true) <= 65536;
     if (!fused_) return;
-    if (sface.empty()) sface.push_back(0);
     cbeg_.upload(cbeg, stream_);
     sbeg_.upload(sbeg, stream_);
-    sface_.upload(sface, stream_);
     islot_.upload(islot, stream_);
+    sface_host_ = std::move(sface);
   }
 
-  // Dynamic LDS of the fused face kernel.
-  std::size_t faces_lds(bool first) const {
+  // Dynamic LDS of the fused face kernel: the face fluxes (and lambdas), the exact-sum words of
+  // the first residual, the staged cells.
+  std::size_t faces_lds(bool first, bool stage = false) const {
     const std::size_t comp = first ? 7 : 5;
     const std::size_t fl = (comp * std::size_t(max_slots_) * sizeof(C) + 15) & ~std::size_t(15);
-    return fl + (first ? std::size_t(kSums) * kWords * sizeof(long long) : 0);
+    return fl + (first ? std::size_t(kSums) * kWords * sizeof(long long) : 0) +
+           (stage ? std::size_t(28) * kBlock * sizeof(C) : 0);
   }
 
   void build_stencil(const mxeuler_topology& t, const mxeuler_stencil& st) {
@@ -718,6 +733,7 @@ class Stepper final : public StepperBase {
     for (std::size_t c = 0; c < nc; ++c) vol[c] = static_cast<C>(g.cell_volume[c]);
     vol_.upload(vol, stream_);
     vol64_.upload(vol64, stream_);
+    if (fused_) build_slot_records(t, g);
     // FP64 x_f - x_c of the boundary faces, for the wall loads.
     std::vector<double> rb(3 * std::max<std::size_t>(nf - ni, 1), 0.0);
     for (std::size_t f = ni; f < nf; ++f) {
@@ -743,6 +759,49 @@ class Stepper final : public StepperBase {
     slen_host_.clear();
     slen_host_.shrink_to_fit();
     check(hipStreamSynchronize(stream_), "geometry");
+  }
+
+  // The fused face kernel's per-slot face records (face_core's 12 values) and moments, in slot
+  // order: each block reads its faces' geometry as one contiguous run.
+  void build_slot_records(const mxeuler_topology& t, const mxeuler_geometry& g) {
+    const std::size_t nf = std::size_t(nf_), ni = std::size_t(ni_);
+    const std::size_t ns = std::max<std::size_t>(sface_host_.size(), 1);
+    auto bits = [](int i) {
+      C v;
+      if constexpr (sizeof(C) == 4) {
+        const int32_t x = i;
+        std::memcpy(&v, &x, 4);
+      } else {
+        const int64_t x = i;
+        std::memcpy(&v, &x, 8);
+      }
+      return v;
+    };
+    std::vector<C> rec(12 * ns, C(0));
+    for (std::size_t sl = 0; sl < sface_host_.size(); ++sl) {
+      const auto f = std::size_t(sface_host_[sl]);
+      C* r = rec.data() + 12 * sl;
+      for (std::size_t u = 0; u < 3; ++u) {
+        r[u] = static_cast<C>(g.face_normal[u * nf + f]);
+        r[4 + u] = static_cast<C>(g.r_left[u * nf + f]);
+        r[8 + u] = f < ni ? static_cast<C>(g.r_right[u * nf + f]) : C(0);
+      }
+      r[3] = static_cast<C>(g.face_area[f]);
+      r[7] = bits(t.owner[f]);
+      r[11] = bits(f < ni ? t.neighbour[f] : -static_cast<int>(f - ni) - 1);
+    }
+    srec_.upload(rec, stream_);
+    if (!planar_) {
+      std::vector<C> m(9 * ns, C(0));
+      for (std::size_t sl = 0; sl < sface_host_.size(); ++sl) {
+        const auto f = std::size_t(sface_host_[sl]);
+        for (std::size_t q = 0; q < 9; ++q) m[9 * sl + q] = static_cast<C>(g.face_moment[q * nf + f]);
+      }
+      smom_.upload(m, stream_);
+    }
+    check(hipStreamSynchronize(stream_), "slot records");
+    sface_host_.clear();
+    sface_host_.shrink_to_fit();
   }
 
   void set_boundary(const double* bc_kind, const double* bc) {
@@ -826,11 +885,23 @@ class Stepper final : public StepperBase {
   template <int FLUX, bool PLANAR, bool SECOND>
   void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
     const int g = xcd_grid(face_blocks_);
+    if constexpr (std::is_same_v<C, float>) {
+      if (stage_faces_) {
+        if (first) {
+          k_faces<C, FLUX, PLANAR, SECOND, true, true><<<g, kBlock, faces_lds(true, true), stream_>>>(
+              d, pa_, cfl, local ? 1 : 0, partial_.get());
+        } else {
+          k_faces<C, FLUX, PLANAR, SECOND, false, true><<<g, kBlock, faces_lds(false, true), stream_>>>(d, pa_, cfl, 0,
+                                                                                                       nullptr);
+        }
+        return;
+      }
+    }
     if (first) {
-      k_faces<C, FLUX, PLANAR, SECOND, true><<<g, kBlock, faces_lds(true), stream_>>>(d, pa_, cfl, local ? 1 : 0,
-                                                                                   partial_.get());
+      k_faces<C, FLUX, PLANAR, SECOND, true, false><<<g, kBlock, faces_lds(true), stream_>>>(d, pa_, cfl, local ? 1 : 0,
+                                                                                          partial_.get());
     } else {
-      k_faces<C, FLUX, PLANAR, SECOND, false><<<g, kBlock, faces_lds(false), stream_>>>(d, pa_, cfl, 0, nullptr);
+      k_faces<C, FLUX, PLANAR, SECOND, false, false><<<g, kBlock, faces_lds(false), stream_>>>(d, pa_, cfl, 0, nullptr);
     }
   }
 
@@ -951,7 +1022,10 @@ class Stepper final : public StepperBase {
   Buffer<C> prim_, prim_b_, rec_, fflux_, lam_;
   Buffer<Small> small_;
   Buffer<unsigned long long> partial_;
-  Buffer<int> cbeg_, sbeg_, sface_;
+  Buffer<int> cbeg_, sbeg_;
+  std::vector<int> sface_host_;  // the faces of every slot, until the slot records are built
+  bool stage_faces_ = false;     // fp32: the face kernel stages its block's cells in LDS
+  Buffer<C> srec_, smom_;
   int face_blocks_ = 0;
   Buffer<int> tbeg_, tent_, lens_;
   Buffer<unsigned short> sloc_;
@@ -1196,6 +1270,13 @@ int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const dou
       force[k] = sums[k].value();
       moment[k] = sums[3 + k].value();
     }
+  });
+}
+
+int mxeuler_tune(mxeuler_stepper* s, const char* key, int32_t value, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !key) throw std::invalid_argument("null argument");
+    s->impl->tune(key, value);
   });
 }
 

@@ -327,6 +327,10 @@ class DeviceStepper:
         )
         return np.array(force[:]), np.array(moment[:])
 
+    def tune(self, key: str, value: int) -> None:
+        """Selects a kernel variant (same bits, different speed): "face_stage" 0/1."""
+        self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))
+
     def conservation(self) -> dict[str, np.ndarray]:
         """EulerPhysics::check_conservation's exact sums on the current state: E = sum R^F - sum
         F_b, D = sum |F| over the incidences, the boundary fluxes and sum R^F, 5 each."""

@@ -279,6 +279,7 @@ def main() -> int:
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--steady-iterations", type=int, default=400)
     ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
+    ap.add_argument("--face-stage", type=int, default=None, help="face kernel variant: stage the block's cells in LDS")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
@@ -301,6 +302,8 @@ def main() -> int:
     w = make_workload(args.workload, args, size)
     m = w.mesh
     dev = DeviceStepper(m, w.scheme, w.policy, w.bc_kind, w.bc, w.source)
+    if args.face_stage is not None:
+        dev.tune("face_stage", args.face_stage)
     dev.upload(w.u)
     log("uploaded; warming up")
     warmup = 0 if args.workload == "vortex" else args.warmup

@@ -115,6 +115,28 @@ def test_step_matches_oracle(name, policy, local):
         assert r.time_step == h
 
 
+@pytest.mark.parametrize("local", [False, True])
+@pytest.mark.parametrize("name", ["tets-hllc2", "poly-ausm2-warped", "bump-hybrid-hllc2"])
+def test_staged_face_kernel_gives_the_same_bits(name, local):
+    """The face kernel variant that stages its block's cells in LDS (fp32+acc64) computes the same
+    residual, norms and state as the unstaged one, bit for bit."""
+    case = CASES[name]()
+    plain, _ = build(case, "fp32+acc64")
+    staged, _ = build(case, "fp32+acc64")
+    staged.tune("face_stage", 1)
+    np.testing.assert_array_equal(staged.residual(case.u), plain.residual(case.u))
+    for dev in (plain, staged):
+        dev.upload(case.u)
+    for _ in range(2):
+        a = plain.step(cfl=0.5, local=local)
+        b = staged.step(cfl=0.5, local=local)
+        np.testing.assert_array_equal(a.residuals, b.residuals)
+        assert a.time_step == b.time_step
+    np.testing.assert_array_equal(plain.download(), staged.download())
+    with pytest.raises(RuntimeError, match="face_stage"):
+        build(case, "fp64")[0].tune("face_stage", 1)
+
+
 @pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
 def test_fixed_dt_steps_and_evaluate(policy):
     case = CASES["tets-hllc2"]()
