@@ -45,6 +45,7 @@ struct Params {
   double ref[5];
   double p_ref;  // FP64 p_ref for the validity check, rounded there as the reference rounds it
   double gamma;
+  long long cell0;  // the case's first cell: invalid cells are reported relative to it
 };
 
 template <class C>
@@ -75,7 +76,18 @@ struct Dev {
   double* res;       // 5 x nc
   C* lam;            // nc
   double* dt;        // nc
-  Small* small;
+  Small* small;  // one per case
+  // Cases: meshes stepped together in one index space, each case's cells starting on a 256-cell
+  // block (one case: ncase = 1 and no tables). Case k's parameters are ptab[k]; bcase gives the
+  // case of each 256-cell block, fcase of each face block, fcbeg each case's first face block;
+  // cflags bit 0 marks a case active (stepped), bit 1 at second order (null: all active).
+  const Params<C>* ptab;
+  const int* bcase;
+  const int* fcase;
+  const int* fcbeg;
+  const int* cflags;
+  int ncase;
+  int nslot;  // exact-sum slots per case
   // Block-local faces: the faces of the cells of logical block b are the slots sbeg[b] ..
   // sbeg[b + 1] (ascending face order; a face on two blocks has a slot in each); incidence k of a
   // cell reads its face's flux from LDS slot islot[k] & 0x3fff, bit 14 set where the cell is the
@@ -100,7 +112,8 @@ struct Dev {
   const C* ir2;
 };
 
-constexpr int kNormSlots = 64;
+constexpr int kNormSlots = 64;  // exact-sum slots of one case
+constexpr int kCaseSlots = 8;   // per case when several cases run together
 constexpr int kMaxSlots = 2 * kBlock;  // faces per face block
 
 // Workgroups are handed to the eight XCDs in turn (workgroup b to XCD b mod 8), each XCD with its
@@ -114,6 +127,23 @@ __device__ __forceinline__ int xcd_block() {
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 __device__ __forceinline__ int xcd_index() { return xcd_block() * kBlock + threadIdx.x; }
+
+template <class C>
+__device__ __forceinline__ int block_case(const Dev<C>& d, int blk) {
+  return d.bcase != nullptr ? d.bcase[blk] : 0;
+}
+template <class C>
+__device__ __forceinline__ int cell_case(const Dev<C>& d, int c) {
+  return d.bcase != nullptr ? d.bcase[c / kBlock] : 0;
+}
+template <class C>
+__device__ __forceinline__ bool case_active(const Dev<C>& d, int cs) {
+  return d.cflags == nullptr || (d.cflags[cs] & 1) != 0;
+}
+template <class C>
+__device__ __forceinline__ bool case_second(const Dev<C>& d, int cs) {
+  return d.cflags == nullptr || (d.cflags[cs] & 2) != 0;
+}
 
 template <class C>
 __device__ __forceinline__ W5<C> load_w(const C* __restrict__ p, long long i) {
@@ -202,19 +232,26 @@ __device__ __forceinline__ W5<C> primitive_of(const double* __restrict__ u, int 
 }
 
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_prim(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
-  const int c = xcd_index();
+__global__ __launch_bounds__(kBlock) void k_prim(Dev<C> d, const double* __restrict__ u) {
+  const int blk = xcd_block();
+  const int c = blk * kBlock + threadIdx.x;
   if (c >= d.nc) return;
-  store_w(d.prim, c, primitive_of(u, d.nc, c, pa));
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs)) return;
+  store_w(d.prim, c, primitive_of(u, d.nc, c, d.ptab[cs]));
 }
 
 // boundary_states (kernels/euler.py:442-466): the boundary value of every boundary face from its
 // owner's primitives.
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d, Params<C> pa) {
+__global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d) {
   const int b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= d.nb) return;
-  const W5<C> w = load_w(d.prim, d.bowner[b]);
+  const int owner = d.bowner[b];
+  const int cs = cell_case(d, owner);
+  if (!case_active(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
+  const W5<C> w = load_w(d.prim, owner);
   C geo[8], data[8];
   load_n<C, 8>(d.fgeo + 8LL * (d.ni + b), geo);
   load_n<C, 8>(d.bc + 8LL * b, data);
@@ -229,9 +266,13 @@ __global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d, Params<C> pa) {
 // Sums accumulate in FP64 and are stored in C; bounds and the limiter's minimum are taken in C,
 // ties keeping the first entry. Without the limiter phi = 1 (second order unlimited).
 template <class C, bool LIMITER>
-__global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d, Params<C> pa) {
-  const int c = xcd_index();
+__global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d) {
+  const int blk = xcd_block();
+  const int c = blk * kBlock + threadIdx.x;
   if (c >= d.nc) return;
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs) || !case_second(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
   const W5<C> wc = load_w(d.prim, c);
   const C wcv[5] = {wc.r, wc.u, wc.v, wc.w, wc.p};
   double acc[15];
@@ -309,10 +350,13 @@ __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d, Params<C> pa) {
 // Weights, positions and the limiter's face offsets stream in the wave-sliced layout. The sums,
 // bounds and minima run in the same order as k_grad.
 template <class C, bool LIMITER>
-__global__ __launch_bounds__(kBlock) void k_grad_lds(Dev<C> d, Params<C> pa) {
+__global__ __launch_bounds__(kBlock) void k_grad_lds(Dev<C> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   C* tab = reinterpret_cast<C*>(smem);
   const int blk = xcd_block();
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs) || !case_second(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
   const int t0 = d.tbeg[blk], tn = d.tbeg[blk + 1] - t0;
   for (int j = threadIdx.x; j < tn; j += kBlock) {
     const int e = d.tent[t0 + j];
@@ -547,7 +591,7 @@ __device__ __forceinline__ F5<C> face_core(const Dev<C>& d, const Params<C>& pa,
 
 // face_core for face f from the per-face arrays (fgeo, frr, lr, bowner, the SoA moments).
 template <class C, int FLUX, bool PLANAR, bool SECOND, bool LAMBDA = false>
-__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa, int f, C* lam = nullptr) {
+__device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, int f, C* lam = nullptr) {
   C g[12];
   load_n<C, 8>(d.fgeo + 8LL * f, g);
   if (f < d.ni) {
@@ -565,6 +609,7 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
     g[8] = g[9] = g[10] = C(0);
     g[11] = index_bits<C>(-b - 1);
   }
+  const Params<C>& pa = d.ptab[cell_case(d, bits_index(g[7]))];
   return face_core<C, FLUX, PLANAR, SECOND, LAMBDA>(d, pa, GlobalCells<C>{d.prim, d.rec}, g,
                                                    PLANAR ? nullptr : d.moment + f, d.nf, lam);
 }
@@ -572,10 +617,10 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, const Params<C>& pa,
 // The face loops' fluxes stored per face for the per-cell gather (the store half of the
 // reference's store_gather), for meshes whose blocks have too many faces for the fused kernel.
 template <class C, int FLUX, bool PLANAR, bool SECOND>
-__global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d, Params<C> pa) {
+__global__ __launch_bounds__(kBlock) void k_flux(Dev<C> d) {
   const int f = xcd_index();
   if (f >= d.nf) return;
-  const F5<C> out = face_flux<C, FLUX, PLANAR, SECOND>(d, pa, f);
+  const F5<C> out = face_flux<C, FLUX, PLANAR, SECOND>(d, f);
   C o8[8] = {out.f[0], out.f[1], out.f[2], out.f[3], out.f[4], C(0), C(0), C(0)};
   store_n<C, 8>(d.fflux + 8LL * f, o8);
 }
@@ -653,11 +698,12 @@ __device__ __forceinline__ T wave_sum(T v) {
 // physics/src/device_stepper.cuh:83-89) and each block's least dt_c as an ordered key in
 // partial[block]; k_min_key reduces those exactly (no same-address atomics from every wave).
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, Params<C> pa, double cfl, int store_dt,
+__global__ __launch_bounds__(kBlock) void k_spectral(Dev<C> d, double cfl, int store_dt,
                                                      unsigned long long* partial) {
   __shared__ unsigned long long wave_key[kBlock / kWave];
   const int blk = xcd_block();
   const int c = blk * kBlock + threadIdx.x;
+  const Params<C>& pa = d.ptab[block_case(d, blk)];
   unsigned long long key = ~0ULL;
   if (c < d.nc) {
     const W5<C> wc = load_w(d.prim, c);
@@ -743,15 +789,19 @@ __device__ __forceinline__ void flag_least(int* slot, bool flagged, int c) {
 // One SSP-RK3 stage (physics/src/device_stepper.cuh:92-103): next = stage - (h / V) R,
 // stage = a u + b next; on the first stage `stage` is u itself. Then either the primitives of the
 // new stage for the next residual, or (last stage) the validity of the new state and residual.
+// The time step is the cell's own (local), the case's least (use_hkey) or hvalue.
 template <class C, bool PRIM, bool CHECK>
-__global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, Params<C> pa, double a, double b,
-                                               const double* __restrict__ u, const double* stage_in,
-                                               double* stage_out, const unsigned long long* hkey, double hvalue,
+__global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, double a, double b, const double* __restrict__ u,
+                                               const double* stage_in, double* stage_out, int use_hkey, double hvalue,
                                                int local) {
-  const int c = xcd_index();
+  const int blk = xcd_block();
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
+  const int c = blk * kBlock + threadIdx.x;
   bool bad_state = false, bad_res = false;
   if (c < d.nc) {
-    const double h = local ? d.dt[c] : (hkey != nullptr ? key_value(*hkey) : hvalue);
+    const double h = local ? d.dt[c] : (use_hkey ? key_value(d.small[cs].hkey) : hvalue);
     const double hv = h / d.vol64[c];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -765,23 +815,44 @@ __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, Params<C> pa, double a,
     if constexpr (CHECK) bad_state = invalid_state(stage_out, d.nc, c, pa);
   }
   if constexpr (CHECK) {
-    flag_least(&d.small->invalid_state, bad_state, c);
-    flag_least(&d.small->invalid_residual, bad_res, c);
+    const int cl = c - static_cast<int>(pa.cell0);
+    flag_least(&d.small[cs].invalid_state, bad_state, cl);
+    flag_least(&d.small[cs].invalid_residual, bad_res, cl);
   }
+}
+
+// The gradient records of the active cases' cells cleared (their first-order state restored after
+// a second-order evaluation).
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_zero_rec(Dev<C> d) {
+  const int blk = xcd_block();
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs)) return;
+  const int c = blk * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  C z[20];
+#pragma unroll
+  for (int q = 0; q < 20; ++q) z[q] = C(0);
+  store_n<C, 20>(d.rec + 20LL * c, z);
 }
 
 // DeviceStepper::evaluate's validity checks alone.
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_check(Dev<C> d, Params<C> pa, const double* __restrict__ u) {
-  const int c = xcd_index();
+__global__ __launch_bounds__(kBlock) void k_check(Dev<C> d, const double* __restrict__ u) {
+  const int blk = xcd_block();
+  const int cs = block_case(d, blk);
+  if (!case_active(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
+  const int c = blk * kBlock + threadIdx.x;
   bool bad_state = false, bad_res = false;
   if (c < d.nc) {
     bad_state = invalid_state(u, d.nc, c, pa);
 #pragma unroll
     for (int k = 0; k < 5; ++k) bad_res = bad_res || !__builtin_isfinite(d.res[static_cast<long long>(k) * d.nc + c]);
   }
-  flag_least(&d.small->invalid_state, bad_state, c);
-  flag_least(&d.small->invalid_residual, bad_res, c);
+  const int cl = c - static_cast<int>(pa.cell0);
+  flag_least(&d.small[cs].invalid_state, bad_state, cl);
+  flag_least(&d.small[cs].invalid_residual, bad_res, cl);
 }
 
 // ------------------------------------------------------------------------ exact norms
@@ -862,17 +933,19 @@ __global__ __launch_bounds__(kBlock) void k_norms(const double* __restrict__ res
 }
 
 // Clears the step's results: block 0 the Small record, block b > 0 exact-sum slot b - 1.
-__global__ void k_clear(Small* small, long long* nslots) {
-  if (blockIdx.x > 0) {
-    long long* w = nslots + static_cast<long long>(blockIdx.x - 1) * kSums * kWords;
+// Block k < ncase clears case k's record, block ncase + j the exact-sum slot j.
+__global__ void k_clear(Small* small, int ncase, long long* nslots) {
+  if (static_cast<int>(blockIdx.x) >= ncase) {
+    long long* w = nslots + static_cast<long long>(blockIdx.x - ncase) * kSums * kWords;
     for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) w[t] = 0;
     return;
   }
-  for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) (&small->words[0][0])[t] = 0;
+  Small* sm = small + blockIdx.x;
+  for (int t = threadIdx.x; t < kSums * kWords; t += blockDim.x) (&sm->words[0][0])[t] = 0;
   if (threadIdx.x == 0) {
-    small->hkey = 0xfff0000000000000ULL;  // order_key(+inf): no cell, an infinite step
-    small->invalid_state = INT32_MAX;
-    small->invalid_residual = INT32_MAX;
+    sm->hkey = 0xfff0000000000000ULL;  // order_key(+inf): no cell, an infinite step
+    sm->invalid_state = INT32_MAX;
+    sm->invalid_residual = INT32_MAX;
   }
 }
 
@@ -885,7 +958,7 @@ __global__ void k_clear(Small* small, long long* nslots) {
 // reference's device path, with no face array in HBM. FIRST (the step's R(u)) also gathers the
 // spectral radius, the cell's time step and the block's least one, and the exact sums of R^2.
 template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE>
-__global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double cfl, int store_dt,
+__global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, double cfl, int store_dt,
                                                   unsigned long long* partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kComp = FIRST ? 7 : 5;
@@ -895,6 +968,9 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double
   long long* acc = reinterpret_cast<long long*>(smem + lf_bytes);
   __shared__ unsigned long long wave_key[kBlock / kWave];
   const int blk = xcd_block();
+  const int cs = d.fcase != nullptr ? d.fcase[blk] : 0;
+  if (!case_active(d, cs)) return;
+  const Params<C>& pa = d.ptab[cs];
   const int s0 = d.sbeg[blk], ns = d.sbeg[blk + 1] - s0;
   if constexpr (FIRST) {
     for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) acc[t] = 0;
@@ -994,7 +1070,7 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double
       }
     }
     __syncthreads();
-    long long* out = d.nslots + static_cast<long long>(blk % kNormSlots) * kSums * kWords;
+    long long* out = d.nslots + (static_cast<long long>(cs) * d.nslot + blk % d.nslot) * kSums * kWords;
     for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) {
       const long long v = acc[t];
       if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(out + t), static_cast<unsigned long long>(v));
@@ -1002,25 +1078,30 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, Params<C> pa, double
   }
 }
 
-// The step's reductions, one block: the exact-sum slots added into small->words (integer sums:
-// any order gives the same words), and the least of the blocks' time-step keys.
-__global__ __launch_bounds__(1024) void k_step_reduce(const long long* __restrict__ nslots,
-                                                      const unsigned long long* __restrict__ partial, int n,
+// The step's reductions, one block per case: its exact-sum slots added into its words (integer
+// sums: any order gives the same words), and the least of its face blocks' time-step keys.
+__global__ __launch_bounds__(1024) void k_step_reduce(const long long* __restrict__ nslots, int nslot,
+                                                      const unsigned long long* __restrict__ partial,
+                                                      const int* __restrict__ fcbeg, const int* __restrict__ cflags,
                                                       Small* small) {
   __shared__ unsigned long long wave_key[1024 / kWave];
+  const int cs = blockIdx.x;
+  if (cflags != nullptr && (cflags[cs] & 1) == 0) return;
+  Small* sm = small + cs;
+  const long long* slots = nslots + static_cast<long long>(cs) * nslot * kSums * kWords;
   for (int t = threadIdx.x; t < kSums * kWords; t += 1024) {
     long long v = 0;
-    for (int q = 0; q < kNormSlots; ++q) v += nslots[static_cast<long long>(q) * kSums * kWords + t];
-    (&small->words[0][0])[t] += v;
+    for (int q = 0; q < nslot; ++q) v += slots[static_cast<long long>(q) * kSums * kWords + t];
+    (&sm->words[0][0])[t] += v;
   }
   unsigned long long m = ~0ULL;
-  for (int i = threadIdx.x; i < n; i += 1024) m = partial[i] < m ? partial[i] : m;
+  for (int i = fcbeg[cs] + threadIdx.x; i < fcbeg[cs + 1]; i += 1024) m = partial[i] < m ? partial[i] : m;
   m = wave_min(m);
   if ((threadIdx.x % kWave) == 0) wave_key[threadIdx.x / kWave] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 0; w < 1024 / kWave; ++w) m = wave_key[w] < m ? wave_key[w] : m;
-    if (m < small->hkey) small->hkey = m;
+    if (m < sm->hkey) sm->hkey = m;
   }
 }
 
@@ -1082,13 +1163,14 @@ __global__ __launch_bounds__(kBlock) void k_conservation(Dev<C> d, long long* __
 // where the reconstructed density or pressure is not positive) and the pressure force
 // p_f S + phi_p M^T grad p. rb: FP64 x_f - x_c of every boundary face (3 per face); out: 3 x n SoA.
 template <class C>
-__global__ __launch_bounds__(kBlock) void k_face_force(Dev<C> d, Params<C> pa, const int* __restrict__ faces, int n,
+__global__ __launch_bounds__(kBlock) void k_face_force(Dev<C> d, const int* __restrict__ faces, int n,
                                                       const double* __restrict__ rb, double* __restrict__ out) {
   const int i = static_cast<int>(blockIdx.x) * kBlock + static_cast<int>(threadIdx.x);
   if (i >= n) return;
   const int f = faces[i];
   const int b = f - d.ni;
   const long long c = d.bowner[b];
+  const Params<C>& pa = d.ptab[cell_case(d, static_cast<int>(c))];
   const C* g = d.rec + 20 * c;
   double drho = 0.0, dp = 0.0;
   for (int j = 0; j < 3; ++j) {

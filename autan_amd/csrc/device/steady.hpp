@@ -185,137 +185,175 @@ inline void call(int rc, const char* what) {
   if (rc != 0) throw std::runtime_error(std::string("steady run: ") + what + " failed");
 }
 
-// One steady run. history (5 per iteration) and provisional (1 per iteration) may be null.
-inline void run(const mxsteady_ops& ops, const mxsteady_options& o, mxsteady_result& out, double* history,
-                int32_t* provisional) {
-  if (!ops.step || !ops.evaluate || !ops.set_second_order) throw std::invalid_argument("steady run: missing callback");
+inline void check_options(const mxsteady_options& o) {
   if (o.max_iterations < 1) throw std::invalid_argument("steady run: max_iterations must be >= 1");
   if (!(o.residual_orders > 0.0)) throw std::invalid_argument("steady run: residual_orders must be > 0");
   if (!(o.cfl > 0.0)) throw std::invalid_argument("steady run: cfl must be > 0");
   if (o.start_order != 1 && o.start_order != 2) throw std::invalid_argument("steady run: start_order is 1 or 2");
-  out = mxsteady_result{};
-  out.diverged_cell = -1;
-  const bool levels = o.level_rule != 0;
-  std::array<double, kEq> floors{};  // the runtime's convergence floors (fp32+acc64), none: zeros
-  const bool has_floor = levels && o.scales != nullptr;
-  if (has_floor) {
+  if (o.level_rule && o.scales != nullptr) {
     for (int e = 0; e < kEq; ++e) {
       if (!(std::isfinite(o.scales[e]) && o.scales[e] >= 0.0)) throw std::invalid_argument("steady run: bad scale");
-      floors[e] = kRoundOffFloor * o.scales[e];
     }
   }
-  // The physics' side: order, exemption, drops and the first-order start.
-  bool second = o.start_order == 2;
-  call(ops.set_second_order(ops.ctx, second ? 1 : 0), "set_second_order");
-  Exemption phys;
-  std::vector<double> first_norms;
-  FirstOrderStart start(levels);
-  bool checked = false;
-  // The runtime's side.
-  Exemption runtime;
-  std::array<double, kEq> first{};
-  bool have_first = false;
-  if (o.reference != nullptr) {
-    std::copy(o.reference, o.reference + kEq, first.begin());
-    have_first = true;
-  }
-  std::vector<ResidualPlateau> plateaus(kEq);
-  const double drop_to = std::pow(10.0, -o.residual_orders);
+}
 
-  for (int64_t it = 1; it <= o.max_iterations; ++it) {
-    const bool prov = !second;
-    double reported[kEq];
-    bool have_reported = false;
-    int32_t invalid = -1;
-    bool stepped = false;
-    if (!checked) {
-      checked = true;
-      int32_t inv = -1;
-      call(ops.evaluate(ops.ctx, reported, &inv), "evaluate");
-      if (inv >= 0) {
-        invalid = inv;
-        have_reported = true;
+// One steady run's decisions, iteration by iteration: the physics' side (exemption, relative
+// drop, first-order start) and the runtime's (history, divergence, convergence).
+class SteadyCase {
+ public:
+  explicit SteadyCase(const mxsteady_options& o) : o_(o), start_(o.level_rule != 0) {
+    check_options(o);
+    out_.diverged_cell = -1;
+    second_ = o.start_order == 2;
+    has_floor_ = o.level_rule != 0 && o.scales != nullptr;
+    for (int e = 0; e < kEq && has_floor_; ++e) floors_[std::size_t(e)] = kRoundOffFloor * o.scales[e];
+    if (o.reference != nullptr) {
+      std::copy(o.reference, o.reference + kEq, first_.begin());
+      have_first_ = true;
+    }
+    plateaus_.resize(kEq);
+    drop_to_ = std::pow(10.0, -o.residual_orders);
+  }
+
+  bool second() const { return second_; }
+  bool done() const { return done_; }
+  // Whether the last record() ended the first-order start.
+  bool switched() const { return switched_; }
+  const mxsteady_result& result() const { return out_; }
+
+  // The initial state is invalid (the first evaluate's check): the run diverges before stepping.
+  void diverged_before(int32_t invalid, const double* reported) {
+    std::copy(reported, reported + kEq, out_.residuals);
+    finish(MXSTEADY_DIVERGED);
+    out_.diverged_cell = invalid;
+  }
+
+  // One iteration: r the step's norms (of R(u) at the order solved), reported what the history
+  // records (the second-order residual of the same state during the first-order start; null: r),
+  // invalid the step's invalid cell (-1: none). history (5 per iteration) and provisional may be
+  // null. Returns done().
+  bool record(const double* r, const double* reported, int32_t invalid, double* history, int32_t* provisional) {
+    switched_ = false;
+    const bool prov = !second_;
+    const double* rep = reported != nullptr ? reported : r;
+    // EulerPhysics::relative_drop on the residuals the runtime receives, then switch_order.
+    phys_.update(rep);
+    if (first_norms_.empty()) first_norms_.assign(r, r + kEq);
+    double drop = 0.0;
+    for (int k = 0; k < kEq; ++k) {
+      if (phys_.exempt(k)) continue;
+      const double from = phys_.woken(k) ? phys_.woken_reference(k) : first_norms_[std::size_t(k)];
+      if (from > 0.0) drop = std::max(drop, r[k] / from);
+    }
+    if (!second_) {
+      double lf[kEq] = {};
+      if (o_.level_rule && o_.scales != nullptr) {
+        for (int k = 0; k < kEq; ++k) lf[k] = phys_.exempt(k) ? 0.0 : kRoundOffFloor * o_.scales[k];
+      }
+      if (start_.ends_after(drop, r, lf)) {
+        second_ = true;
+        switched_ = true;
+        first_norms_.clear();
+        out_.first_order_end = start_.end();
       }
     }
-    if (invalid < 0) {
-      have_reported = false;
-      if (!second) {
-        int32_t ignored = -1;
-        call(ops.set_second_order(ops.ctx, 1), "set_second_order");
-        call(ops.evaluate(ops.ctx, reported, &ignored), "evaluate");
-        call(ops.set_second_order(ops.ctx, 0), "set_second_order");
-        have_reported = true;
-      }
-      double r[kEq];
-      call(ops.step(ops.ctx, o.cfl, r, &invalid), "step");
-      stepped = true;
-      // relative_drop, on the residuals the runtime receives.
-      const double* rep = have_reported ? reported : r;
-      phys.update(rep);
-      if (first_norms.empty()) first_norms.assign(r, r + kEq);
-      double drop = 0.0;
-      for (int k = 0; k < kEq; ++k) {
-        if (phys.exempt(k)) continue;
-        const double from = phys.woken(k) ? phys.woken_reference(k) : first_norms[std::size_t(k)];
-        if (from > 0.0) drop = std::max(drop, r[k] / from);
-      }
-      // switch_order
-      if (!second) {
-        double lf[kEq] = {};
-        if (levels && o.scales != nullptr) {
-          for (int k = 0; k < kEq; ++k) lf[k] = phys.exempt(k) ? 0.0 : kRoundOffFloor * o.scales[k];
-        }
-        if (start.ends_after(drop, r, lf)) {
-          second = true;
-          call(ops.set_second_order(ops.ctx, 1), "set_second_order");
-          first_norms.clear();
-          out.first_order_end = start.end();
-        }
-      }
-      if (!have_reported) std::copy(r, r + kEq, reported);
-    }
-    (void)stepped;
+    // The runtime's loop: divergence, history, convergence.
     bool finite = invalid < 0;
-    for (int e = 0; e < kEq; ++e) finite = finite && std::isfinite(reported[e]);
-    std::copy(reported, reported + kEq, out.residuals);
+    for (int e = 0; e < kEq; ++e) finite = finite && std::isfinite(rep[e]);
+    std::copy(rep, rep + kEq, out_.residuals);
     if (!finite) {
-      out.status = MXSTEADY_DIVERGED;
-      out.diverged_cell = invalid;
-      std::copy(first.begin(), first.end(), out.reference);
-      return;
+      finish(MXSTEADY_DIVERGED);
+      out_.diverged_cell = invalid;
+      return true;
     }
-    if (prov) ++out.first_order_steps;
-    if (history != nullptr) std::copy(reported, reported + kEq, history + kEq * (it - 1));
-    if (provisional != nullptr) provisional[it - 1] = prov ? 1 : 0;
-    out.iterations = it;
-    if (!have_first) {
-      std::copy(reported, reported + kEq, first.begin());
-      have_first = true;
+    const int64_t it = out_.iterations;
+    if (prov) ++out_.first_order_steps;
+    if (history != nullptr) std::copy(rep, rep + kEq, history + kEq * it);
+    if (provisional != nullptr) provisional[it] = prov ? 1 : 0;
+    out_.iterations = it + 1;
+    if (!have_first_) {
+      std::copy(rep, rep + kEq, first_.begin());
+      have_first_ = true;
     }
-    runtime.reference(first.data());
-    runtime.update(reported);
+    runtime_.reference(first_.data());
+    runtime_.update(rep);
     bool converged = true;
     int32_t at_plateau = 0;
     for (int e = 0; e < kEq; ++e) {
-      const double r = reported[e];
-      if (prov) plateaus[std::size_t(e)].reset();
-      const bool flat = !prov && plateaus[std::size_t(e)].add(r);
-      const bool met = runtime.exempt(e) ||
-                       r <= (runtime.woken(e) ? runtime.woken_reference(e) : runtime.drop_reference(e, first.data())) *
-                                drop_to;
-      const bool plateau = has_floor && flat && r <= kRoundOffBand * floors[std::size_t(e)];
+      const double v = rep[e];
+      if (prov) plateaus_[std::size_t(e)].reset();
+      const bool flat = !prov && plateaus_[std::size_t(e)].add(v);
+      const bool met = runtime_.exempt(e) ||
+                       v <= (runtime_.woken(e) ? runtime_.woken_reference(e) : runtime_.drop_reference(e, first_.data())) *
+                                drop_to_;
+      const bool plateau = has_floor_ && flat && v <= kRoundOffBand * floors_[std::size_t(e)];
       if (!met && plateau) at_plateau |= 1 << e;
       converged = converged && (met || plateau);
     }
     if (converged) {
-      out.status = MXSTEADY_CONVERGED;
-      out.plateau_mask = at_plateau;
-      std::copy(first.begin(), first.end(), out.reference);
+      out_.plateau_mask = at_plateau;
+      finish(MXSTEADY_CONVERGED);
+      return true;
+    }
+    if (out_.iterations >= o_.max_iterations) {
+      finish(MXSTEADY_BUDGET);
+      return true;
+    }
+    return false;
+  }
+
+ private:
+  void finish(int32_t status) {
+    out_.status = status;
+    std::copy(first_.begin(), first_.end(), out_.reference);
+    done_ = true;
+  }
+
+  mxsteady_options o_;
+  mxsteady_result out_{};
+  bool second_ = false, switched_ = false, done_ = false, has_floor_ = false, have_first_ = false;
+  std::array<double, kEq> floors_{}, first_{};
+  double drop_to_ = 0.0;
+  Exemption phys_, runtime_;
+  std::vector<double> first_norms_;
+  FirstOrderStart start_;
+  std::vector<ResidualPlateau> plateaus_;
+};
+
+// One steady run through the callbacks (EulerPhysics::device_step and the runtime's loop).
+// history (5 per iteration) and provisional (1 per iteration) may be null.
+inline void run(const mxsteady_ops& ops, const mxsteady_options& o, mxsteady_result& out, double* history,
+                int32_t* provisional) {
+  if (!ops.step || !ops.evaluate || !ops.set_second_order) throw std::invalid_argument("steady run: missing callback");
+  SteadyCase sc(o);
+  call(ops.set_second_order(ops.ctx, sc.second() ? 1 : 0), "set_second_order");
+  {
+    // As the host's step: a stored state the kernels cannot evaluate diverges at once.
+    double reported[kEq];
+    int32_t inv = -1;
+    call(ops.evaluate(ops.ctx, reported, &inv), "evaluate");
+    if (inv >= 0) {
+      sc.diverged_before(inv, reported);
+      out = sc.result();
       return;
     }
   }
-  out.status = MXSTEADY_BUDGET;
-  std::copy(first.begin(), first.end(), out.reference);
+  while (!sc.done()) {
+    double reported[kEq];
+    const bool prov = !sc.second();
+    if (prov) {
+      int32_t ignored = -1;
+      call(ops.set_second_order(ops.ctx, 1), "set_second_order");
+      call(ops.evaluate(ops.ctx, reported, &ignored), "evaluate");
+      call(ops.set_second_order(ops.ctx, 0), "set_second_order");
+    }
+    double r[kEq];
+    int32_t invalid = -1;
+    call(ops.step(ops.ctx, o.cfl, r, &invalid), "step");
+    sc.record(r, prov ? reported : nullptr, invalid, history, provisional);
+    if (sc.switched()) call(ops.set_second_order(ops.ctx, 1), "set_second_order");
+  }
+  out = sc.result();
 }
 
 }  // namespace mxe::steady

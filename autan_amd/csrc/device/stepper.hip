@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -106,6 +107,10 @@ class StepperBase {
   virtual void conservation(double* out) = 0;
   virtual void tune(const std::string& key, int value) = 0;
   virtual double area_scale() const = 0;
+  virtual int ncase() const = 0;
+  virtual void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) = 0;
+  virtual void set_flags(const int* flags) = 0;
+  virtual void batch_steady(const mxsteady_options& o, mxsteady_result* results, double* history) = 0;
   virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
                       int max_kernels, std::string* names) = 0;
@@ -116,8 +121,24 @@ class Stepper final : public StepperBase {
  public:
   Stepper(const mxeuler_scheme& sc, const mxeuler_topology& t, const mxeuler_geometry& geo, const mxeuler_stencil& st,
           const double* bc_kind, const double* bc, const double* source, hipStream_t stream)
-      : sc_(sc) {
-    validate(sc, t, geo, st);
+      : Stepper(&sc, 1, std::array<long long, 2>{0, t.n_cells}.data(), std::array<long long, 1>{t.n_cells}.data(), t, geo,
+                st, bc_kind, bc, source, stream) {}
+
+  // ncase cases in one index space: case k's cells are [case_first[k], case_first[k] +
+  // case_cells[k]), the rest of its range up to case_first[k + 1] filler cells without faces or
+  // stencil; every case_first a multiple of kBlock, the last n_cells. The cases share the flux,
+  // order, limiter, planarity and gas; their reference states, scales and cut-off Mach numbers are
+  // their own.
+  Stepper(const mxeuler_scheme* schemes, int ncase, const long long* case_first, const long long* case_cells,
+          const mxeuler_topology& t, const mxeuler_geometry& geo, const mxeuler_stencil& st, const double* bc_kind,
+          const double* bc, const double* source, hipStream_t stream)
+      : sc_(schemes[0]) {
+    validate(schemes[0], t, geo, st);
+    validate_cases(schemes, ncase, case_first, case_cells, t.n_cells);
+    ncase_ = ncase;
+    nslot_ = ncase > 1 ? kCaseSlots : kNormSlots;
+    case_first_.assign(case_first, case_first + ncase + 1);
+    case_cells_.assign(case_cells, case_cells + ncase);
     if (stream != nullptr) {
       stream_ = stream;
     } else {
@@ -128,10 +149,23 @@ class Stepper final : public StepperBase {
     nf_ = t.n_faces;
     ni_ = t.n_interior;
     nb_ = nf_ - ni_;
-    second_ = sc.second_order != 0;
-    planar_ = sc.planar != 0;
-    params(sc);
+    second_ = sc_.second_order != 0;
+    planar_ = sc_.planar != 0;
+    for (int k = 0; k < ncase; ++k) ptab_host_.push_back(params(schemes[k], case_first[k]));
+    ptab_.upload(ptab_host_, stream_);
+    if (ncase > 1) {
+      std::vector<int> bcase(std::size_t(grid(nc_)), ncase - 1);
+      for (int k = 0; k < ncase; ++k) {
+        for (long long c = case_first[k]; c < case_first[k + 1]; c += kBlock) bcase[std::size_t(c / kBlock)] = k;
+      }
+      bcase_.upload(bcase, stream_);
+      cflags_host_.assign(std::size_t(ncase), 1 | (second_ ? 2 : 0));
+      cflags_.upload(cflags_host_, stream_);
+      eflags_.upload(cflags_host_, stream_);
+      small_eval_ = Buffer<Small>(std::size_t(ncase));
+    }
     build_topology(t);
+    if (ncase > 1 && !fused_) throw std::invalid_argument("cases need the fused face kernel (blocks with too many faces)");
     build_stencil(t, st);
     build_geometry(t, geo);
     set_boundary(bc_kind, bc);
@@ -146,10 +180,11 @@ class Stepper final : public StepperBase {
     fflux_ = Buffer<C>(8 * std::size_t(nf_));
     lam_ = Buffer<C>(std::size_t(nc_));
     dt_ = Buffer<double>(std::size_t(nc_));
-    small_ = Buffer<Small>(1);
+    small_ = Buffer<Small>(std::size_t(ncase_));
     partial_ = Buffer<unsigned long long>(std::size_t(std::max(grid(nc_), xcd_grid(face_blocks_))));
-    nslots_ = Buffer<long long>(std::size_t(kNormSlots) * kSums * kWords);
-    check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small), hipHostMallocDefault), "hipHostMalloc");
+    nslots_ = Buffer<long long>(std::size_t(ncase_) * nslot_ * kSums * kWords);
+    check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small) * std::size_t(ncase_), hipHostMallocDefault),
+          "hipHostMalloc");
     check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "create");
   }
@@ -182,14 +217,14 @@ class Stepper final : public StepperBase {
             int* invalid) override {
     enqueue_step(cfl, has_dt, dt, local);
     finish(residuals, invalid);
-    *time_step = has_dt ? dt : key_value(host_small_->hkey);
+    *time_step = has_dt ? dt : key_value(host_small_[0].hkey);
   }
 
   void evaluate(double* residuals, int* invalid) override {
     clear();
     launch_prim(u_.get());
     residual_from_prim(true, 1.0, false);
-    launch("check", [&] { k_check<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), pa_, u_.get()); });
+    launch("check", [&] { k_check<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), u_.get()); });
     finish(residuals, invalid);
   }
 
@@ -262,7 +297,7 @@ class Stepper final : public StepperBase {
     residual_from_prim();
     const int ni = static_cast<int>(n);
     launch("face_force", [&] {
-      k_face_force<C><<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(dev(), pa_, list.get(), ni, rb_.get(),
+      k_face_force<C><<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(dev(), list.get(), ni, rb_.get(),
                                                                           out.get());
     });
     out.download(force, 3 * static_cast<std::size_t>(n), stream_);
@@ -305,6 +340,123 @@ class Stepper final : public StepperBase {
   }
 
   double area_scale() const override { return area_scale_; }
+  int ncase() const override { return ncase_; }
+
+  // One SSP-RK3 step of every active case: norms, time step and invalid cell per case.
+  void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) override {
+    enqueue_step(cfl, false, 0.0, local);
+    finish(residuals, invalid);
+    for (int k = 0; k < ncase_; ++k) time_steps[k] = key_value(host_small_[k].hkey);
+  }
+
+  // Per case: bit 0 active (stepped; a case made inactive keeps its state), bit 1 second order
+  // (a case turned to first order has its gradients cleared).
+  void set_flags(const int* flags) override {
+    if (ncase_ == 1) throw std::invalid_argument("case flags need a batch");
+    for (int k = 0; k < ncase_; ++k) {
+      if (flags[k] < 0 || flags[k] > 3) throw std::invalid_argument("case flags: bit 0 active, bit 1 second order");
+      if ((flags[k] & 2) && !second_) throw std::invalid_argument("case flags: the batch was built at first order");
+    }
+    for (int k = 0; k < ncase_; ++k) {
+      const int old = cflags_host_[std::size_t(k)];
+      const std::size_t first = std::size_t(case_first_[std::size_t(k)]);
+      const std::size_t range = std::size_t(case_first_[std::size_t(k) + 1]) - first;
+      if ((old & 1) && !(flags[k] & 1)) {
+        // Both state buffers hold the frozen state, whichever the next step reads.
+        for (std::size_t q = 0; q < 5; ++q) {
+          const std::size_t at = q * std::size_t(nc_) + first;
+          check(hipMemcpyAsync(stage_.get() + at, u_.get() + at, range * sizeof(double), hipMemcpyDeviceToDevice,
+                               stream_),
+                "hipMemcpyAsync");
+        }
+      }
+      if ((old & 2) && !(flags[k] & 2)) {
+        check(hipMemsetAsync(rec_.get() + 20 * first, 0, 20 * range * sizeof(C), stream_), "hipMemsetAsync");
+      }
+      cflags_host_[std::size_t(k)] = flags[k];
+    }
+    cflags_.upload(cflags_host_.data(), cflags_host_.size(), stream_);
+    check(hipStreamSynchronize(stream_), "set_flags");
+  }
+
+  // Norms of R(u) at second order for the cases marked in `which` (the others skipped), then
+  // those cases' gradients cleared: the residual a first-order start reports.
+  void batch_evaluate_second(const std::vector<int>& which, double* reported) {
+    std::vector<int> ef(static_cast<std::size_t>(ncase_), 0);
+    for (int k = 0; k < ncase_; ++k) ef[std::size_t(k)] = which[std::size_t(k)] ? 3 : 0;
+    eflags_.upload(ef.data(), ef.size(), stream_);
+    eval_mode_ = true;
+    try {
+      clear();
+      launch_prim(u_.get());
+      residual_from_prim(true, 1.0, false);
+      launch("zero_rec", [&] { k_zero_rec<C><<<grid(nc_), kBlock, 0, stream_>>>(dev()); });
+    } catch (...) {
+      eval_mode_ = false;
+      throw;
+    }
+    eval_mode_ = false;
+    check(hipMemcpyAsync(host_small_, small_eval_.get(), sizeof(Small) * std::size_t(ncase_), hipMemcpyDeviceToHost,
+                         stream_),
+          "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "evaluate");
+    for (int cs = 0; cs < ncase_; ++cs) {
+      for (int k = 0; k < kSums; ++k) {
+        reported[kSums * cs + k] =
+            std::sqrt(exact_value(host_small_[cs].words[k]) / double(case_cells_[std::size_t(cs)]));
+      }
+    }
+  }
+
+  // Every case's steady run together: each case's first-order start, local time steps and
+  // convergence decided on its own norms (mxe::steady::SteadyCase); a case that ends is frozen
+  // and its blocks skipped. history: 5 per iteration per case, case k at history + 5 k
+  // max_iterations (or null).
+  void batch_steady(const mxsteady_options& o, mxsteady_result* results, double* history) override {
+    if (ncase_ == 1) throw std::invalid_argument("batch_steady needs a batch");
+    steady::check_options(o);
+    if (o.start_order == 1 && !second_) throw std::invalid_argument("a first-order start needs a second-order batch");
+    std::vector<steady::SteadyCase> runs;
+    runs.reserve(std::size_t(ncase_));
+    const std::size_t nk = static_cast<std::size_t>(ncase_);
+    std::vector<int> flags(nk);
+    for (int k = 0; k < ncase_; ++k) {
+      runs.emplace_back(o);
+      flags[std::size_t(k)] = 1 | ((runs.back().second() && second_) ? 2 : 0);
+    }
+    set_flags(flags.data());
+    const std::size_t n5 = std::size_t(kSums) * std::size_t(ncase_);
+    std::vector<double> r(n5), rep(n5), ts(nk);
+    std::vector<int> inv(nk), which(nk);
+    for (;;) {
+      bool any = false, any_prov = false;
+      for (int k = 0; k < ncase_; ++k) {
+        const auto& run = runs[std::size_t(k)];
+        which[std::size_t(k)] = !run.done() && !run.second();
+        any = any || !run.done();
+        any_prov = any_prov || which[std::size_t(k)];
+      }
+      if (!any) break;
+      if (any_prov) batch_evaluate_second(which, rep.data());
+      batch_step(o.cfl, true, r.data(), ts.data(), inv.data());
+      bool changed = false;
+      for (int k = 0; k < ncase_; ++k) {
+        auto& run = runs[std::size_t(k)];
+        if (run.done()) continue;
+        const bool prov = !run.second();
+        double* h = history != nullptr ? history + std::size_t(k) * kSums * std::size_t(o.max_iterations) : nullptr;
+        run.record(&r[kSums * std::size_t(k)], prov ? &rep[kSums * std::size_t(k)] : nullptr, inv[std::size_t(k)], h,
+                   nullptr);
+        int f = flags[std::size_t(k)];
+        if (run.switched()) f |= 2;
+        if (run.done()) f &= ~1;
+        changed = changed || f != flags[std::size_t(k)];
+        flags[std::size_t(k)] = f;
+      }
+      if (changed) set_flags(flags.data());
+    }
+    for (int k = 0; k < ncase_; ++k) results[k] = runs[std::size_t(k)].result();
+  }
   double gamma() const override { return sc_.gamma; }
 
   int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches, int max_kernels,
@@ -392,6 +544,14 @@ class Stepper final : public StepperBase {
     d.lam = lam_.get();
     d.dt = dt_.get();
     d.small = small_.get();
+    d.ptab = ptab_.get();
+    d.bcase = ncase_ > 1 ? bcase_.get() : nullptr;
+    d.fcase = ncase_ > 1 ? fcase_.get() : nullptr;
+    d.fcbeg = fcbeg_.get();
+    d.cflags = ncase_ > 1 ? (eval_mode_ ? eflags_.get() : cflags_.get()) : nullptr;
+    if (eval_mode_) d.small = small_eval_.get();
+    d.ncase = ncase_;
+    d.nslot = nslot_;
     d.cbeg = cbeg_.get();
     d.sbeg = sbeg_.get();
     d.srec = srec_.get();
@@ -467,22 +627,45 @@ class Stepper final : public StepperBase {
     }
   }
 
-  void params(const mxeuler_scheme& sc) {
-    // physics/src/euler.cpp:238-245, in host code built with -ffp-contract=off.
+  static void validate_cases(const mxeuler_scheme* sc, int ncase, const long long* first, const long long* cells,
+                             long long nc) {
+    if (ncase < 1 || ncase > 4096) throw std::invalid_argument("cases: 1 to 4096");
+    if (first[0] != 0 || first[ncase] != nc) throw std::invalid_argument("cases: ranges must cover the cells");
+    for (int k = 0; k < ncase; ++k) {
+      if (first[k + 1] <= first[k] || (k > 0 && first[k] % kBlock != 0)) {
+        throw std::invalid_argument("cases: each case starts on a 256-cell block");
+      }
+      if (cells[k] < 1 || cells[k] > first[k + 1] - first[k]) throw std::invalid_argument("cases: cell counts");
+      const mxeuler_scheme& a = sc[0];
+      const mxeuler_scheme& b = sc[k];
+      if (a.flux != b.flux || a.second_order != b.second_order || a.limiter != b.limiter || a.planar != b.planar ||
+          a.gamma != b.gamma || a.gas_constant != b.gas_constant || a.venkat_k != b.venkat_k || a.length != b.length) {
+        throw std::invalid_argument("cases: flux, order, limiter, planarity and gas must agree");
+      }
+      if (!(b.mach_ref > 0.0) || !(b.reference[0] > 0.0)) throw std::invalid_argument("cases: reference state");
+    }
+  }
+
+  // One case's parameters (physics/src/euler.cpp:238-245, in host code built with
+  // -ffp-contract=off).
+  static Params<C> params(const mxeuler_scheme& sc, long long cell0) {
+    Params<C> pa{};
     const double* r = sc.reference;
     const double p_ref = (sc.gamma - 1.0) * (r[4] - 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[0]);
     const double k3 = std::pow(sc.venkat_k / sc.length, 3);
-    pa_.g = static_cast<C>(sc.gamma);
-    pa_.rgas = static_cast<C>(sc.gas_constant);
-    pa_.pref = static_cast<C>(p_ref);
-    pa_.minf = static_cast<C>(sc.mach_ref);
+    pa.g = static_cast<C>(sc.gamma);
+    pa.rgas = static_cast<C>(sc.gas_constant);
+    pa.pref = static_cast<C>(p_ref);
+    pa.minf = static_cast<C>(sc.mach_ref);
     for (int k = 0; k < 5; ++k) {
-      pa_.ur[k] = static_cast<C>(r[k]);
-      pa_.scale[k] = static_cast<C>(k3 * sc.scale[k] * sc.scale[k]);
-      pa_.ref[k] = r[k];
+      pa.ur[k] = static_cast<C>(r[k]);
+      pa.scale[k] = static_cast<C>(k3 * sc.scale[k] * sc.scale[k]);
+      pa.ref[k] = r[k];
     }
-    pa_.p_ref = p_ref;
-    pa_.gamma = sc.gamma;
+    pa.p_ref = p_ref;
+    pa.gamma = sc.gamma;
+    pa.cell0 = cell0;
+    return pa;
   }
 
   void build_topology(const mxeuler_topology& t) {
@@ -520,11 +703,16 @@ class Stepper final : public StepperBase {
     int max_slots = 0;
     bool fits = true;
     int c = 0;
+    std::vector<int> fcase, fcbeg(std::size_t(ncase_) + 1, 0);
+    int cs = 0;
     while (c < nc_) {
       const int b = static_cast<int>(cbeg.size()) - 1;
       faces.clear();
       const int c0 = c;
-      while (c < nc_ && c - c0 < kBlock) {
+      while (c0 >= case_first_[std::size_t(cs) + 1]) fcbeg[std::size_t(++cs)] = b;
+      const int cend = static_cast<int>(case_first_[std::size_t(cs) + 1]);
+      fcase.push_back(cs);
+      while (c < cend && c - c0 < kBlock) {
         int fresh = 0;
         for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) fresh += mark[std::size_t(t.cell_faces[k])] != b;
         if (static_cast<int>(faces.size()) + fresh > kMaxSlots && c > c0) break;
@@ -554,11 +742,16 @@ class Stepper final : public StepperBase {
       cbeg.push_back(c);
     }
     face_blocks_ = static_cast<int>(cbeg.size()) - 1;
+    while (cs < ncase_) fcbeg[std::size_t(++cs)] = face_blocks_;
     // Pad to the XCD-aware grid: the extra blocks hold no cell and no face.
     while (static_cast<int>(cbeg.size()) < xcd_grid(face_blocks_) + 1) {
       cbeg.push_back(nc_);
       sbeg.push_back(sbeg.back());
+      fcase.push_back(ncase_ - 1);
     }
+    if (ncase_ == 1) fcbeg[1] = xcd_grid(face_blocks_);
+    fcbeg_.upload(fcbeg, stream_);
+    if (ncase_ > 1) fcase_.upload(fcase, stream_);
     max_slots_ = max_slots;
     fused_ = fits && faces_lds(true) <= 65536;
     if (!fused_) return;
@@ -835,41 +1028,46 @@ class Stepper final : public StepperBase {
   void launch_spectral(double cfl, bool local) {
     const int g = grid(nc_);
     launch("spectral", [&] {
-      k_spectral<C><<<g, kBlock, 0, stream_>>>(dev(), pa_, cfl, local ? 1 : 0, partial_.get());
+      k_spectral<C><<<g, kBlock, 0, stream_>>>(dev(), cfl, local ? 1 : 0, partial_.get());
     });
     launch("min_dt", [&] { k_min_key<<<1, 1024, 0, stream_>>>(partial_.get(), g, small_.get()); });
   }
 
   void clear() {
-    launch("clear", [&] { k_clear<<<1 + kNormSlots, kBlock, 0, stream_>>>(small_.get(), nslots_.get()); });
+    launch("clear", [&] {
+      k_clear<<<ncase_ * (1 + nslot_), kBlock, 0, stream_>>>(eval_mode_ ? small_eval_.get() : small_.get(), ncase_,
+                                                            nslots_.get());
+    });
   }
 
   void launch_prim(const double* u) {
-    launch("prim", [&] { k_prim<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), pa_, u); });
+    launch("prim", [&] { k_prim<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), u); });
   }
 
   // Impl::cell_loops and Impl::faces from the primitives already stored. `first` (the step's
   // R(u)) also leaves the spectral radius, the time steps and the exact norms' words.
   void residual_from_prim(bool first = false, double cfl = 0.0, bool local = false) {
     const Dev<C> d = dev();
-    if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d, pa_); });
+    if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d); });
     if (second_) {
       const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
       if (staged_ && sc_.limiter) {
-        launch("grad_limiter", [&] { k_grad_lds<C, true><<<grid(nc_), kBlock, lds, stream_>>>(d, pa_); });
+        launch("grad_limiter", [&] { k_grad_lds<C, true><<<grid(nc_), kBlock, lds, stream_>>>(d); });
       } else if (staged_) {
-        launch("grad", [&] { k_grad_lds<C, false><<<grid(nc_), kBlock, lds, stream_>>>(d, pa_); });
+        launch("grad", [&] { k_grad_lds<C, false><<<grid(nc_), kBlock, lds, stream_>>>(d); });
       } else if (sc_.limiter) {
-        launch("grad_limiter", [&] { k_grad<C, true><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
+        launch("grad_limiter", [&] { k_grad<C, true><<<grid(nc_), kBlock, 0, stream_>>>(d); });
       } else {
-        launch("grad", [&] { k_grad<C, false><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_); });
+        launch("grad", [&] { k_grad<C, false><<<grid(nc_), kBlock, 0, stream_>>>(d); });
       }
     }
     if (fused_) {
       launch(first ? "faces_first" : "faces", [&] { launch_faces(d, first, cfl, local); });
       if (first) {
         launch("step_reduce", [&] {
-          k_step_reduce<<<1, 1024, 0, stream_>>>(nslots_.get(), partial_.get(), xcd_grid(face_blocks_), small_.get());
+          const Dev<C> dd = dev();
+          k_step_reduce<<<ncase_, 1024, 0, stream_>>>(nslots_.get(), nslot_, partial_.get(), fcbeg_.get(), dd.cflags,
+                                                      dd.small);
         });
       }
       return;
@@ -889,19 +1087,19 @@ class Stepper final : public StepperBase {
       if (stage_faces_) {
         if (first) {
           k_faces<C, FLUX, PLANAR, SECOND, true, true><<<g, kBlock, faces_lds(true, true), stream_>>>(
-              d, pa_, cfl, local ? 1 : 0, partial_.get());
+              d, cfl, local ? 1 : 0, partial_.get());
         } else {
-          k_faces<C, FLUX, PLANAR, SECOND, false, true><<<g, kBlock, faces_lds(false, true), stream_>>>(d, pa_, cfl, 0,
+          k_faces<C, FLUX, PLANAR, SECOND, false, true><<<g, kBlock, faces_lds(false, true), stream_>>>(d, cfl, 0,
                                                                                                        nullptr);
         }
         return;
       }
     }
     if (first) {
-      k_faces<C, FLUX, PLANAR, SECOND, true, false><<<g, kBlock, faces_lds(true), stream_>>>(d, pa_, cfl, local ? 1 : 0,
+      k_faces<C, FLUX, PLANAR, SECOND, true, false><<<g, kBlock, faces_lds(true), stream_>>>(d, cfl, local ? 1 : 0,
                                                                                           partial_.get());
     } else {
-      k_faces<C, FLUX, PLANAR, SECOND, false, false><<<g, kBlock, faces_lds(false), stream_>>>(d, pa_, cfl, 0, nullptr);
+      k_faces<C, FLUX, PLANAR, SECOND, false, false><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0, nullptr);
     }
   }
 
@@ -941,15 +1139,15 @@ class Stepper final : public StepperBase {
     const int g = grid(nf_);
     if (planar_) {
       if (second_) {
-        k_flux<C, FLUX, true, true><<<g, kBlock, 0, stream_>>>(d, pa_);
+        k_flux<C, FLUX, true, true><<<g, kBlock, 0, stream_>>>(d);
       } else {
-        k_flux<C, FLUX, true, false><<<g, kBlock, 0, stream_>>>(d, pa_);
+        k_flux<C, FLUX, true, false><<<g, kBlock, 0, stream_>>>(d);
       }
     } else {
       if (second_) {
-        k_flux<C, FLUX, false, true><<<g, kBlock, 0, stream_>>>(d, pa_);
+        k_flux<C, FLUX, false, true><<<g, kBlock, 0, stream_>>>(d);
       } else {
-        k_flux<C, FLUX, false, false><<<g, kBlock, 0, stream_>>>(d, pa_);
+        k_flux<C, FLUX, false, false><<<g, kBlock, 0, stream_>>>(d);
       }
     }
   }
@@ -974,7 +1172,7 @@ class Stepper final : public StepperBase {
     launch_prim(u_.get());
     const bool use_local = !has_dt && local;
     residual_from_prim(true, has_dt ? 1.0 : cfl, use_local);
-    const unsigned long long* hkey = has_dt ? nullptr : &small_.get()->hkey;
+    const int use_hkey = has_dt ? 0 : 1;
     // Shu and Osher (1988), eq. 2.18.
     static constexpr double w[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
     for (int s = 0; s < 3; ++s) {
@@ -983,13 +1181,13 @@ class Stepper final : public StepperBase {
       const Dev<C> d = dev();
       if (s < 2) {
         launch("rk_prim", [&] {
-          k_rk<C, true, false><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
-                                                                     stage_.get(), hkey, dt, use_local ? 1 : 0);
+          k_rk<C, true, false><<<grid(nc_), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
+                                                                     stage_.get(), use_hkey, dt, use_local ? 1 : 0);
         });
       } else {
         launch("rk_check", [&] {
-          k_rk<C, false, true><<<grid(nc_), kBlock, 0, stream_>>>(d, pa_, w[s][0], w[s][1], u_.get(), in,
-                                                                     stage_.get(), hkey, dt, use_local ? 1 : 0);
+          k_rk<C, false, true><<<grid(nc_), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
+                                                                     stage_.get(), use_hkey, dt, use_local ? 1 : 0);
         });
       }
     }
@@ -998,11 +1196,17 @@ class Stepper final : public StepperBase {
 
   // The step's single copy back: norms, time step and validity.
   void finish(double* residuals, int* invalid) {
-    check(hipMemcpyAsync(host_small_, small_.get(), sizeof(Small), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipMemcpyAsync(host_small_, small_.get(), sizeof(Small) * std::size_t(ncase_), hipMemcpyDeviceToHost, stream_),
+          "hipMemcpyAsync");
     check(hipStreamSynchronize(stream_), "step");
-    for (int k = 0; k < kSums; ++k) residuals[k] = std::sqrt(exact_value(host_small_->words[k]) / double(nc_));
-    const int a = host_small_->invalid_state, b = host_small_->invalid_residual;
-    *invalid = a != INT32_MAX ? a : (b != INT32_MAX ? b : -1);
+    for (int cs = 0; cs < ncase_; ++cs) {
+      const Small& sm = host_small_[cs];
+      for (int k = 0; k < kSums; ++k) {
+        residuals[kSums * cs + k] = std::sqrt(exact_value(sm.words[k]) / double(case_cells_[std::size_t(cs)]));
+      }
+      const int a = sm.invalid_state, b = sm.invalid_residual;
+      invalid[cs] = a != INT32_MAX ? a : (b != INT32_MAX ? b : -1);
+    }
   }
 
   mxeuler_scheme sc_;
@@ -1012,7 +1216,17 @@ class Stepper final : public StepperBase {
   bool own_stream_ = false;
   int nc_ = 0, nf_ = 0, ni_ = 0, nb_ = 0;
   bool second_ = true, planar_ = true;
-  Params<C> pa_{};
+  int ncase_ = 1;
+  int nslot_ = kNormSlots;
+  std::vector<long long> case_first_;  // ncase + 1: each case's first cell (a multiple of kBlock)
+  std::vector<long long> case_cells_;  // each case's cells (the rest of its range is filler)
+  std::vector<Params<C>> ptab_host_;
+  std::vector<int> cflags_host_;
+  Buffer<int> eflags_;       // the flags of a second-order evaluation of some cases
+  Buffer<Small> small_eval_;  // its norms
+  bool eval_mode_ = false;
+  Buffer<Params<C>> ptab_;
+  Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
   Buffer<int2> lr_;
   Buffer<int> bowner_, cfo_, inc_, slen_, sidx_, bkind_;
   Buffer<long long> soff_;
@@ -1113,6 +1327,7 @@ int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err
 int mxeuler_step(mxeuler_stepper* s, double cfl, int32_t has_dt, double dt, int32_t local, double residuals[5],
                  double* time_step, int32_t* invalid_cell, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
+    if (s->impl->ncase() != 1) throw std::invalid_argument("a batch steps with mxeuler_batch_step");
     if (has_dt && !(dt >= 0.0)) throw std::invalid_argument("time step must be non-negative");
     int inv = -1;
     s->impl->step(cfl, has_dt != 0, dt, local != 0, residuals, time_step, &inv);
@@ -1122,6 +1337,7 @@ int mxeuler_step(mxeuler_stepper* s, double cfl, int32_t has_dt, double dt, int3
 
 int mxeuler_evaluate(mxeuler_stepper* s, double residuals[5], int32_t* invalid_cell, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
+    if (s->impl->ncase() != 1) throw std::invalid_argument("evaluate takes one case");
     int inv = -1;
     s->impl->evaluate(residuals, &inv);
     *invalid_cell = inv;
@@ -1308,6 +1524,62 @@ int mxeuler_residual_scales(mxeuler_stepper* s, double density, double speed, do
   });
 }
 
+int mxeuler_create_batch(int32_t policy, int32_t n_cases, const mxeuler_scheme* schemes, const int64_t* case_first,
+                         const int64_t* case_cells, const mxeuler_topology* topology, const mxeuler_geometry* geometry,
+                         const mxeuler_stencil* stencil, const double* bc_kind, const double* bc, const double* source,
+                         void* stream, mxeuler_stepper** out, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!schemes || !case_first || !case_cells || !topology || !geometry || !stencil || !out || n_cases < 1) {
+      throw std::invalid_argument("null argument");
+    }
+    auto h = std::make_unique<mxeuler_stepper>();
+    const auto s = static_cast<hipStream_t>(stream);
+    const auto* first = reinterpret_cast<const long long*>(case_first);
+    const auto* cells = reinterpret_cast<const long long*>(case_cells);
+    if (policy == MXEULER_FP64) {
+      h->impl = std::make_unique<mxe::Stepper<double>>(schemes, n_cases, first, cells, *topology, *geometry, *stencil,
+                                                       bc_kind, bc, source, s);
+    } else if (policy == MXEULER_FP32_ACC64) {
+      h->impl = std::make_unique<mxe::Stepper<float>>(schemes, n_cases, first, cells, *topology, *geometry, *stencil,
+                                                      bc_kind, bc, source, s);
+    } else {
+      throw std::invalid_argument("unknown precision policy");
+    }
+    h->nc = topology->n_cells;
+    h->nb = topology->n_faces - topology->n_interior;
+    *out = h.release();
+  });
+}
+
+int mxeuler_cases(const mxeuler_stepper* s) { return s != nullptr ? s->impl->ncase() : 0; }
+
+int mxeuler_batch_step(mxeuler_stepper* s, double cfl, int32_t local, double* residuals, double* time_steps,
+                       int32_t* invalid_cells, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!residuals || !time_steps || !invalid_cells) throw std::invalid_argument("null argument");
+    const int n = s->impl->ncase();
+    std::vector<int> inv(static_cast<std::size_t>(n));
+    s->impl->batch_step(cfl, local != 0, residuals, time_steps, inv.data());
+    for (int k = 0; k < n; ++k) invalid_cells[k] = inv[std::size_t(k)];
+  });
+}
+
+int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!flags) throw std::invalid_argument("null argument");
+    std::vector<int> f(flags, flags + s->impl->ncase());
+    s->impl->set_flags(f.data());
+  });
+}
+
+int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* results,
+                         double* history, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!options || !results) throw std::invalid_argument("null argument");
+    s->impl->batch_steady(*options, results, history);
+  });
+}
+
 int mxsteady_run(const mxsteady_ops* ops, const mxsteady_options* options, mxsteady_result* result,
                  double* history, int32_t* provisional, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
@@ -1356,6 +1628,7 @@ int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady
                    int32_t* provisional, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!s || !options || !result) throw std::invalid_argument("null argument");
+    if (s->impl->ncase() != 1) throw std::invalid_argument("a batch runs with mxeuler_batch_steady");
     const mxsteady_ops ops{s, steady_step, steady_evaluate, steady_order};
     mxe::steady::run(ops, *options, *result, history, provisional);
   });

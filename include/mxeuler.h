@@ -218,6 +218,31 @@ int mxsteady_run(const mxsteady_ops* ops, const mxsteady_options* options, mxste
 int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result, double* history,
                    int32_t* provisional, char* err, size_t err_len);
 
+/* ---- Batches: several meshes (cases) stepped together in one index space ------------------------
+ * No counterpart in the reference (a parameter sweep on one GPU). The topology, geometry, stencil
+ * and boundary arrays are the union of the cases': case k owns cells [case_first[k], case_first[k]
+ * + case_cells[k]); the rest of its range up to case_first[k + 1] are filler cells without faces
+ * or stencil entries; every case_first a multiple of 256, case_first[n_cases] = n_cells. Faces
+ * never join two cases. schemes[k] is case k's scheme: flux, order, limiter, planarity and gas
+ * agree across cases. The state arrays (upload, download, residual) cover the whole index space. */
+int mxeuler_create_batch(int32_t policy, int32_t n_cases, const mxeuler_scheme* schemes, const int64_t* case_first,
+                         const int64_t* case_cells, const mxeuler_topology* topology, const mxeuler_geometry* geometry,
+                         const mxeuler_stencil* stencil, const double* bc_kind, const double* bc, const double* source,
+                         void* stream, mxeuler_stepper** out, char* err, size_t err_len);
+/* The number of cases of a stepper (1 for mxeuler_create's). */
+int mxeuler_cases(const mxeuler_stepper* s);
+/* One SSP-RK3 step of every active case: residuals 5 x n_cases (case-major), the time step and the
+ * first invalid cell (relative to the case's first cell, -1: none) per case. */
+int mxeuler_batch_step(mxeuler_stepper* s, double cfl, int32_t local, double* residuals, double* time_steps,
+                       int32_t* invalid_cells, char* err, size_t err_len);
+/* Per case: bit 0 active (an inactive case is not stepped and keeps its state), bit 1 second order
+ * (a case turned to first order has its gradients cleared). */
+int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, size_t err_len);
+/* Every case's steady run at once (mxeuler_steady's rules per case): results per case; history (5
+ * per iteration, case k from history + 5 k max_iterations) may be NULL. */
+int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* results,
+                         double* history, char* err, size_t err_len);
+
 const char* mxeuler_version(void);
 
 #ifdef __cplusplus
