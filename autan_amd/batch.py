@@ -228,6 +228,9 @@ class BatchStepper:
                    _p(inv.ctypes.data))
         return r.reshape(self.n, 5), h, [None if v < 0 else int(v) for v in inv]
 
+    def tune(self, key: str, value: int) -> None:
+        self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))
+
     def set_flags(self, flags) -> None:
         f = np.ascontiguousarray(flags, dtype=np.int32)
         self._call(self.lib.mxeuler_batch_flags, _p(f.ctypes.data))

@@ -191,6 +191,7 @@ class Stepper final : public StepperBase {
 
   ~Stepper() override {
     (void)hipStreamSynchronize(stream_);
+    drop_graphs();
     if (host_small_ != nullptr) (void)hipHostFree(host_small_);
     if (own_stream_) (void)hipStreamDestroy(stream_);
   }
@@ -215,7 +216,11 @@ class Stepper final : public StepperBase {
 
   void step(double cfl, bool has_dt, double dt, bool local, double* residuals, double* time_step,
             int* invalid) override {
-    enqueue_step(cfl, has_dt, dt, local);
+    if (use_graph_ && !has_dt) {
+      graph_step(cfl, local);
+    } else {
+      enqueue_step(cfl, has_dt, dt, local);
+    }
     finish(residuals, invalid);
     *time_step = has_dt ? dt : key_value(host_small_[0].hkey);
   }
@@ -229,12 +234,14 @@ class Stepper final : public StepperBase {
   }
 
   void set_second_order(bool on) override {
+    drop_graphs();
     second_ = on;
     check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
     check(hipStreamSynchronize(stream_), "set_second_order");
   }
 
   void set_boundary_data(const double* bc) override {
+    drop_graphs();
     std::vector<C> b(8 * std::size_t(std::max(nb_, 1)), C(0));
     for (int i = 0; i < nb_; ++i) {
       for (int k = 0; k < 5; ++k) b[8 * std::size_t(i) + k] = static_cast<C>(bc[std::size_t(k) * nb_ + i]);
@@ -244,6 +251,7 @@ class Stepper final : public StepperBase {
   }
 
   void set_source(const double* source) override {
+    drop_graphs();
     if (source == nullptr) {
       source_ = Buffer<C>();
       return;
@@ -333,7 +341,13 @@ class Stepper final : public StepperBase {
       if (value != 0 && !(std::is_same_v<C, float> && fused_ && faces_lds(true, true) <= 65536)) {
         throw std::invalid_argument("face_stage: needs fp32+acc64 and the fused face kernel");
       }
+      drop_graphs();
       stage_faces_ = value != 0;
+      return;
+    }
+    if (key == "graph") {
+      drop_graphs();
+      use_graph_ = value != 0;
       return;
     }
     throw std::invalid_argument("unknown tuning key '" + key + "'");
@@ -344,7 +358,11 @@ class Stepper final : public StepperBase {
 
   // One SSP-RK3 step of every active case: norms, time step and invalid cell per case.
   void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) override {
-    enqueue_step(cfl, false, 0.0, local);
+    if (use_graph_) {
+      graph_step(cfl, local);
+    } else {
+      enqueue_step(cfl, false, 0.0, local);
+    }
     finish(residuals, invalid);
     for (int k = 0; k < ncase_; ++k) time_steps[k] = key_value(host_small_[k].hkey);
   }
@@ -1194,6 +1212,46 @@ class Stepper final : public StepperBase {
     std::swap(u_, stage_);
   }
 
+  // A step's launches recorded once as a HIP graph and replayed: two graphs, one for each way the
+  // state and stage buffers alternate. Recorded again when the CFL number, the time stepping or a
+  // kernel variant changes, or a buffer the kernels read is replaced.
+  void graph_step(double cfl, bool local) {
+    if (!graphs_valid_ || graph_cfl_ != cfl || graph_local_ != local) {
+      drop_graphs();
+      for (int p = 0; p < 2; ++p) {
+        graph_u_[p] = u_.get();
+        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+          enqueue_step(cfl, false, 0.0, local);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream_, &g);
+          if (g != nullptr) (void)hipGraphDestroy(g);
+          throw;
+        }
+        check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&graph_exec_[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e, "hipGraphInstantiate");
+      }
+      graphs_valid_ = true;
+      graph_cfl_ = cfl;
+      graph_local_ = local;
+    }
+    const int p = graph_u_[0] == u_.get() ? 0 : 1;
+    if (graph_u_[p] != u_.get()) throw std::logic_error("graph_step: state buffer not recorded");
+    check(hipGraphLaunch(graph_exec_[p], stream_), "hipGraphLaunch");
+    std::swap(u_, stage_);
+  }
+
+  void drop_graphs() {
+    for (auto& e : graph_exec_) {
+      if (e != nullptr) (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+    graphs_valid_ = false;
+  }
+
   // The step's single copy back: norms, time step and validity.
   void finish(double* residuals, int* invalid) {
     check(hipMemcpyAsync(host_small_, small_.get(), sizeof(Small) * std::size_t(ncase_), hipMemcpyDeviceToHost, stream_),
@@ -1225,6 +1283,10 @@ class Stepper final : public StepperBase {
   Buffer<int> eflags_;       // the flags of a second-order evaluation of some cases
   Buffer<Small> small_eval_;  // its norms
   bool eval_mode_ = false;
+  bool use_graph_ = false, graphs_valid_ = false, graph_local_ = false;
+  double graph_cfl_ = 0.0;
+  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+  const double* graph_u_[2] = {nullptr, nullptr};
   Buffer<Params<C>> ptab_;
   Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
   Buffer<int2> lr_;

@@ -158,7 +158,8 @@ int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const dou
                   double moment[3], char* err, size_t err_len);
 /* Chooses between kernel variants that give the same bits (no counterpart in the reference: the
  * autotuner's handle). Keys: "face_stage" (0/1, fp32+acc64): the fused face kernel stages its
- * block's cells in LDS. An unknown key or a variant this stepper cannot run is an error. */
+ * block's cells in LDS; "graph" (0/1): steps without a fixed time step replay a recorded HIP graph
+ * of their launches. An unknown key or a variant this stepper cannot run is an error. */
 int mxeuler_tune(mxeuler_stepper* s, const char* key, int32_t value, char* err, size_t err_len);
 /* EulerPhysics::check_conservation (physics/src/euler_physics.cpp:1750-1784; docs/design/euler.md,
  * Conservation check) on the current state: per equation k, exact sums over the cells and faces,

@@ -70,11 +70,13 @@ def test_inactive_cases_keep_their_state():
     assert not np.array_equal(after[0], cs[0].u)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("start", ["second", "first"])
-def test_batch_steady_matches_each_solo_run(start):
+def test_batch_steady_matches_each_solo_run(start, graph):
     cs = make_cases()
     policy = "fp32+acc64"
     b = batch.BatchStepper(cs, policy)
+    b.tune("graph", graph)
     b.upload()
     kw = dict(max_iterations=230, residual_orders=8.0, start_order=start)
     got = b.steady(**kw)

@@ -137,6 +137,28 @@ def test_staged_face_kernel_gives_the_same_bits(name, local):
         build(case, "fp64")[0].tune("face_stage", 1)
 
 
+@pytest.mark.parametrize("local", [False, True])
+@pytest.mark.parametrize("name", ["tets-hllc2", "bump-hybrid-ausm2"])
+def test_graph_replays_give_the_eager_bits(name, local):
+    """Steps replayed from a recorded HIP graph, with eager work (downloads, an evaluation, a
+    residual) between the replays, give the eager steps' bits at every step."""
+    case = CASES[name]()
+    eager, _ = build(case, "fp32+acc64")
+    graphed, _ = build(case, "fp32+acc64")
+    graphed.tune("graph", 1)
+    for d in (eager, graphed):
+        d.upload(case.u)
+    for k in range(5):
+        a = eager.step(cfl=0.5, local=local)
+        b = graphed.step(cfl=0.5, local=local)
+        np.testing.assert_array_equal(a.residuals, b.residuals)
+        assert a.time_step == b.time_step
+        np.testing.assert_array_equal(eager.download(), graphed.download())
+        if k == 2:
+            np.testing.assert_array_equal(eager.evaluate().residuals, graphed.evaluate().residuals)
+            np.testing.assert_array_equal(eager.residual(case.u), graphed.residual(case.u))
+
+
 @pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
 def test_fixed_dt_steps_and_evaluate(policy):
     case = CASES["tets-hllc2"]()
