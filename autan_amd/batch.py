@@ -235,16 +235,23 @@ class BatchStepper:
         f = np.ascontiguousarray(flags, dtype=np.int32)
         self._call(self.lib.mxeuler_batch_flags, _p(f.ctypes.data))
 
+    def autotune(self, steps: int = 5, cfl: float = _steady.STEADY_CFL, local: bool = True) -> str:
+        """Keeps the fastest kernel variant (same bits) for these cases; the state is restored."""
+        report = ctypes.create_string_buffer(2048)
+        self._call(self.lib.mxeuler_autotune, _i32(steps), _d(cfl), _i32(local), report, ctypes.c_size_t(len(report)))
+        return report.value.decode()
+
     def steady(self, max_iterations: int = 10000, residual_orders: float = 8.0, cfl: float = _steady.STEADY_CFL,
                level_rule: bool | None = None, start_order: str = "first", scales=None, history: bool = True):
-        """Every case's steady run (autan_amd.steady.solve's rules per case)."""
+        """Every case's steady run (autan_amd.steady.solve's rules per case); scales: 5 per case."""
         if level_rule is None:
             level_rule = self.policy == 1
         keep: list = []
-        opts = _steady._options(max_iterations, residual_orders, cfl, level_rule, start_order, scales, None, keep)
+        opts = _steady._options(max_iterations, residual_orders, cfl, level_rule, start_order, None, None, keep)
+        cs = None if scales is None else np.ascontiguousarray(scales, dtype=np.float64).reshape(self.n, 5)
         res = (_steady._Result * self.n)()
         hist = np.zeros(5 * max_iterations * self.n) if history else None
-        self._call(self.lib.mxeuler_batch_steady, ctypes.byref(opts), res,
+        self._call(self.lib.mxeuler_batch_steady, ctypes.byref(opts), _p(None if cs is None else cs.ctypes.data), res,
                    _p(None if hist is None else hist.ctypes.data))
         out = []
         for j in range(self.n):

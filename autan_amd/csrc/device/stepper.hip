@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -110,7 +111,9 @@ class StepperBase {
   virtual int ncase() const = 0;
   virtual void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) = 0;
   virtual void set_flags(const int* flags) = 0;
-  virtual void batch_steady(const mxsteady_options& o, mxsteady_result* results, double* history) = 0;
+  virtual void batch_steady(const mxsteady_options& o, const double* case_scales, mxsteady_result* results,
+                            double* history) = 0;
+  virtual std::string autotune(int steps, double cfl, bool local) = 0;
   virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
                       int max_kernels, std::string* names) = 0;
@@ -356,6 +359,58 @@ class Stepper final : public StepperBase {
   double area_scale() const override { return area_scale_; }
   int ncase() const override { return ncase_; }
 
+  // Times `steps` steps of every variant (face kernel staging where it runs, graph replay) from
+  // the same state, keeps the fastest and restores the state. Every variant gives the same bits.
+  std::string autotune(int steps, double cfl, bool local) override {
+    if (steps < 1) throw std::invalid_argument("autotune: steps >= 1");
+    const std::size_t n = 5 * std::size_t(nc_);
+    Buffer<double> saved(n);
+    check(hipMemcpyAsync(saved.get(), u_.get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
+    const bool can_stage = std::is_same_v<C, float> && fused_ && faces_lds(true, true) <= 65536;
+    const std::size_t nk = static_cast<std::size_t>(ncase_);
+    std::vector<double> r(kSums * nk), ts(nk);
+    std::vector<int> inv(nk);
+    auto one = [&] {
+      if (ncase_ > 1) {
+        batch_step(cfl, local, r.data(), ts.data(), inv.data());
+      } else {
+        double h = 0.0;
+        step(cfl, false, 0.0, local, r.data(), &h, inv.data());
+      }
+    };
+    auto restore = [&] {
+      check(hipMemcpyAsync(u_.get(), saved.get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+            "hipMemcpyAsync");
+      check(hipStreamSynchronize(stream_), "autotune");
+    };
+    std::string report;
+    double best = std::numeric_limits<double>::infinity();
+    int best_stage = 0, best_graph = 0;
+    for (int st = 0; st <= (can_stage ? 1 : 0); ++st) {
+      for (int gr = 0; gr <= 1; ++gr) {
+        tune("face_stage", st);
+        tune("graph", gr);
+        restore();
+        one();  // warm-up (records the graph)
+        restore();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < steps; ++k) one();
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        report += "face_stage=" + std::to_string(st) + ",graph=" + std::to_string(gr) + ":" + std::to_string(ms) + "ms;";
+        if (ms < best) {
+          best = ms;
+          best_stage = st;
+          best_graph = gr;
+        }
+      }
+    }
+    tune("face_stage", best_stage);
+    tune("graph", best_graph);
+    restore();
+    return "best face_stage=" + std::to_string(best_stage) + ",graph=" + std::to_string(best_graph) + ";" + report;
+  }
+
   // One SSP-RK3 step of every active case: norms, time step and invalid cell per case.
   void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) override {
     if (use_graph_) {
@@ -430,7 +485,8 @@ class Stepper final : public StepperBase {
   // convergence decided on its own norms (mxe::steady::SteadyCase); a case that ends is frozen
   // and its blocks skipped. history: 5 per iteration per case, case k at history + 5 k
   // max_iterations (or null).
-  void batch_steady(const mxsteady_options& o, mxsteady_result* results, double* history) override {
+  void batch_steady(const mxsteady_options& o, const double* case_scales, mxsteady_result* results,
+                    double* history) override {
     if (ncase_ == 1) throw std::invalid_argument("batch_steady needs a batch");
     steady::check_options(o);
     if (o.start_order == 1 && !second_) throw std::invalid_argument("a first-order start needs a second-order batch");
@@ -439,7 +495,9 @@ class Stepper final : public StepperBase {
     const std::size_t nk = static_cast<std::size_t>(ncase_);
     std::vector<int> flags(nk);
     for (int k = 0; k < ncase_; ++k) {
-      runs.emplace_back(o);
+      mxsteady_options ok = o;
+      ok.scales = case_scales != nullptr ? case_scales + kSums * std::size_t(k) : nullptr;
+      runs.emplace_back(ok);
       flags[std::size_t(k)] = 1 | ((runs.back().second() && second_) ? 2 : 0);
     }
     set_flags(flags.data());
@@ -1634,11 +1692,22 @@ int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, siz
   });
 }
 
-int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* results,
-                         double* history, char* err, size_t err_len) {
+int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, const double* case_scales,
+                         mxsteady_result* results, double* history, char* err, size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!options || !results) throw std::invalid_argument("null argument");
-    s->impl->batch_steady(*options, results, history);
+    s->impl->batch_steady(*options, case_scales, results, history);
+  });
+}
+
+int mxeuler_autotune(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local, char* report, size_t report_len,
+                     char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    const std::string r = s->impl->autotune(steps, cfl, local != 0);
+    if (report != nullptr && report_len > 0) {
+      std::strncpy(report, r.c_str(), report_len - 1);
+      report[report_len - 1] = '\0';
+    }
   });
 }
 

@@ -327,6 +327,12 @@ class DeviceStepper:
         )
         return np.array(force[:]), np.array(moment[:])
 
+    def autotune(self, steps: int = 5, cfl: float = 0.5, local: bool = False) -> str:
+        """Keeps the fastest kernel variant (same bits); the state is restored."""
+        report = ctypes.create_string_buffer(2048)
+        self._call(self.lib.mxeuler_autotune, _i32(steps), _d(cfl), _i32(local), report, ctypes.c_size_t(len(report)))
+        return report.value.decode()
+
     def tune(self, key: str, value: int) -> None:
         """Selects a kernel variant (same bits, different speed): "face_stage" 0/1."""
         self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))

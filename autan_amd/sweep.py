@@ -1,0 +1,124 @@
+"""The bump-channel parameter sweep: cases of uneven size, bump height, Mach number and flow angle
+(fixed seed), stepped together to their steady state by one batch stepper.
+
+Case k: cell count log-uniform in [cells_lo, cells_hi], bump height uniform in [h_lo, h_hi], Mach
+uniform in [mach_lo, mach_hi], flow angle uniform in [alpha_lo, alpha_hi] degrees, drawn in that
+order per case from numpy's default generator with the given seed. Each case is the hybrid bump
+channel (prism rows on the wall, a band of merged polyhedra) on a lattice of cubic cells
+3 n x n x n with n chosen for the drawn count; second-order HLLC (or the given flux) in
+fp32+acc64, local time steps at CFL 0.8, to a 1e-8 relative drop or the round-off plateau rule.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from autan_amd import batch, workloads
+
+
+@dataclass
+class Spec:
+    cells: int  # the drawn count
+    h: float
+    mach: float
+    alpha: float
+    lattice: tuple  # nx, ny, nz
+
+
+def lattice_for(cells: float) -> tuple[int, int, int]:
+    """A 3n x n x n lattice of about `cells` hexahedra (n >= 8, nx and nz even)."""
+    n = max(8, int(round((cells / 3.0) ** (1.0 / 3.0))))
+    nx = 3 * n + (3 * n) % 2
+    nz = n + n % 2
+    return nx, n, nz
+
+
+def specs(n_cases: int = 64, seed: int = 1, cells=(2e4, 5e5), h=(0.02, 0.12), mach=(0.3, 0.8),
+          alpha=(-4.0, 4.0)) -> list[Spec]:
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_cases):
+        n = math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))
+        hh = rng.uniform(*h)
+        m = rng.uniform(*mach)
+        a = rng.uniform(*alpha)
+        out.append(Spec(int(round(n)), float(hh), float(m), float(a), lattice_for(n)))
+    return out
+
+
+def build(spec: Spec, policy: str = "fp32+acc64", flux: str = "hllc") -> batch.Case:
+    nx, ny, nz = spec.lattice
+    w = workloads.bump(nx, ny, nz, policy, flux, h=spec.h, mach=spec.mach, alpha_deg=spec.alpha, prism_layers=2)
+    w.scheme.planar = False
+    return batch.Case(w.mesh, w.scheme, w.u, w.bc_kind, w.bc)
+
+
+def residual_scales(case: batch.Case) -> np.ndarray:
+    """Phi_k of EulerPhysics::residual_scales for the case's freestream (rho q A, rho q^2 A, rho H q A
+    with A = sqrt(sum A_i^2 / N))."""
+    m = case.mesh
+    area = np.zeros(m.n_cells)
+    np.add.at(area, m.owner, m.face_area)
+    np.add.at(area, m.neighbour, m.face_area[: m.n_interior])
+    a = math.sqrt(float(np.sum(area * area)) / m.n_cells)
+    ref = np.array(case.scheme.reference)
+    g = case.scheme.gamma
+    speed = float(np.linalg.norm(ref[1:4])) / ref[0]
+    p = (g - 1.0) * (ref[4] - 0.5 * ref[0] * speed * speed)
+    c = math.sqrt(g * p / ref[0])
+    q = speed + c
+    hh = c * c / (g - 1.0) + 0.5 * speed * speed
+    mass = ref[0] * q * a
+    return np.array([mass, mass * q, mass * q, mass * q, mass * hh])
+
+
+def run(n_cases: int = 64, seed: int = 1, max_iterations: int = 20000, residual_orders: float = 8.0,
+        policy: str = "fp32+acc64", flux: str = "hllc", start_order: str = "first", autotune: bool = True,
+        cells=(2e4, 5e5), log=print) -> dict:
+    """The sweep: build, tune, solve; aggregate cells x iterations per second and per-case counts."""
+    t0 = time.time()
+    sp = specs(n_cases, seed, cells)
+    cases = [build(s, policy, flux) for s in sp]
+    log(f"sweep: {n_cases} cases, {sum(c.mesh.n_cells for c in cases)} cells, built in {time.time() - t0:.1f} s")
+    b = batch.BatchStepper(cases, policy)
+    tuned = b.autotune(steps=5) if autotune else None
+    if tuned is not None:
+        log(f"sweep: autotuned {tuned}")
+    b.tune("graph", 1)
+    b.upload()
+    # One scale set for the round-off floors: each case's own.
+    scales = np.array([residual_scales(c) for c in cases])
+    t1 = time.perf_counter()
+    runs = b.steady(max_iterations=max_iterations, residual_orders=residual_orders, start_order=start_order,
+                    scales=scales, history=False)
+    wall = time.perf_counter() - t1
+    states = b.download()
+    cells_iter = sum(c.mesh.n_cells * r.iterations for c, r in zip(cases, runs))
+    status = ["converged", "budget", "diverged"]
+    return {
+        "cases": [
+            {
+                "cells": c.mesh.n_cells,
+                "h": s.h,
+                "mach": s.mach,
+                "alpha": s.alpha,
+                "iterations": r.iterations,
+                "status": status[r.status],
+                "first_order_steps": r.first_order_steps,
+                "plateau_mask": r.plateau_mask,
+                "orders": [round(math.log10(a / max(bb, 1e-300)), 2) for a, bb in zip(r.reference, r.residuals)],
+            }
+            for c, s, r in zip(cases, sp, runs)
+        ],
+        "wall_s": wall,
+        "cells_iter_per_s": cells_iter / wall,
+        "total_cells": sum(c.mesh.n_cells for c in cases),
+        "tuned": tuned,
+        "states": states,
+        "specs": sp,
+        "case_objects": cases,
+    }

@@ -263,12 +263,57 @@ def bump_extras(dev, w, args) -> dict:
     return out
 
 
+def sweep_main(args) -> int:
+    """The batched sweep: every case stepped together to its steady state; aggregate cells x
+    iterations per second over the whole run (one batch, HIP graph replay, autotuned variant)."""
+    import numpy as np
+
+    from autan_amd import sweep
+
+    t0 = time.time()
+    rep = sweep.run(n_cases=args.cases, seed=1, max_iterations=args.max_iterations, policy=args.policy or "fp32+acc64",
+                    flux=args.flux or "hllc", log=log)
+    cases = rep["cases"]
+    iters = [c["iterations"] for c in cases]
+    line = {
+        "metric": "ns/cell/iter",
+        "value": round(1e9 / rep["cells_iter_per_s"], 4),
+        "unit": "ns/cell/iter",
+        "n_gpus": 1,
+        "steps": int(max(iters)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64acc",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.cases} hybrid bump channels (seed 1): cells log-uniform in [2e4, 5e5], h U(0.02, 0.12), "
+            f"Mach U(0.3, 0.8), alpha U(-4, 4) deg; second-order HLLC fp32+acc64, first-order start, local time steps "
+            f"CFL 0.8, to a 1e-8 drop or the round-off plateau; one batch stepper, HIP graph replay",
+            "cells": rep["total_cells"],
+        },
+        "cells_iter_per_s": round(rep["cells_iter_per_s"], 1),
+        "wall_s": round(rep["wall_s"], 2),
+        "tuned": rep["tuned"],
+        "converged": sum(c["status"] == "converged" for c in cases),
+        "iterations": {"min": int(min(iters)), "median": float(np.median(iters)), "max": int(max(iters))},
+        "cases": cases,
+        "setup_s": round(time.time() - t0 - rep["wall_s"], 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (tets 20, bump 20, vortex 200)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump"])
+    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump", "sweep"])
+    ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
+    ap.add_argument("--max-iterations", type=int, default=20000, help="sweep: iteration budget per case")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
     ap.add_argument("--bump-size", default="462,154,154", help="bump: lattice nx,ny,nz")
     ap.add_argument("--policy", default=None, choices=["fp32+acc64", "fp64"])
@@ -285,6 +330,8 @@ def main() -> int:
         print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
               "(the sharding design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.workload == "sweep":
+        return sweep_main(args)
     defaults = {
         "tets": ("fp32+acc64", "hllc", 20),
         "vortex": ("fp64", "roe", 200),

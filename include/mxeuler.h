@@ -239,10 +239,16 @@ int mxeuler_batch_step(mxeuler_stepper* s, double cfl, int32_t local, double* re
 /* Per case: bit 0 active (an inactive case is not stepped and keeps its state), bit 1 second order
  * (a case turned to first order has its gradients cleared). */
 int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, size_t err_len);
-/* Every case's steady run at once (mxeuler_steady's rules per case): results per case; history (5
- * per iteration, case k from history + 5 k max_iterations) may be NULL. */
-int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* results,
-                         double* history, char* err, size_t err_len);
+/* Every case's steady run at once (mxeuler_steady's rules per case): case_scales (5 per case, the
+ * fp32+acc64 floors' Phi_k; NULL: none) replace options->scales; results per case; history (5 per
+ * iteration, case k from history + 5 k max_iterations) may be NULL. */
+int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, const double* case_scales,
+                         mxsteady_result* results, double* history, char* err, size_t err_len);
+/* Times `steps` steps of every kernel variant from the current state (restored afterwards), keeps
+ * the fastest and describes the trials in `report` ("best face_stage=..,graph=..;..."). No
+ * counterpart in the reference: the project's autotuner. */
+int mxeuler_autotune(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local, char* report, size_t report_len,
+                     char* err, size_t err_len);
 
 const char* mxeuler_version(void);
 

@@ -89,3 +89,18 @@ def test_batch_steady_matches_each_solo_run(start, graph):
         assert got[j].first_order_steps == want.first_order_steps and got[j].first_order_end == want.first_order_end
         np.testing.assert_array_equal(got[j].history, want.history)
         np.testing.assert_array_equal(states[j], d.download())
+
+
+def test_autotune_restores_the_state_and_keeps_the_bits():
+    cs = make_cases()
+    b = batch.BatchStepper(cs, "fp32+acc64")
+    b.upload()
+    report = b.autotune(steps=2)
+    assert report.startswith("best face_stage=")
+    for got, c in zip(b.download(), cs):
+        np.testing.assert_array_equal(got, c.u)
+    r, h, inv = b.step(cfl=0.8, local=True)
+    for j, c in enumerate(cs):
+        d = solo(c, "fp32+acc64")
+        d.upload(c.u)
+        np.testing.assert_array_equal(r[j], d.step(cfl=0.8, local=True).residuals)
