@@ -2,6 +2,7 @@
 
     libmxmesh.so   host set-up, g++ -O3 -ffp-contract=off
     libmxeuler.so  HIP kernels and the C ABI, hipcc for gfx950
+    libmxmlp.so    the warm-start network (BF16 matrix-core kernels), hipcc for gfx950
     oracle/_build/liboracle.so   the CPU oracle (test infrastructure), g++ -O3 -ffp-contract=off
 
 Every build passes -ffp-contract=off, as the reference builds host and device code
@@ -57,6 +58,12 @@ def build_device() -> Path:
     return out
 
 
+def build_mlp() -> Path:
+    out = PKG / "libmxmlp.so"
+    _run([HIPCC, *HIPFLAGS, "-o", out, PKG / "csrc" / "mlp" / "mlp.hip"])
+    return out
+
+
 def build_oracle() -> Path:
     out = ROOT / "oracle" / "_build" / "liboracle.so"
     out.parent.mkdir(parents=True, exist_ok=True)
@@ -71,9 +78,10 @@ def build_all() -> None:
     build_mesh()
     build_oracle()
     build_device()
+    build_mlp()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "all": build_all}[w]()
+        {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "mlp": build_mlp, "all": build_all}[w]()

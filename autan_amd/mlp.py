@@ -1,0 +1,129 @@
+"""The warm-start network over libmxmlp.so (include/mxmlp.h): inputs are a cell's centroid, its
+wall distance and the case's bump height, Mach number and flow angle; outputs the initial state
+(rho, u, v, w, p)."""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "libmxmlp.so"
+_p = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    if not LIB.exists():
+        raise RuntimeError(f"{LIB} is not built: run autan_amd/build.py mlp (or __graft_entry__.build())")
+    L = ctypes.CDLL(str(LIB))
+    L.mxmlp_param_count.restype = ctypes.c_int64
+    L.mxmlp_param_count.argtypes = [_p]
+    L.mxmlp_destroy.argtypes = [_p]
+    return L
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+    def __init__(self, inputs: int = 7, hidden: int = 256, layers: int = 4, outputs: int = 5, seed: int = 0) -> None:
+        self.lib = lib()
+        self.inputs, self.hidden, self.layers, self.outputs = inputs, hidden, layers, outputs
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        rc = self.lib.mxmlp_create(ctypes.c_int32(inputs), ctypes.c_int32(hidden), ctypes.c_int32(layers),
+                                   ctypes.c_int32(outputs), ctypes.c_uint64(seed), ctypes.byref(h), err,
+                                   ctypes.c_size_t(len(err)))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        self.h = h
+        self.norm = [np.zeros(inputs, np.float32), np.ones(inputs, np.float32), np.zeros(outputs, np.float32),
+                     np.ones(outputs, np.float32)]
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.mxmlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self) -> None:
+        self.close()
+
+    def _call(self, fn, *args):
+        err = ctypes.create_string_buffer(1024)
+        rc = fn(self.h, *args, err, ctypes.c_size_t(len(err)))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+
+    @property
+    def param_count(self) -> int:
+        return int(self.lib.mxmlp_param_count(self.h))
+
+    def set_normalization(self, in_mean, in_scale, out_mean, out_scale) -> None:
+        self.norm = [_f32(in_mean), _f32(in_scale), _f32(out_mean), _f32(out_scale)]
+        self._call(self.lib.mxmlp_set_normalization, *[_p(a.ctypes.data) for a in self.norm])
+
+    def fit_normalization(self, x: np.ndarray, t: np.ndarray) -> None:
+        """Means and standard deviations of the data (a scale of 1 where a column is constant)."""
+        def ms(a):
+            m = a.mean(axis=0)
+            s = a.std(axis=0)
+            return m, np.where(s > 1e-12, s, 1.0)
+
+        im, isc = ms(np.asarray(x, np.float64))
+        om, osc = ms(np.asarray(t, np.float64))
+        self.set_normalization(im, isc, om, osc)
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.param_count, np.float32)
+        self._call(self.lib.mxmlp_get_params, _p(out.ctypes.data))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = _f32(p)
+        assert p.size == self.param_count
+        self._call(self.lib.mxmlp_set_params, _p(p.ctypes.data))
+
+    def predict(self, x: np.ndarray) -> np.ndarray:
+        x = _f32(x).reshape(-1, self.inputs)
+        y = np.empty((x.shape[0], self.outputs), np.float32)
+        self._call(self.lib.mxmlp_predict, _p(x.ctypes.data), ctypes.c_int64(x.shape[0]), _p(y.ctypes.data))
+        return y
+
+    def gradients(self, x: np.ndarray, t: np.ndarray) -> tuple[float, np.ndarray]:
+        x = _f32(x).reshape(-1, self.inputs)
+        t = _f32(t).reshape(-1, self.outputs)
+        g = np.empty(self.param_count, np.float32)
+        loss = ctypes.c_float()
+        self._call(self.lib.mxmlp_gradients, _p(x.ctypes.data), _p(t.ctypes.data), ctypes.c_int32(x.shape[0]),
+                   _p(g.ctypes.data), ctypes.byref(loss))
+        return loss.value, g
+
+    def adam_step(self, lr: float) -> None:
+        self._call(self.lib.mxmlp_adam_step, ctypes.c_float(lr))
+
+    def train(self, x: np.ndarray, t: np.ndarray, epochs: int, batch: int = 4096, lr: float = 1e-3,
+              seed: int = 0) -> tuple[float, float]:
+        """Epochs of shuffled batches on device-resident data: (last epoch's loss, seconds)."""
+        x = _f32(x).reshape(-1, self.inputs)
+        t = _f32(t).reshape(-1, self.outputs)
+        self._call(self.lib.mxmlp_set_data, _p(x.ctypes.data), _p(t.ctypes.data), ctypes.c_int64(x.shape[0]))
+        loss = ctypes.c_float()
+        sec = ctypes.c_double()
+        self._call(self.lib.mxmlp_train, ctypes.c_int32(epochs), ctypes.c_int32(batch), ctypes.c_float(lr),
+                   ctypes.c_uint64(seed), ctypes.byref(loss), ctypes.byref(sec))
+        return loss.value, sec.value
+
+    def time_forward(self, n: int, repeats: int = 10) -> float:
+        ms = ctypes.c_double()
+        self._call(self.lib.mxmlp_time_forward, ctypes.c_int32(n), ctypes.c_int32(repeats), ctypes.byref(ms))
+        return ms.value
+
+    def widths(self) -> list[int]:
+        return [32] + [self.hidden] * self.layers + [16]
+
+    def flops_per_sample(self) -> int:
+        """Multiply-adds x 2 of the real (unpadded) layers."""
+        w = [self.inputs] + [self.hidden] * self.layers + [self.outputs]
+        return 2 * sum(w[i] * w[i + 1] for i in range(len(w) - 1))

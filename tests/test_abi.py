@@ -20,7 +20,13 @@ def declared(header: str) -> list[str]:
     return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\**\s*(mx\w+)\s*\(", text, flags=re.M)))
 
 
-@pytest.mark.parametrize("header,loader", [("mxmesh.h", _lib.mesh), ("mxeuler.h", _lib.euler)])
+def _mlp():
+    from autan_amd import mlp
+
+    return mlp.lib()
+
+
+@pytest.mark.parametrize("header,loader", [("mxmesh.h", _lib.mesh), ("mxeuler.h", _lib.euler), ("mxmlp.h", _mlp)])
 def test_every_declared_symbol_is_exported(header, loader):
     names = declared(header)
     assert len(names) >= 8
