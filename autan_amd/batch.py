@@ -228,6 +228,21 @@ class BatchStepper:
                    _p(inv.ctypes.data))
         return r.reshape(self.n, 5), h, [None if v < 0 else int(v) for v in inv]
 
+    def loads(self, k: int, faces: np.ndarray, center=(0.0, 0.0, 0.0), p_inf: float = 0.0):
+        """EulerPhysics::loads over case k's boundary faces `faces` (the case's own face indices)."""
+        m = self.cases[k].mesh
+        fi, fb = self.u.face_first[k]
+        f = np.asarray(faces, dtype=np.int64)
+        g = np.ascontiguousarray(np.where(f < m.n_interior, fi + f, fb + (f - m.n_interior)), dtype=np.int32)
+        av = np.ascontiguousarray(m.face_area_vector.reshape(3, -1)[:, f])
+        fc = np.ascontiguousarray(m.face_centroid.reshape(3, -1)[:, f])
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        force = (ctypes.c_double * 3)()
+        moment = (ctypes.c_double * 3)()
+        self._call(self.lib.mxeuler_loads, _p(g.ctypes.data), ctypes.c_int64(g.size), _p(av.ctypes.data),
+                   _p(fc.ctypes.data), _p(c.ctypes.data), _d(p_inf), force, moment)
+        return np.array(force[:]), np.array(moment[:])
+
     def tune(self, key: str, value: int) -> None:
         self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))
 
@@ -242,17 +257,20 @@ class BatchStepper:
         return report.value.decode()
 
     def steady(self, max_iterations: int = 10000, residual_orders: float = 8.0, cfl: float = _steady.STEADY_CFL,
-               level_rule: bool | None = None, start_order: str = "first", scales=None, history: bool = True):
-        """Every case's steady run (autan_amd.steady.solve's rules per case); scales: 5 per case."""
+               level_rule: bool | None = None, start_order: str = "first", scales=None, references=None,
+               history: bool = True):
+        """Every case's steady run (autan_amd.steady.solve's rules per case); scales and references:
+        5 per case (references: the residuals to drop from, else each case's iteration 1)."""
         if level_rule is None:
             level_rule = self.policy == 1
         keep: list = []
         opts = _steady._options(max_iterations, residual_orders, cfl, level_rule, start_order, None, None, keep)
         cs = None if scales is None else np.ascontiguousarray(scales, dtype=np.float64).reshape(self.n, 5)
+        cr = None if references is None else np.ascontiguousarray(references, dtype=np.float64).reshape(self.n, 5)
         res = (_steady._Result * self.n)()
         hist = np.zeros(5 * max_iterations * self.n) if history else None
-        self._call(self.lib.mxeuler_batch_steady, ctypes.byref(opts), _p(None if cs is None else cs.ctypes.data), res,
-                   _p(None if hist is None else hist.ctypes.data))
+        self._call(self.lib.mxeuler_batch_steady, ctypes.byref(opts), _p(None if cs is None else cs.ctypes.data),
+                   _p(None if cr is None else cr.ctypes.data), res, _p(None if hist is None else hist.ctypes.data))
         out = []
         for j in range(self.n):
             h = hist[5 * max_iterations * j : 5 * max_iterations * (j + 1)] if hist is not None else np.zeros(0)

@@ -111,8 +111,8 @@ class StepperBase {
   virtual int ncase() const = 0;
   virtual void batch_step(double cfl, bool local, double* residuals, double* time_steps, int* invalid) = 0;
   virtual void set_flags(const int* flags) = 0;
-  virtual void batch_steady(const mxsteady_options& o, const double* case_scales, mxsteady_result* results,
-                            double* history) = 0;
+  virtual void batch_steady(const mxsteady_options& o, const double* case_scales, const double* case_references,
+                            mxsteady_result* results, double* history) = 0;
   virtual std::string autotune(int steps, double cfl, bool local) = 0;
   virtual double gamma() const = 0;
   virtual int profile(int steps, double cfl, bool local, double* ms_step, double* ms_kernel, int* launches,
@@ -485,8 +485,8 @@ class Stepper final : public StepperBase {
   // convergence decided on its own norms (mxe::steady::SteadyCase); a case that ends is frozen
   // and its blocks skipped. history: 5 per iteration per case, case k at history + 5 k
   // max_iterations (or null).
-  void batch_steady(const mxsteady_options& o, const double* case_scales, mxsteady_result* results,
-                    double* history) override {
+  void batch_steady(const mxsteady_options& o, const double* case_scales, const double* case_references,
+                    mxsteady_result* results, double* history) override {
     if (ncase_ == 1) throw std::invalid_argument("batch_steady needs a batch");
     steady::check_options(o);
     if (o.start_order == 1 && !second_) throw std::invalid_argument("a first-order start needs a second-order batch");
@@ -497,6 +497,7 @@ class Stepper final : public StepperBase {
     for (int k = 0; k < ncase_; ++k) {
       mxsteady_options ok = o;
       ok.scales = case_scales != nullptr ? case_scales + kSums * std::size_t(k) : nullptr;
+      ok.reference = case_references != nullptr ? case_references + kSums * std::size_t(k) : nullptr;
       runs.emplace_back(ok);
       flags[std::size_t(k)] = 1 | ((runs.back().second() && second_) ? 2 : 0);
     }
@@ -1693,10 +1694,11 @@ int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, siz
 }
 
 int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, const double* case_scales,
-                         mxsteady_result* results, double* history, char* err, size_t err_len) {
+                         const double* case_references, mxsteady_result* results, double* history, char* err,
+                         size_t err_len) {
   return guarded(err, err_len, [&] {
     if (!options || !results) throw std::invalid_argument("null argument");
-    s->impl->batch_steady(*options, case_scales, results, history);
+    s->impl->batch_steady(*options, case_scales, case_references, results, history);
   });
 }
 

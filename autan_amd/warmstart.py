@@ -1,0 +1,127 @@
+"""The learned warm start of the bump sweep: a network maps a cell (centroid, distance to the bump
+wall) and its case (bump height, Mach number, flow angle) to an initial state (rho, u, v, w, p). It
+is trained on converged states that the batch stepper computes for cases of the sweep's
+generator, and judged on held-out cases solved to the same tolerance from the freestream and from
+the network's state: every run still converges under the plain solver's rules, so the network
+only changes where the iterations start.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from autan_amd import batch, sweep
+
+GAMMA = 1.4
+
+
+def wall_distance(points: np.ndarray, h: float, w: float = 0.2, x0: float = 1.5, length: float = 3.0,
+                  samples: int = 30001) -> np.ndarray:
+    """Distance from each point to the bump wall y = h exp(-((x - x0) / w)^2), 0 <= x <= length
+    (a curve extruded along z): the nearest of `samples` points on the curve, then a Newton
+    refinement of the foot point along the curve."""
+    xs = np.linspace(0.0, length, samples)
+    ys = h * np.exp(-(((xs - x0) / w) ** 2))
+    tree = cKDTree(np.stack([xs, ys], axis=1))
+    p = np.asarray(points, dtype=np.float64)[:, :2]
+    _, idx = tree.query(p)
+    x = xs[idx]
+    for _ in range(3):
+        t = (x - x0) / w
+        f = h * np.exp(-t * t)
+        f1 = -2.0 * t / w * f
+        f2 = (4.0 * t * t - 2.0) / (w * w) * f
+        # minimise (x - px)^2 + (f(x) - py)^2
+        g = (x - p[:, 0]) + (f - p[:, 1]) * f1
+        gg = 1.0 + f1 * f1 + (f - p[:, 1]) * f2
+        x = np.clip(x - g / np.where(np.abs(gg) > 1e-12, gg, 1.0), 0.0, length)
+    f = h * np.exp(-(((x - x0) / w) ** 2))
+    return np.hypot(p[:, 0] - x, p[:, 1] - f)
+
+
+def features(mesh, spec: sweep.Spec) -> np.ndarray:
+    """n x 7: centroid x, y, z, distance to the bump wall, bump height, Mach number, flow angle."""
+    x = mesh.centroids()
+    d = wall_distance(x, spec.h)
+    n = mesh.n_cells
+    return np.column_stack([x, d, np.full(n, spec.h), np.full(n, spec.mach), np.full(n, spec.alpha)]).astype(np.float32)
+
+
+def primitive(u: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
+    """n x 5 (rho, u, v, w, p) of a 5 x n conserved state."""
+    q = np.asarray(u).reshape(5, -1)
+    rho = q[0]
+    v = q[1:4] / rho
+    p = (gamma - 1.0) * (q[4] - 0.5 * rho * np.sum(v * v, axis=0))
+    return np.column_stack([rho, v[0], v[1], v[2], p])
+
+
+def conserved(w: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
+    """5 x n conserved state of n x 5 primitives."""
+    rho, u, v, ww, p = (np.asarray(w, dtype=np.float64)[:, k] for k in range(5))
+    return np.stack([rho, rho * u, rho * v, rho * ww, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + ww * ww)]).reshape(-1)
+
+
+def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=20000, residual_orders=8.0,
+          policy="fp32+acc64", flux="hllc", batch_cases=64, log=print):
+    """Solves the cases (from the freestream, or from `states`) in batches of `batch_cases`;
+    returns per case (result, final state, case, seconds of its batch) and the total seconds."""
+    out = []
+    total = 0.0
+    for b0 in range(0, len(specs), batch_cases):
+        part = specs[b0 : b0 + batch_cases]
+        cases = [sweep.build(s, policy, flux) for s in part]
+        if states is not None:
+            for c, u in zip(cases, states[b0 : b0 + batch_cases]):
+                c.u = u
+        bs = batch.BatchStepper(cases, policy)
+        bs.tune("graph", 1)
+        bs.upload()
+        scales = np.array([sweep.residual_scales(c) for c in cases])
+        refs = None if references is None else np.asarray(references[b0 : b0 + batch_cases])
+        t0 = time.perf_counter()
+        runs = bs.steady(max_iterations=max_iterations, residual_orders=residual_orders, scales=scales, references=refs,
+                         history=False)
+        sec = time.perf_counter() - t0
+        total += sec
+        finals = bs.download()
+        wall = []
+        for k, c in enumerate(cases):
+            faces = np.nonzero(c.mesh.face_zone == 2)[0] + c.mesh.n_interior
+            wall.append(bs.loads(k, faces, center=(1.5, 0.0, 0.5), p_inf=1.0 / GAMMA))
+        bs.close()
+        log(f"warm start: solved cases {b0}..{b0 + len(part) - 1} in {sec:.1f} s")
+        for r, u, c, ld in zip(runs, finals, cases, wall):
+            out.append({"result": r, "state": u, "case": c, "loads": ld})
+    return out, total
+
+
+def training_set(solved, per_case: int = 4096, seed: int = 0):
+    """A random sample of each solved case's cells: features and converged primitives."""
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    for item, spec in solved:
+        mesh = item["case"].mesh
+        n = mesh.n_cells
+        pick = rng.choice(n, size=min(per_case, n), replace=False)
+        f = features(mesh, spec)[pick]
+        w = primitive(item["state"])[pick].astype(np.float32)
+        xs.append(f)
+        ts.append(w)
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def warm_states(net, specs: list[sweep.Spec], policy="fp32+acc64", flux="hllc"):
+    """The network's initial state for each case (density and pressure kept positive)."""
+    out = []
+    for s in specs:
+        c = sweep.build(s, policy, flux)
+        w = net.predict(features(c.mesh, s)).astype(np.float64)
+        w[:, 0] = np.maximum(w[:, 0], 0.05)
+        w[:, 4] = np.maximum(w[:, 4], 0.05 / GAMMA)
+        out.append(conserved(w))
+    return out

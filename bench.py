@@ -306,12 +306,86 @@ def sweep_main(args) -> int:
     return 0
 
 
+def warmstart_main(args) -> int:
+    """The learned warm start: solve the training cases (the sweep's generator, seed 11, meshes of
+    2e4 to 1e5 cells), train the network on their converged cells, then solve the held-out cases
+    (seed 12, the sweep's full range) from the freestream and from the network's state."""
+    import numpy as np
+
+    from autan_amd import sweep, warmstart
+    from autan_amd.mlp import MLP
+
+    t0 = time.time()
+    train_specs = sweep.specs(args.train_cases, seed=11, cells=(2e4, 1e5))
+    test_specs = sweep.specs(args.test_cases, seed=12)
+    solved, t_gen = warmstart.solve(train_specs, max_iterations=args.max_iterations,
+                                    residual_orders=args.train_orders, log=log)
+    x, t = warmstart.training_set(list(zip(solved, train_specs)))
+    log(f"training set: {x.shape[0]} cells of {len(train_specs)} cases ({t_gen:.1f} s of solves)")
+    net = MLP(7, 256, 4, 5, seed=0)
+    net.fit_normalization(x, t)
+    loss, t_train = net.train(x, t, epochs=args.epochs, batch=4096, lr=1e-3, seed=0)
+    fwd_ms = net.time_forward(1 << 18, 5)
+    tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
+    log(f"network: loss {loss:.3e} after {args.epochs} epochs ({t_train:.1f} s); forward {tflops:.1f} TFLOP/s")
+    free, t_free = warmstart.solve(test_specs, max_iterations=args.max_iterations, log=log)
+    t1 = time.perf_counter()
+    warm_u = warmstart.warm_states(net, test_specs)
+    t_infer = time.perf_counter() - t1
+    warm, t_warm = warmstart.solve(test_specs, states=warm_u, max_iterations=args.max_iterations, log=log)
+    refs = [f["result"].reference for f in free]
+    warm_abs, t_warm_abs = warmstart.solve(test_specs, states=warm_u, references=refs,
+                                           max_iterations=args.max_iterations, log=log)
+    rows = []
+    for s, f, w, wa in zip(test_specs, free, warm, warm_abs):
+        ff = f["loads"][0]
+        dw = float(np.linalg.norm(wa["loads"][0] - ff) / max(np.linalg.norm(ff), 1e-300))
+        rows.append({
+            "cells": f["case"].mesh.n_cells, "h": s.h, "mach": s.mach, "alpha": s.alpha,
+            "freestream": [f["result"].iterations, ["converged", "budget", "diverged"][f["result"].status]],
+            "warm_own_reference": [w["result"].iterations, ["converged", "budget", "diverged"][w["result"].status]],
+            "warm_freestream_reference": [wa["result"].iterations,
+                                          ["converged", "budget", "diverged"][wa["result"].status]],
+            "force_free": [float(v) for v in ff], "force_rel_diff": dw,
+        })
+    it_free = sum(r["freestream"][0] for r in rows)
+    it_warm = sum(r["warm_freestream_reference"][0] for r in rows)
+    line = {
+        "metric": "wall time to 1e-8 residual, learned vs freestream",
+        "value": round(t_free / (t_warm_abs + t_infer), 3),
+        "unit": "x (freestream seconds / warm seconds)",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "bf16 (network), f32+f64acc (solver)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"MLP 7-256x4-5 SiLU, BF16 matrix cores, trained on {len(train_specs)} solved bump channels "
+            f"(seed 11, 2e4-1e5 cells, {args.train_orders:g} orders), tested on {len(test_specs)} held-out channels "
+            f"(seed 12, 2e4-5e5 cells) solved to 1e-8",
+        },
+        "train": {"cells": int(x.shape[0]), "solve_s": round(t_gen, 1), "train_s": round(t_train, 2),
+                  "loss": loss, "forward_TFLOPs": round(tflops, 1), "forward_ms_262144": round(fwd_ms, 3)},
+        "test": {"freestream_s": round(t_free, 2), "warm_s_own_reference": round(t_warm, 2),
+                 "warm_s_freestream_reference": round(t_warm_abs, 2), "inference_s": round(t_infer, 2),
+                 "iterations_freestream": it_free, "iterations_warm": it_warm,
+                 "max_force_rel_diff": max(r["force_rel_diff"] for r in rows)},
+        "cases": rows,
+        "total_s": round(time.time() - t0, 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (tets 20, bump 20, vortex 200)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump", "sweep"])
+    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump", "sweep", "warmstart"])
+    ap.add_argument("--train-cases", type=int, default=512, help="warmstart: solved cases the network learns from")
+    ap.add_argument("--test-cases", type=int, default=32, help="warmstart: held-out cases")
+    ap.add_argument("--train-orders", type=float, default=6.0, help="warmstart: residual drop of the training solves")
+    ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
     ap.add_argument("--max-iterations", type=int, default=20000, help="sweep: iteration budget per case")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
@@ -332,6 +406,8 @@ def main() -> int:
         return 2
     if args.workload == "sweep":
         return sweep_main(args)
+    if args.workload == "warmstart":
+        return warmstart_main(args)
     defaults = {
         "tets": ("fp32+acc64", "hllc", 20),
         "vortex": ("fp64", "roe", 200),

@@ -240,10 +240,12 @@ int mxeuler_batch_step(mxeuler_stepper* s, double cfl, int32_t local, double* re
  * (a case turned to first order has its gradients cleared). */
 int mxeuler_batch_flags(mxeuler_stepper* s, const int32_t* flags, char* err, size_t err_len);
 /* Every case's steady run at once (mxeuler_steady's rules per case): case_scales (5 per case, the
- * fp32+acc64 floors' Phi_k; NULL: none) replace options->scales; results per case; history (5 per
- * iteration, case k from history + 5 k max_iterations) may be NULL. */
+ * fp32+acc64 floors' Phi_k; NULL: none) and case_references (5 per case, the residuals to drop
+ * from; NULL: each case's iteration 1) replace options->scales and options->reference; results per
+ * case; history (5 per iteration, case k from history + 5 k max_iterations) may be NULL. */
 int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, const double* case_scales,
-                         mxsteady_result* results, double* history, char* err, size_t err_len);
+                         const double* case_references, mxsteady_result* results, double* history, char* err,
+                         size_t err_len);
 /* Times `steps` steps of every kernel variant from the current state (restored afterwards), keeps
  * the fastest and describes the trials in `report` ("best face_stage=..,graph=..;..."). No
  * counterpart in the reference: the project's autotuner. */
