@@ -349,8 +349,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d) {
 // records), then every stencil entry gathers its primitives from LDS through a 16-bit position.
 // Weights, positions and the limiter's face offsets stream in the wave-sliced layout. The sums,
 // bounds and minima run in the same order as k_grad.
-template <class C, bool LIMITER>
-__global__ __launch_bounds__(kBlock) void k_grad_lds(Dev<C> d) {
+// OCC > 0 asks the compiler for OCC waves per SIMD (registers capped accordingly): a variant for
+// the autotuner, same bits.
+template <class C, bool LIMITER, int OCC = 0>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_grad_lds(Dev<C> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   C* tab = reinterpret_cast<C*>(smem);
   const int blk = xcd_block();
@@ -957,9 +959,9 @@ __global__ void k_clear(Small* small, int ncase, long long* nslots) {
 // from zero in FP64, onto -V s (kernels/euler.py:611-616): the store_gather order of the
 // reference's device path, with no face array in HBM. FIRST (the step's R(u)) also gathers the
 // spectral radius, the cell's time step and the block's least one, and the exact sums of R^2.
-template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE>
-__global__ __launch_bounds__(kBlock) void k_faces(Dev<C> d, double cfl, int store_dt,
-                                                  unsigned long long* partial) {
+template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE, int OCC = 0>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_faces(
+    Dev<C> d, double cfl, int store_dt, unsigned long long* partial) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kComp = FIRST ? 7 : 5;
   const int ms = d.max_slots;

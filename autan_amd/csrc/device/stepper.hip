@@ -353,6 +353,12 @@ class Stepper final : public StepperBase {
       use_graph_ = value != 0;
       return;
     }
+    if (key == "grad_occupancy" || key == "face_occupancy") {
+      if (value != 0 && value != 8) throw std::invalid_argument(key + ": 0 or 8");
+      drop_graphs();
+      (key == "grad_occupancy" ? grad_occ_ : face_occ_) = value;
+      return;
+    }
     throw std::invalid_argument("unknown tuning key '" + key + "'");
   }
 
@@ -386,10 +392,16 @@ class Stepper final : public StepperBase {
     std::string report;
     double best = std::numeric_limits<double>::infinity();
     int best_stage = 0, best_graph = 0;
-    for (int st = 0; st <= (can_stage ? 1 : 0); ++st) {
-      for (int gr = 0; gr <= 1; ++gr) {
+    int best_go = 0, best_fo = 0;
+    for (int vi = 0; vi < 16; ++vi) {
+      const int st = vi & 1, gr = (vi >> 1) & 1, go = (vi >> 2) & 1 ? 8 : 0, fo = (vi >> 3) & 1 ? 8 : 0;
+      if (st && !can_stage) continue;
+      if (fo && (st || !std::is_same_v<C, float>)) continue;
+      {
         tune("face_stage", st);
         tune("graph", gr);
+        tune("grad_occupancy", go);
+        tune("face_occupancy", fo);
         restore();
         one();  // warm-up (records the graph)
         restore();
@@ -397,18 +409,25 @@ class Stepper final : public StepperBase {
         for (int k = 0; k < steps; ++k) one();
         const double ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-        report += "face_stage=" + std::to_string(st) + ",graph=" + std::to_string(gr) + ":" + std::to_string(ms) + "ms;";
+        const std::string name = "face_stage=" + std::to_string(st) + ",graph=" + std::to_string(gr) +
+                                 ",grad_occupancy=" + std::to_string(go) + ",face_occupancy=" + std::to_string(fo);
+        report += name + ":" + std::to_string(ms) + "ms;";
         if (ms < best) {
           best = ms;
           best_stage = st;
           best_graph = gr;
+          best_go = go;
+          best_fo = fo;
         }
       }
     }
     tune("face_stage", best_stage);
     tune("graph", best_graph);
+    tune("grad_occupancy", best_go);
+    tune("face_occupancy", best_fo);
     restore();
-    return "best face_stage=" + std::to_string(best_stage) + ",graph=" + std::to_string(best_graph) + ";" + report;
+    return "best face_stage=" + std::to_string(best_stage) + ",graph=" + std::to_string(best_graph) +
+           ",grad_occupancy=" + std::to_string(best_go) + ",face_occupancy=" + std::to_string(best_fo) + ";" + report;
   }
 
   // One SSP-RK3 step of every active case: norms, time step and invalid cell per case.
@@ -1128,7 +1147,9 @@ class Stepper final : public StepperBase {
     if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d); });
     if (second_) {
       const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
-      if (staged_ && sc_.limiter) {
+      if (staged_ && sc_.limiter && grad_occ_ == 8) {
+        launch("grad_limiter", [&] { k_grad_lds<C, true, 8><<<grid(nc_), kBlock, lds, stream_>>>(d); });
+      } else if (staged_ && sc_.limiter) {
         launch("grad_limiter", [&] { k_grad_lds<C, true><<<grid(nc_), kBlock, lds, stream_>>>(d); });
       } else if (staged_) {
         launch("grad", [&] { k_grad_lds<C, false><<<grid(nc_), kBlock, lds, stream_>>>(d); });
@@ -1161,6 +1182,16 @@ class Stepper final : public StepperBase {
   void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
     const int g = xcd_grid(face_blocks_);
     if constexpr (std::is_same_v<C, float>) {
+      if (face_occ_ == 8 && !stage_faces_) {
+        if (first) {
+          k_faces<C, FLUX, PLANAR, SECOND, true, false, 8><<<g, kBlock, faces_lds(true), stream_>>>(
+              d, cfl, local ? 1 : 0, partial_.get());
+        } else {
+          k_faces<C, FLUX, PLANAR, SECOND, false, false, 8><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0,
+                                                                                                     nullptr);
+        }
+        return;
+      }
       if (stage_faces_) {
         if (first) {
           k_faces<C, FLUX, PLANAR, SECOND, true, true><<<g, kBlock, faces_lds(true, true), stream_>>>(
@@ -1360,6 +1391,7 @@ class Stepper final : public StepperBase {
   Buffer<int> cbeg_, sbeg_;
   std::vector<int> sface_host_;  // the faces of every slot, until the slot records are built
   bool stage_faces_ = false;     // fp32: the face kernel stages its block's cells in LDS
+  int grad_occ_ = 0, face_occ_ = 0;  // 8: the gradient / face kernel built for 8 waves per SIMD
   Buffer<C> srec_, smom_;
   int face_blocks_ = 0;
   Buffer<int> tbeg_, tent_, lens_;

@@ -115,24 +115,30 @@ def test_step_matches_oracle(name, policy, local):
         assert r.time_step == h
 
 
+@pytest.mark.parametrize("variant", [("face_stage", 1), ("grad_occupancy", 8), ("face_occupancy", 8)])
 @pytest.mark.parametrize("local", [False, True])
 @pytest.mark.parametrize("name", ["tets-hllc2", "poly-ausm2-warped", "bump-hybrid-hllc2"])
-def test_staged_face_kernel_gives_the_same_bits(name, local):
-    """The face kernel variant that stages its block's cells in LDS (fp32+acc64) computes the same
-    residual, norms and state as the unstaged one, bit for bit."""
+def test_kernel_variants_give_the_same_bits(name, local, variant):
+    """The autotuner's kernel variants (the face kernel staging its block's cells in LDS, the
+    gradient and face kernels built for 8 waves per SIMD) compute the same residual, norms and
+    state as the default kernels, bit for bit (fp32+acc64)."""
     case = CASES[name]()
     plain, _ = build(case, "fp32+acc64")
-    staged, _ = build(case, "fp32+acc64")
-    staged.tune("face_stage", 1)
-    np.testing.assert_array_equal(staged.residual(case.u), plain.residual(case.u))
-    for dev in (plain, staged):
+    other, _ = build(case, "fp32+acc64")
+    other.tune(*variant)
+    np.testing.assert_array_equal(other.residual(case.u), plain.residual(case.u))
+    for dev in (plain, other):
         dev.upload(case.u)
     for _ in range(2):
         a = plain.step(cfl=0.5, local=local)
-        b = staged.step(cfl=0.5, local=local)
+        b = other.step(cfl=0.5, local=local)
         np.testing.assert_array_equal(a.residuals, b.residuals)
         assert a.time_step == b.time_step
-    np.testing.assert_array_equal(plain.download(), staged.download())
+    np.testing.assert_array_equal(plain.download(), other.download())
+
+
+def test_face_stage_needs_fp32():
+    case = CASES["tets-hllc2"]()
     with pytest.raises(RuntimeError, match="face_stage"):
         build(case, "fp64")[0].tune("face_stage", 1)
 
