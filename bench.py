@@ -398,7 +398,9 @@ def main() -> int:
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--steady-iterations", type=int, default=400)
     ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
-    ap.add_argument("--face-stage", type=int, default=None, help="face kernel variant: stage the block's cells in LDS")
+    ap.add_argument("--tune", default="", help="kernel variants, key=value,... (face_stage, graph, grad_occupancy, "
+                    "face_occupancy); set after the autotuner")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
@@ -425,9 +427,14 @@ def main() -> int:
     w = make_workload(args.workload, args, size)
     m = w.mesh
     dev = DeviceStepper(m, w.scheme, w.policy, w.bc_kind, w.bc, w.source)
-    if args.face_stage is not None:
-        dev.tune("face_stage", args.face_stage)
     dev.upload(w.u)
+    tuned = None
+    if not args.no_autotune:
+        tuned = dev.autotune(steps=3, cfl=w.cfl, local=w.local)
+        log(f"autotune: {tuned}")
+    for kv in filter(None, args.tune.split(",")):
+        key, value = kv.split("=")
+        dev.tune(key, int(value))
     log("uploaded; warming up")
     warmup = 0 if args.workload == "vortex" else args.warmup
     elapsed, r = timed_steps(dev, w, warmup, steps)
@@ -479,6 +486,7 @@ def main() -> int:
             "stencil_entries": m.n_stencil,
         },
         "cells_iter_per_s": round(m.n_cells / (ms_step * 1e-3), 1),
+        "variant": None if tuned is None else tuned.split(";")[0].removeprefix("best "),
         "roofline": roof,
         "step": step,
         "cpu_baseline": cpu,
