@@ -386,10 +386,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
   const int lens = d.lens[c];
   const int len = lens & 0xffff;
   const long long base = d.soff[c / kWave] + (c % kWave);
+  // Entry j + 1's position and weights are loaded while entry j is summed (its loads' latency
+  // hidden behind the arithmetic); the sums run in entry order as before.
+  int l_next = 0;
+  C w0_next = C(0), w1_next = C(0), w2_next = C(0);
+  if (len > 0) {
+    l_next = d.sloc[base];
+    w0_next = d.sw0[base];
+    w1_next = d.sw1[base];
+    w2_next = d.sw2[base];
+  }
   for (int j = 0; j < len; ++j) {
-    const long long k = base + static_cast<long long>(j) * kWave;
-    const int l = d.sloc[k];
-    const C w0 = d.sw0[k], w1 = d.sw1[k], w2 = d.sw2[k];
+    const int l = l_next;
+    const C w0 = w0_next, w1 = w1_next, w2 = w2_next;
+    if (j + 1 < len) {
+      const long long k = base + static_cast<long long>(j + 1) * kWave;
+      l_next = d.sloc[k];
+      w0_next = d.sw0[k];
+      w1_next = d.sw1[k];
+      w2_next = d.sw2[k];
+    }
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       const C wn = tab[i * tn + l];
@@ -416,9 +432,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
     }
     const int ilen = lens >> 16;
     const long long ib = d.ioff[c / kWave] + (c % kWave);
+    C r0_next = C(0), r1_next = C(0), r2_next = C(0);
+    if (ilen > 0) {
+      r0_next = d.ir0[ib];
+      r1_next = d.ir1[ib];
+      r2_next = d.ir2[ib];
+    }
     for (int j = 0; j < ilen; ++j) {
-      const long long k = ib + static_cast<long long>(j) * kWave;
-      const C r0 = d.ir0[k], r1 = d.ir1[k], r2 = d.ir2[k];
+      const C r0 = r0_next, r1 = r1_next, r2 = r2_next;
+      if (j + 1 < ilen) {
+        const long long k = ib + static_cast<long long>(j + 1) * kWave;
+        r0_next = d.ir0[k];
+        r1_next = d.ir1[k];
+        r2_next = d.ir2[k];
+      }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
         const C dd = out[3 * i] * r0 + out[3 * i + 1] * r1 + out[3 * i + 2] * r2;
