@@ -45,7 +45,8 @@ struct Params {
   double ref[5];
   double p_ref;  // FP64 p_ref for the validity check, rounded there as the reference rounds it
   double gamma;
-  long long cell0;  // the case's first cell: invalid cells are reported relative to it
+  long long cell0;     // the case's first cell: invalid cells are reported relative to it
+  long long cell_end;  // one past its last real cell; the filler cells after it keep their state
 };
 
 template <class C>
@@ -829,7 +830,15 @@ __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, double a, double b, con
   const Params<C>& pa = d.ptab[cs];
   const int c = blk * kBlock + threadIdx.x;
   bool bad_state = false, bad_res = false;
-  if (c < d.nc) {
+  if (c >= pa.cell_end && c < d.nc) {
+    // A filler cell (no faces, so no time step of its own): its state stays.
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const long long i = static_cast<long long>(k) * d.nc + c;
+      stage_out[i] = u[i];
+    }
+    if constexpr (PRIM) store_w(d.prim, c, primitive_of(stage_out, d.nc, c, pa));
+  } else if (c < d.nc) {
     const double h = local ? d.dt[c] : (use_hkey ? key_value(d.small[cs].hkey) : hvalue);
     const double hv = h / d.vol64[c];
 #pragma unroll

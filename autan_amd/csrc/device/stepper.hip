@@ -154,7 +154,7 @@ class Stepper final : public StepperBase {
     nb_ = nf_ - ni_;
     second_ = sc_.second_order != 0;
     planar_ = sc_.planar != 0;
-    for (int k = 0; k < ncase; ++k) ptab_host_.push_back(params(schemes[k], case_first[k]));
+    for (int k = 0; k < ncase; ++k) ptab_host_.push_back(params(schemes[k], case_first[k], case_first[k] + case_cells[k]));
     ptab_.upload(ptab_host_, stream_);
     if (ncase > 1) {
       std::vector<int> bcase(std::size_t(grid(nc_)), ncase - 1);
@@ -744,7 +744,7 @@ class Stepper final : public StepperBase {
 
   // One case's parameters (physics/src/euler.cpp:238-245, in host code built with
   // -ffp-contract=off).
-  static Params<C> params(const mxeuler_scheme& sc, long long cell0) {
+  static Params<C> params(const mxeuler_scheme& sc, long long cell0, long long cell_end) {
     Params<C> pa{};
     const double* r = sc.reference;
     const double p_ref = (sc.gamma - 1.0) * (r[4] - 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[0]);
@@ -761,6 +761,7 @@ class Stepper final : public StepperBase {
     pa.p_ref = p_ref;
     pa.gamma = sc.gamma;
     pa.cell0 = cell0;
+    pa.cell_end = cell_end;
     return pa;
   }
 
