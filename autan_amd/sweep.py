@@ -110,7 +110,8 @@ def run(n_cases: int = 64, seed: int = 1, max_iterations: int = 20000, residual_
                 "status": status[r.status],
                 "first_order_steps": r.first_order_steps,
                 "plateau_mask": r.plateau_mask,
-                "orders": [round(math.log10(a / max(bb, 1e-300)), 2) for a, bb in zip(r.reference, r.residuals)],
+                "orders": [round(math.log10(a / bb), 2) if a > 0 and bb > 0 else None
+                           for a, bb in zip(r.reference, r.residuals)],
             }
             for c, s, r in zip(cases, sp, runs)
         ],

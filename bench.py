@@ -236,8 +236,8 @@ def bump_extras(dev, w, args) -> dict:
         "seconds": round(t, 3),
         "first_order_steps": run.first_order_steps,
         "first_order_end": run.first_order_end,
-        "orders_dropped": [round(math.log10(d), 3) for d in drop],
-        "mass_orders_per_second": round(math.log10(drop[0]) / t, 3),
+        "orders_dropped": [round(math.log10(d), 3) if d > 0 and math.isfinite(d) else None for d in drop],
+        "mass_orders_per_second": round(math.log10(drop[0]) / t, 3) if drop[0] > 0 else None,
         "status": ["converged", "budget", "diverged"][run.status],
     }
     c = dev.conservation()
