@@ -88,7 +88,8 @@ def run(n_cases: int = 64, seed: int = 1, max_iterations: int = 20000, residual_
     tuned = b.autotune(steps=5) if autotune else None
     if tuned is not None:
         log(f"sweep: autotuned {tuned}")
-    b.tune("graph", 1)
+    else:
+        b.tune("graph", 1)
     b.upload()
     # One scale set for the round-off floors: each case's own.
     scales = np.array([residual_scales(c) for c in cases])
