@@ -161,10 +161,11 @@ __global__ void k_inputs(const float* __restrict__ x, int n, int d, const float*
 }
 
 // Mean squared error of the first d of `width` outputs against normalised targets, and its
-// gradient 2 (y - t) / (n d) in BF16 (zero in the padding columns); loss partials per block.
+// gradient 2 (y - t) / (n d) in BF16 (zero in the padding columns); the squared errors summed into
+// *sum (each block's partial added once).
 __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, int n, int d, int width,
                        const float* __restrict__ mean, const float* __restrict__ scale, bf16* __restrict__ dy,
-                       float* __restrict__ partial) {
+                       double* __restrict__ sum) {
   __shared__ float red[256];
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   float e2 = 0.0f;
@@ -185,16 +186,17 @@ __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
     if (static_cast<int>(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) atomicAdd(sum, static_cast<double>(red[0]));
 }
 
-// Column sums of a BF16 n x width matrix (the bias gradients), one thread per column, into g.
+// Column sums of a BF16 n x width matrix (the bias gradients) added into g: thread (x, y) sums
+// column x over the rows y, y + gridDim.y blockDim.y, ...
 __global__ void k_colsum(const bf16* __restrict__ dz, int n, int width, float* __restrict__ g) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= width) return;
   float s = 0.0f;
-  for (int r = 0; r < n; ++r) s += static_cast<float>(dz[static_cast<long long>(r) * width + c]);
-  g[c] = s;
+  for (int r = blockIdx.y; r < n; r += gridDim.y) s += static_cast<float>(dz[static_cast<long long>(r) * width + c]);
+  atomicAdd(g + c, s);
 }
 
 // Adam (Kingma and Ba 2015) on the FP32 master weights, then their BF16 copy. `mask` marks the
@@ -368,7 +370,13 @@ class Net {
     Buf<float> xd(std::size_t(n) * d_in_), td(std::size_t(n) * d_out_);
     check(hipMemcpyAsync(xd.get(), x, std::size_t(n) * d_in_ * sizeof(float), hipMemcpyHostToDevice, s_), "hipMemcpy");
     check(hipMemcpyAsync(td.get(), t, std::size_t(n) * d_out_ * sizeof(float), hipMemcpyHostToDevice, s_), "hipMemcpy");
-    const float loss = backward(xd.get(), td.get(), n);
+    ensure(n);
+    check(hipMemsetAsync(loss_sum_.get(), 0, sizeof(double), s_), "hipMemsetAsync");
+    backward(xd.get(), td.get(), n);
+    double sum = 0.0;
+    check(hipMemcpyAsync(&sum, loss_sum_.get(), sizeof(double), hipMemcpyDeviceToHost, s_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(s_), "gradients");
+    const float loss = static_cast<float>(sum / (static_cast<double>(n) * d_out_));
     if (grads != nullptr) check(hipMemcpy(grads, grads_.get(), np_ * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
     return loss;
   }
@@ -401,13 +409,17 @@ class Net {
       std::shuffle(order.begin(), order.end(), rng);
       check(hipMemcpyAsync(idx.get(), order.data(), order.size() * sizeof(long long), hipMemcpyHostToDevice, s_),
             "hipMemcpyAsync");
-      double sum = 0.0;
+      ensure(batch);
+      check(hipMemsetAsync(loss_sum_.get(), 0, sizeof(double), s_), "hipMemsetAsync");
       for (long long b = 0; b < nb; ++b) {
         gather(idx.get() + b * batch, batch, bx.get(), bt.get());
-        sum += backward(bx.get(), bt.get(), batch);
+        backward(bx.get(), bt.get(), batch);
         adam(lr);
       }
-      last = static_cast<float>(sum / nb);
+      double sum = 0.0;
+      check(hipMemcpyAsync(&sum, loss_sum_.get(), sizeof(double), hipMemcpyDeviceToHost, s_), "hipMemcpyAsync");
+      check(hipStreamSynchronize(s_), "train");
+      last = static_cast<float>(sum / (static_cast<double>(nb) * batch * d_out_));
     }
     check(hipEventRecord(e1, s_), "hipEventRecord");
     check(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -465,7 +477,7 @@ class Net {
       z_.emplace_back(std::size_t(n) * w_[l]);
       dz_.emplace_back(std::size_t(n) * w_[l]);
     }
-    partial_ = Buf<float>(std::size_t(cdiv(static_cast<long long>(n) * kOutPad, 256)));
+    if (loss_sum_.size() == 0) loss_sum_ = Buf<double>(1);
   }
 
   template <bool TA, bool TB, int EPI>
@@ -500,12 +512,14 @@ class Net {
 
   void gather(const long long* idx, int n, float* bx, float* bt);
 
-  float backward(const float* x, const float* t, int n) {
+  // Forward and backward of one batch: gradients into grads_, the squared errors added to
+  // loss_sum_ (cleared by the caller). No host synchronisation.
+  void backward(const float* x, const float* t, int n) {
     forward(x, n);
     const int nl = layers();
     const int blocks = cdiv(static_cast<long long>(n) * kOutPad, 256);
     k_loss<<<blocks, 256, 0, s_>>>(z_.back().get(), t, n, d_out_, kOutPad, norm_.get() + 64, norm_.get() + 96,
-                                  dz_.back().get(), partial_.get());
+                                  dz_.back().get(), loss_sum_.get());
     check(hipGetLastError(), "k_loss");
     check(hipMemsetAsync(grads_.get(), 0, np_ * sizeof(float), s_), "hipMemsetAsync");
     const int splits = std::max(1, std::min(32, n / 512));
@@ -514,7 +528,8 @@ class Net {
       // dW_l = dZ_l^T H_{l-1} (N x K, over the samples), db_l = column sums of dZ_l.
       gemm<true, false, kAtomicF32>(dz_[std::size_t(l)].get(), acts_[std::size_t(l)].get(), N, K, n, N, K,
                                     grads_.get() + woff_[std::size_t(l)], nullptr, nullptr, K, splits);
-      k_colsum<<<cdiv(N, 256), 256, 0, s_>>>(dz_[std::size_t(l)].get(), n, N, grads_.get() + boff_[std::size_t(l)]);
+      k_colsum<<<dim3(cdiv(N, 64), std::min(n, 256)), 64, 0, s_>>>(dz_[std::size_t(l)].get(), n, N,
+                                                                    grads_.get() + boff_[std::size_t(l)]);
       check(hipGetLastError(), "k_colsum");
       if (l > 0) {
         // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}).
@@ -522,13 +537,6 @@ class Net {
                                       dz_[std::size_t(l) - 1].get(), nullptr, K);
       }
     }
-    std::vector<float> part(static_cast<std::size_t>(blocks));
-    check(hipMemcpyAsync(part.data(), partial_.get(), part.size() * sizeof(float), hipMemcpyDeviceToHost, s_),
-          "hipMemcpyAsync");
-    check(hipStreamSynchronize(s_), "backward");
-    double sum = 0.0;
-    for (float v : part) sum += v;
-    return static_cast<float>(sum / (static_cast<double>(n) * d_out_));
   }
 
   void adam(float lr) {
@@ -545,7 +553,8 @@ class Net {
   std::vector<int> w_;
   std::vector<long long> woff_, boff_;
   long long np_ = 0;
-  Buf<float> params_, grads_, m1_, m2_, norm_, partial_;
+  Buf<float> params_, grads_, m1_, m2_, norm_;
+  Buf<double> loss_sum_;
   Buf<bf16> wb_;
   Buf<unsigned char> mask_;
   std::vector<Buf<bf16>> acts_, dz_;

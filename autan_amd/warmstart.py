@@ -43,11 +43,14 @@ def wall_distance(points: np.ndarray, h: float, w: float = 0.2, x0: float = 1.5,
     return np.hypot(p[:, 0] - x, p[:, 1] - f)
 
 
-def features(mesh, spec: sweep.Spec) -> np.ndarray:
-    """n x 7: centroid x, y, z, distance to the bump wall, bump height, Mach number, flow angle."""
+def features(mesh, spec: sweep.Spec, cells: np.ndarray | None = None) -> np.ndarray:
+    """n x 7 (for every cell, or the listed ones): centroid x, y, z, distance to the bump wall, bump
+    height, Mach number, flow angle."""
     x = mesh.centroids()
+    if cells is not None:
+        x = x[cells]
     d = wall_distance(x, spec.h)
-    n = mesh.n_cells
+    n = x.shape[0]
     return np.column_stack([x, d, np.full(n, spec.h), np.full(n, spec.mach), np.full(n, spec.alpha)]).astype(np.float32)
 
 
@@ -67,9 +70,10 @@ def conserved(w: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
 
 
 def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=20000, residual_orders=8.0,
-          policy="fp32+acc64", flux="hllc", batch_cases=64, log=print):
+          policy="fp32+acc64", flux="hllc", batch_cases=64, sample=None, log=print):
     """Solves the cases (from the freestream, or from `states`) in batches of `batch_cases`;
-    returns per case (result, final state, case, seconds of its batch) and the total seconds."""
+    returns per case its result, cell count and wall loads, and the total seconds of the solves.
+    `sample(index, mesh, spec, state)` sees each case's final state before its mesh is dropped."""
     out = []
     total = 0.0
     for b0 in range(0, len(specs), batch_cases):
@@ -95,24 +99,32 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
             wall.append(bs.loads(k, faces, center=(1.5, 0.0, 0.5), p_inf=1.0 / GAMMA))
         bs.close()
         log(f"warm start: solved cases {b0}..{b0 + len(part) - 1} in {sec:.1f} s")
-        for r, u, c, ld in zip(runs, finals, cases, wall):
-            out.append({"result": r, "state": u, "case": c, "loads": ld})
+        for k, (r, u, c, ld) in enumerate(zip(runs, finals, cases, wall)):
+            if sample is not None:
+                sample(b0 + k, c.mesh, part[k], u)
+            out.append({"result": r, "cells": c.mesh.n_cells, "loads": ld})
+        del cases, finals
     return out, total
 
 
-def training_set(solved, per_case: int = 4096, seed: int = 0):
-    """A random sample of each solved case's cells: features and converged primitives."""
-    rng = np.random.default_rng(seed)
-    xs, ts = [], []
-    for item, spec in solved:
-        mesh = item["case"].mesh
+class TrainingSet:
+    """A random sample of each solved case's cells (solve's `sample` callback): features and final
+    primitives."""
+
+    def __init__(self, per_case: int = 4096, seed: int = 0) -> None:
+        self.rng = np.random.default_rng(seed)
+        self.per_case = per_case
+        self.xs: list[np.ndarray] = []
+        self.ts: list[np.ndarray] = []
+
+    def __call__(self, index, mesh, spec, state) -> None:
         n = mesh.n_cells
-        pick = rng.choice(n, size=min(per_case, n), replace=False)
-        f = features(mesh, spec)[pick]
-        w = primitive(item["state"])[pick].astype(np.float32)
-        xs.append(f)
-        ts.append(w)
-    return np.concatenate(xs), np.concatenate(ts)
+        pick = np.sort(self.rng.choice(n, size=min(self.per_case, n), replace=False))
+        self.xs.append(features(mesh, spec, pick))
+        self.ts.append(primitive(state)[pick].astype(np.float32))
+
+    def arrays(self) -> tuple[np.ndarray, np.ndarray]:
+        return np.concatenate(self.xs), np.concatenate(self.ts)
 
 
 def warm_states(net, specs: list[sweep.Spec], policy="fp32+acc64", flux="hllc"):

@@ -320,12 +320,19 @@ def warmstart_main(args) -> int:
     t0 = time.time()
     train_specs = sweep.specs(args.train_cases, seed=11, cells=(2e4, 1e5))
     test_specs = sweep.specs(args.test_cases, seed=12)
-    solved, t_gen = warmstart.solve(train_specs, max_iterations=args.train_iterations, residual_orders=8.0, log=log)
-    x, t = warmstart.training_set(list(zip(solved, train_specs)))
+    collect = warmstart.TrainingSet()
+    _, t_gen = warmstart.solve(train_specs, max_iterations=args.train_iterations, residual_orders=8.0, sample=collect,
+                               log=log)
+    x, t = collect.arrays()
     log(f"training set: {x.shape[0]} cells of {len(train_specs)} cases ({t_gen:.1f} s of solves)")
     net = MLP(7, 256, 4, 5, seed=0)
     net.fit_normalization(x, t)
-    loss, t_train = net.train(x, t, epochs=args.epochs, batch=4096, lr=1e-3, seed=0)
+    t_train = 0.0
+    for ep in range(args.epochs):
+        lr = 1e-3 * (0.5 ** (ep // max(1, args.epochs // 4)))
+        loss, sec = net.train(x, t, epochs=1, batch=4096, lr=lr, seed=ep)
+        t_train += sec
+        log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
     fwd_ms = net.time_forward(1 << 18, 5)
     tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
     log(f"network: loss {loss:.3e} after {args.epochs} epochs ({t_train:.1f} s); forward {tflops:.1f} TFLOP/s")
@@ -344,7 +351,7 @@ def warmstart_main(args) -> int:
         ff = f["loads"][0]
         dw = float(np.linalg.norm(wa["loads"][0] - ff) / max(np.linalg.norm(ff), 1e-300))
         rows.append({
-            "cells": f["case"].mesh.n_cells, "h": s.h, "mach": s.mach, "alpha": s.alpha,
+            "cells": f["cells"], "h": s.h, "mach": s.mach, "alpha": s.alpha,
             "freestream": [f["result"].iterations, ["converged", "budget", "diverged"][f["result"].status]],
             "warm_own_reference": [w["result"].iterations, ["converged", "budget", "diverged"][w["result"].status]],
             "warm_freestream_reference": [wa["result"].iterations,
