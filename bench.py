@@ -396,7 +396,7 @@ def main() -> int:
     ap.add_argument("--train-cases", type=int, default=512, help="warmstart: solved cases the network learns from")
     ap.add_argument("--test-cases", type=int, default=32, help="warmstart: held-out cases")
     ap.add_argument("--train-iterations", type=int, default=4000, help="warmstart: iterations of the training solves")
-    ap.add_argument("--test-orders", type=float, default=3.0,
+    ap.add_argument("--test-orders", type=float, default=2.0,
                     help="warmstart: residual drop the held-out solves run to, from the freestream run's reference")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
