@@ -73,7 +73,8 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
           policy="fp32+acc64", flux="hllc", batch_cases=64, sample=None, log=print):
     """Solves the cases (from the freestream, or from `states`) in batches of `batch_cases`;
     returns per case its result, cell count and wall loads, and the total seconds of the solves.
-    `sample(index, mesh, spec, state)` sees each case's final state before its mesh is dropped."""
+    `sample(index, mesh, spec, state, result)` sees each case's final state before its mesh is
+    dropped."""
     out = []
     total = 0.0
     for b0 in range(0, len(specs), batch_cases):
@@ -101,7 +102,7 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
         log(f"warm start: solved cases {b0}..{b0 + len(part) - 1} in {sec:.1f} s")
         for k, (r, u, c, ld) in enumerate(zip(runs, finals, cases, wall)):
             if sample is not None:
-                sample(b0 + k, c.mesh, part[k], u)
+                sample(b0 + k, c.mesh, part[k], u, r)
             out.append({"result": r, "cells": c.mesh.n_cells, "loads": ld})
         del cases, finals
     return out, total
@@ -116,8 +117,13 @@ class TrainingSet:
         self.per_case = per_case
         self.xs: list[np.ndarray] = []
         self.ts: list[np.ndarray] = []
+        self.skipped = 0
 
-    def __call__(self, index, mesh, spec, state) -> None:
+    def __call__(self, index, mesh, spec, state, result) -> None:
+        # A case that diverged teaches nothing.
+        if result.status == 2 or not np.all(np.isfinite(state)):
+            self.skipped += 1
+            return
         n = mesh.n_cells
         pick = np.sort(self.rng.choice(n, size=min(self.per_case, n), replace=False))
         self.xs.append(features(mesh, spec, pick))
@@ -128,12 +134,17 @@ class TrainingSet:
 
 
 def warm_states(net, specs: list[sweep.Spec], policy="fp32+acc64", flux="hllc"):
-    """The network's initial state for each case (density and pressure kept positive)."""
+    """The network's initial state for each case (density and pressure kept positive), and the
+    seconds spent in the network's inputs and inference (the meshes' set-up excluded: a freestream
+    start needs it too)."""
     out = []
+    seconds = 0.0
     for s in specs:
         c = sweep.build(s, policy, flux)
+        t0 = time.perf_counter()
         w = net.predict(features(c.mesh, s)).astype(np.float64)
         w[:, 0] = np.maximum(w[:, 0], 0.05)
         w[:, 4] = np.maximum(w[:, 4], 0.05 / GAMMA)
         out.append(conserved(w))
-    return out
+        seconds += time.perf_counter() - t0
+    return out, seconds

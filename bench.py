@@ -324,6 +324,7 @@ def warmstart_main(args) -> int:
     _, t_gen = warmstart.solve(train_specs, max_iterations=args.train_iterations, residual_orders=8.0, sample=collect,
                                log=log)
     x, t = collect.arrays()
+    log(f"training set: {collect.skipped} diverged cases left out")
     log(f"training set: {x.shape[0]} cells of {len(train_specs)} cases ({t_gen:.1f} s of solves)")
     net = MLP(7, 256, 4, 5, seed=0)
     net.fit_normalization(x, t)
@@ -338,9 +339,7 @@ def warmstart_main(args) -> int:
     log(f"network: loss {loss:.3e} after {args.epochs} epochs ({t_train:.1f} s); forward {tflops:.1f} TFLOP/s")
     free, t_free = warmstart.solve(test_specs, max_iterations=args.max_iterations, residual_orders=args.test_orders,
                                    log=log)
-    t1 = time.perf_counter()
-    warm_u = warmstart.warm_states(net, test_specs)
-    t_infer = time.perf_counter() - t1
+    warm_u, t_infer = warmstart.warm_states(net, test_specs)
     warm, t_warm = warmstart.solve(test_specs, states=warm_u, max_iterations=args.max_iterations,
                                    residual_orders=args.test_orders, log=log)
     refs = [f["result"].reference for f in free]
@@ -374,7 +373,8 @@ def warmstart_main(args) -> int:
             f"held-out channels (seed 12, 2e4-5e5 cells) solved to {args.test_orders:g} orders below the freestream "
             f"run's first residual",
         },
-        "train": {"cells": int(x.shape[0]), "solve_s": round(t_gen, 1), "train_s": round(t_train, 2),
+        "train": {"cells": int(x.shape[0]), "diverged_left_out": collect.skipped, "solve_s": round(t_gen, 1),
+                  "train_s": round(t_train, 2),
                   "loss": loss, "forward_TFLOPs": round(tflops, 1), "forward_ms_262144": round(fwd_ms, 3)},
         "test": {"freestream_s": round(t_free, 2), "warm_s_own_reference": round(t_warm, 2),
                  "warm_s_freestream_reference": round(t_warm_abs, 2), "inference_s": round(t_infer, 2),
