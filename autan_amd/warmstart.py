@@ -12,7 +12,6 @@ import math
 import time
 
 import numpy as np
-from scipy.spatial import cKDTree
 
 from autan_amd import batch, sweep
 
@@ -20,27 +19,41 @@ GAMMA = 1.4
 
 
 def wall_distance(points: np.ndarray, h: float, w: float = 0.2, x0: float = 1.5, length: float = 3.0,
-                  samples: int = 30001) -> np.ndarray:
+                  samples: int = 65, chunk: int = 65536) -> np.ndarray:
     """Distance from each point to the bump wall y = h exp(-((x - x0) / w)^2), 0 <= x <= length
-    (a curve extruded along z): the nearest of `samples` points on the curve, then a Newton
-    refinement of the foot point along the curve."""
-    xs = np.linspace(0.0, length, samples)
-    ys = h * np.exp(-(((xs - x0) / w) ** 2))
-    tree = cKDTree(np.stack([xs, ys], axis=1))
+    (a curve extruded along z). The foot x* of a point p lies within r = |f(p_x) - p_y| of p_x (the
+    vertical drop is a distance), so `samples` points of [p_x - r, p_x + r] locate its basin and
+    Newton steps on the stationarity condition refine it."""
     p = np.asarray(points, dtype=np.float64)[:, :2]
-    _, idx = tree.query(p)
-    x = xs[idx]
-    for _ in range(3):
-        t = (x - x0) / w
-        f = h * np.exp(-t * t)
-        f1 = -2.0 * t / w * f
-        f2 = (4.0 * t * t - 2.0) / (w * w) * f
-        # minimise (x - px)^2 + (f(x) - py)^2
-        g = (x - p[:, 0]) + (f - p[:, 1]) * f1
-        gg = 1.0 + f1 * f1 + (f - p[:, 1]) * f2
-        x = np.clip(x - g / np.where(np.abs(gg) > 1e-12, gg, 1.0), 0.0, length)
-    f = h * np.exp(-(((x - x0) / w) ** 2))
-    return np.hypot(p[:, 0] - x, p[:, 1] - f)
+    out = np.empty(p.shape[0])
+
+    def f(x):
+        return h * np.exp(-(((x - x0) / w) ** 2))
+
+    s = np.linspace(-1.0, 1.0, samples)
+    for a in range(0, p.shape[0], chunk):
+        px, py = p[a : a + chunk, 0], p[a : a + chunk, 1]
+        r = np.abs(f(px) - py)
+        lo = np.clip(px - r, 0.0, length)
+        hi = np.clip(px + r, 0.0, length)
+        xs = np.clip(px[:, None] + r[:, None] * s[None, :], lo[:, None], hi[:, None])
+        d2 = (xs - px[:, None]) ** 2 + (f(xs) - py[:, None]) ** 2
+        x = xs[np.arange(xs.shape[0]), np.argmin(d2, axis=1)]
+        for _ in range(4):
+            t = (x - x0) / w
+            fx = h * np.exp(-t * t)
+            f1 = -2.0 * t / w * fx
+            f2 = (4.0 * t * t - 2.0) / (w * w) * fx
+            # minimise (x - px)^2 + (f(x) - py)^2
+            g = (x - px) + (fx - py) * f1
+            gg = 1.0 + f1 * f1 + (fx - py) * f2
+            step = g / np.where(np.abs(gg) > 1e-12, gg, 1.0)
+            x_new = np.clip(x - step, lo, hi)
+            # keep a step only where it does not move uphill
+            better = (x_new - px) ** 2 + (f(x_new) - py) ** 2 <= (x - px) ** 2 + (f(x) - py) ** 2
+            x = np.where(better, x_new, x)
+        out[a : a + chunk] = np.hypot(px - x, py - f(x))
+    return out
 
 
 def features(mesh, spec: sweep.Spec, cells: np.ndarray | None = None) -> np.ndarray:
@@ -70,9 +83,10 @@ def conserved(w: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
 
 
 def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=20000, residual_orders=8.0,
-          policy="fp32+acc64", flux="hllc", batch_cases=64, sample=None, log=print):
+          policy="fp32+acc64", flux="hllc", batch_cases=64, sample=None, start_order="first", log=print):
     """Solves the cases (from the freestream, or from `states`) in batches of `batch_cases`;
-    returns per case its result, cell count and wall loads, and the total seconds of the solves.
+    returns per case its result, cell count, wall loads and wall area, and the total seconds of the
+    solves.
     `sample(index, mesh, spec, state, result)` sees each case's final state before its mesh is
     dropped."""
     out = []
@@ -90,20 +104,22 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
         refs = None if references is None else np.asarray(references[b0 : b0 + batch_cases])
         t0 = time.perf_counter()
         runs = bs.steady(max_iterations=max_iterations, residual_orders=residual_orders, scales=scales, references=refs,
-                         history=False)
+                         start_order=start_order, history=False)
         sec = time.perf_counter() - t0
         total += sec
         finals = bs.download()
         wall = []
+        areas = []
         for k, c in enumerate(cases):
             faces = np.nonzero(c.mesh.face_zone == 2)[0] + c.mesh.n_interior
             wall.append(bs.loads(k, faces, center=(1.5, 0.0, 0.5), p_inf=1.0 / GAMMA))
+            areas.append(float(np.sum(c.mesh.face_area[faces])))
         bs.close()
         log(f"warm start: solved cases {b0}..{b0 + len(part) - 1} in {sec:.1f} s")
-        for k, (r, u, c, ld) in enumerate(zip(runs, finals, cases, wall)):
+        for k, (r, u, c, ld, a) in enumerate(zip(runs, finals, cases, wall, areas)):
             if sample is not None:
                 sample(b0 + k, c.mesh, part[k], u, r)
-            out.append({"result": r, "cells": c.mesh.n_cells, "loads": ld})
+            out.append({"result": r, "cells": c.mesh.n_cells, "loads": ld, "wall_area": a})
         del cases, finals
     return out, total
 

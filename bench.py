@@ -330,32 +330,40 @@ def warmstart_main(args) -> int:
     net.fit_normalization(x, t)
     t_train = 0.0
     for ep in range(args.epochs):
-        lr = 1e-3 * (0.5 ** (ep // max(1, args.epochs // 4)))
+        # cosine decay from 1e-3 to 1e-5
+        lr = 1e-5 + (1e-3 - 1e-5) * 0.5 * (1.0 + math.cos(math.pi * ep / args.epochs))
         loss, sec = net.train(x, t, epochs=1, batch=4096, lr=lr, seed=ep)
         t_train += sec
-        log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
+        if ep % 10 == 0 or ep == args.epochs - 1:
+            log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
     fwd_ms = net.time_forward(1 << 18, 5)
     tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
     log(f"network: loss {loss:.3e} after {args.epochs} epochs ({t_train:.1f} s); forward {tflops:.1f} TFLOP/s")
     free, t_free = warmstart.solve(test_specs, max_iterations=args.max_iterations, residual_orders=args.test_orders,
                                    log=log)
     warm_u, t_infer = warmstart.warm_states(net, test_specs)
+    # A network's state is already smooth: its runs start at second order.
     warm, t_warm = warmstart.solve(test_specs, states=warm_u, max_iterations=args.max_iterations,
-                                   residual_orders=args.test_orders, log=log)
+                                   residual_orders=args.test_orders, start_order="second", log=log)
     refs = [f["result"].reference for f in free]
     warm_abs, t_warm_abs = warmstart.solve(test_specs, states=warm_u, references=refs,
-                                           max_iterations=args.max_iterations, residual_orders=args.test_orders, log=log)
+                                           max_iterations=args.max_iterations, residual_orders=args.test_orders,
+                                           start_order="second", log=log)
     rows = []
     for s, f, w, wa in zip(test_specs, free, warm, warm_abs):
         ff = f["loads"][0]
         dw = float(np.linalg.norm(wa["loads"][0] - ff) / max(np.linalg.norm(ff), 1e-300))
+        # force coefficients: the wall force over the freestream dynamic pressure (rho = 1,
+        # p = 1/gamma: q = M^2 / 2) times the wall's area
+        q_area = 0.5 * s.mach * s.mach * f["wall_area"]
+        dc = float(np.linalg.norm(wa["loads"][0] - ff) / q_area)
         rows.append({
             "cells": f["cells"], "h": s.h, "mach": s.mach, "alpha": s.alpha,
             "freestream": [f["result"].iterations, ["converged", "budget", "diverged"][f["result"].status]],
             "warm_own_reference": [w["result"].iterations, ["converged", "budget", "diverged"][w["result"].status]],
             "warm_freestream_reference": [wa["result"].iterations,
                                           ["converged", "budget", "diverged"][wa["result"].status]],
-            "force_free": [float(v) for v in ff], "force_rel_diff": dw,
+            "force_free": [float(v) for v in ff], "force_rel_diff": dw, "force_coeff_diff": dc,
         })
     it_free = sum(r["freestream"][0] for r in rows)
     it_warm = sum(r["warm_freestream_reference"][0] for r in rows)
@@ -379,7 +387,8 @@ def warmstart_main(args) -> int:
         "test": {"freestream_s": round(t_free, 2), "warm_s_own_reference": round(t_warm, 2),
                  "warm_s_freestream_reference": round(t_warm_abs, 2), "inference_s": round(t_infer, 2),
                  "iterations_freestream": it_free, "iterations_warm": it_warm,
-                 "max_force_rel_diff": max(r["force_rel_diff"] for r in rows)},
+                 "max_force_rel_diff": max(r["force_rel_diff"] for r in rows),
+                 "max_force_coeff_diff": max(r["force_coeff_diff"] for r in rows)},
         "cases": rows,
         "total_s": round(time.time() - t0, 1),
     }
@@ -395,10 +404,10 @@ def main() -> int:
     ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump", "sweep", "warmstart"])
     ap.add_argument("--train-cases", type=int, default=512, help="warmstart: solved cases the network learns from")
     ap.add_argument("--test-cases", type=int, default=32, help="warmstart: held-out cases")
-    ap.add_argument("--train-iterations", type=int, default=4000, help="warmstart: iterations of the training solves")
+    ap.add_argument("--train-iterations", type=int, default=8000, help="warmstart: iterations of the training solves")
     ap.add_argument("--test-orders", type=float, default=2.0,
                     help="warmstart: residual drop the held-out solves run to, from the freestream run's reference")
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
     ap.add_argument("--max-iterations", type=int, default=20000, help="sweep: iteration budget per case")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
