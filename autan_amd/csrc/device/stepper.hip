@@ -96,6 +96,9 @@ class StepperBase {
   virtual void upload(const double* u) = 0;
   virtual void download(double* u) = 0;
   virtual void residual(const double* u, double* res) = 0;
+  virtual void upload_device(const double* u, hipStream_t caller) = 0;
+  virtual void download_device(double* u, hipStream_t caller) = 0;
+  virtual void residual_device(const double* u, double* res, hipStream_t caller) = 0;
   virtual void step(double cfl, bool has_dt, double dt, bool local, double* residuals, double* time_step,
                     int* invalid) = 0;
   virtual void evaluate(double* residuals, int* invalid) = 0;
@@ -196,6 +199,8 @@ class Stepper final : public StepperBase {
     (void)hipStreamSynchronize(stream_);
     drop_graphs();
     if (host_small_ != nullptr) (void)hipHostFree(host_small_);
+    if (ev_in_ != nullptr) (void)hipEventDestroy(ev_in_);
+    if (ev_out_ != nullptr) (void)hipEventDestroy(ev_out_);
     if (own_stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -215,6 +220,33 @@ class Stepper final : public StepperBase {
     residual_from_prim();
     res_.download(res, 5 * std::size_t(nc_), stream_);
     check(hipStreamSynchronize(stream_), "residual");
+  }
+
+  // Device-resident variants: the arrays are device memory, the work is ordered after what the
+  // caller's stream holds and before what it is given next (events, no host wait).
+  void upload_device(const double* u, hipStream_t caller) override {
+    join(caller);
+    check(hipMemcpyAsync(u_.get(), u, 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync D2D");
+    release(caller);
+  }
+
+  void download_device(double* u, hipStream_t caller) override {
+    join(caller);
+    check(hipMemcpyAsync(u, u_.get(), 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync D2D");
+    release(caller);
+  }
+
+  void residual_device(const double* u, double* res, hipStream_t caller) override {
+    join(caller);
+    check(hipMemcpyAsync(stage_.get(), u, 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync D2D");
+    launch_prim(stage_.get());
+    residual_from_prim();
+    check(hipMemcpyAsync(res, res_.get(), 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
+          "hipMemcpyAsync D2D");
+    release(caller);
   }
 
   void step(double cfl, bool has_dt, double dt, bool local, double* residuals, double* time_step,
@@ -1358,8 +1390,25 @@ class Stepper final : public StepperBase {
     }
   }
 
+  // The stepper's stream waits for the caller's work so far.
+  void join(hipStream_t caller) {
+    if (caller == stream_) return;
+    if (ev_in_ == nullptr) check(hipEventCreateWithFlags(&ev_in_, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(ev_in_, caller), "hipEventRecord");
+    check(hipStreamWaitEvent(stream_, ev_in_, 0), "hipStreamWaitEvent");
+  }
+
+  // The caller's stream waits for the stepper's work so far.
+  void release(hipStream_t caller) {
+    if (caller == stream_) return;
+    if (ev_out_ == nullptr) check(hipEventCreateWithFlags(&ev_out_, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventRecord(ev_out_, stream_), "hipEventRecord");
+    check(hipStreamWaitEvent(caller, ev_out_, 0), "hipStreamWaitEvent");
+  }
+
   mxeuler_scheme sc_;
   Buffer<double> rb_;
+  hipEvent_t ev_in_ = nullptr, ev_out_ = nullptr;
   double area_scale_ = 0.0;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
@@ -1476,6 +1525,19 @@ int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len) {
 
 int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len) {
   return guarded(err, err_len, [&] { s->impl->residual(u, res); });
+}
+
+int mxeuler_upload_device(mxeuler_stepper* s, const double* u, void* stream, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->upload_device(u, static_cast<hipStream_t>(stream)); });
+}
+
+int mxeuler_download_device(mxeuler_stepper* s, double* u, void* stream, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->download_device(u, static_cast<hipStream_t>(stream)); });
+}
+
+int mxeuler_residual_device(mxeuler_stepper* s, const double* u, double* res, void* stream, char* err,
+                            size_t err_len) {
+  return guarded(err, err_len, [&] { s->impl->residual_device(u, res, static_cast<hipStream_t>(stream)); });
 }
 
 int mxeuler_step(mxeuler_stepper* s, double cfl, int32_t has_dt, double dt, int32_t local, double residuals[5],

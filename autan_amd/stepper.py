@@ -246,6 +246,36 @@ class DeviceStepper:
         self._call(self.lib.mxeuler_residual, _p(u.ctypes.data), _p(res.ctypes.data))
         return res
 
+    def upload_device(self, u, stream: int | None = None) -> None:
+        """The state from device memory (a 5 x n FP64 torch tensor on the stepper's GPU), ordered
+        against `stream` (a raw HIP stream handle; default: torch's current stream)."""
+        self._call(self.lib.mxeuler_upload_device, _p(self._device_state(u).data_ptr()), _p(self._stream(stream)))
+
+    def download_device(self, u, stream: int | None = None) -> None:
+        """The state into device memory (a 5 x n FP64 torch tensor), ordered as upload_device."""
+        self._call(self.lib.mxeuler_download_device, _p(self._device_state(u).data_ptr()), _p(self._stream(stream)))
+
+    def residual_device(self, u, res, stream: int | None = None) -> None:
+        """R(u) into res, both device memory (5 x n FP64 torch tensors), ordered as upload_device."""
+        self._call(self.lib.mxeuler_residual_device, _p(self._device_state(u).data_ptr()),
+                   _p(self._device_state(res).data_ptr()), _p(self._stream(stream)))
+
+    def _device_state(self, t):
+        import torch
+
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                and t.numel() == 5 * self.n):
+            raise ValueError(f"expected a contiguous float64 device tensor of {5 * self.n} values")
+        return t
+
+    @staticmethod
+    def _stream(stream):
+        if stream is not None:
+            return stream
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+
     def step(self, cfl: float = 0.0, dt: float | None = None, local: bool = False) -> Result:
         r = (ctypes.c_double * 5)()
         h = ctypes.c_double()
@@ -334,7 +364,8 @@ class DeviceStepper:
         return report.value.decode()
 
     def tune(self, key: str, value: int) -> None:
-        """Selects a kernel variant (same bits, different speed): "face_stage" 0/1."""
+        """Selects a kernel variant (same bits, different speed): "face_stage" 0/1, "graph" 0/1,
+        "grad_occupancy" and "face_occupancy" 0/8."""
         self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))
 
     def conservation(self) -> dict[str, np.ndarray]:

@@ -99,6 +99,16 @@ int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len);
 /* DeviceStepper::residual: R(u) into res (both 5 x n_cells, FP64); the stepper's state is kept. */
 int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len);
 
+/* Device-resident upload, download and residual (the hipGraph path's variant of the three calls
+ * above): u and res are device memory on the stepper's GPU (5 x n_cells FP64). The work runs on
+ * the stepper's stream after everything `stream` holds at the call, and `stream` receives an
+ * event that orders its later work after the copies (no host wait; `stream` may be the stepper's
+ * own, or NULL for the null stream). */
+int mxeuler_upload_device(mxeuler_stepper* s, const double* u, void* stream, char* err, size_t err_len);
+int mxeuler_download_device(mxeuler_stepper* s, double* u, void* stream, char* err, size_t err_len);
+int mxeuler_residual_device(mxeuler_stepper* s, const double* u, double* res, void* stream, char* err,
+                            size_t err_len);
+
 /* DeviceStepper::step: one SSP-RK3 step of the uploaded state. residuals receives the RMS norms
  * of R(u) at the start of the step (sqrt(sum_c R_kc^2 / N), summed exactly); time_step the step
  * used: dt when has_dt, else min_c CFL V_c / lambda_c (with local, each cell its own).
@@ -144,7 +154,6 @@ int mxeuler_probe_math(int32_t policy, const double* x, const double* y, int64_t
  * best of `repeats` runs of copy, scale, add and triad; gbps and ms receive 4 values each. */
 int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* err, size_t err_len);
 
-/* Version string of the library. */
 /* Euler::boundary_pressure (physics/src/euler.cpp:1187-1223) for the current state: the residual of
  * u is evaluated first, then for each listed boundary face (global face index) the pressure force
  * p_f S + phi_p M^T grad p in FP64, force 3 x n SoA. */
@@ -159,7 +168,9 @@ int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const dou
 /* Chooses between kernel variants that give the same bits (no counterpart in the reference: the
  * autotuner's handle). Keys: "face_stage" (0/1, fp32+acc64): the fused face kernel stages its
  * block's cells in LDS; "graph" (0/1): steps without a fixed time step replay a recorded HIP graph
- * of their launches. An unknown key or a variant this stepper cannot run is an error. */
+ * of their launches; "grad_occupancy" and "face_occupancy" (0 or 8): the gradient and face kernels
+ * built for at least 8 waves per SIMD (0: the compiler's choice). An unknown key or a variant this
+ * stepper cannot run is an error. */
 int mxeuler_tune(mxeuler_stepper* s, const char* key, int32_t value, char* err, size_t err_len);
 /* EulerPhysics::check_conservation (physics/src/euler_physics.cpp:1750-1784; docs/design/euler.md,
  * Conservation check) on the current state: per equation k, exact sums over the cells and faces,
@@ -252,6 +263,7 @@ int mxeuler_batch_steady(mxeuler_stepper* s, const mxsteady_options* options, co
 int mxeuler_autotune(mxeuler_stepper* s, int32_t steps, double cfl, int32_t local, char* report, size_t report_len,
                      char* err, size_t err_len);
 
+/* Version string of the library. */
 const char* mxeuler_version(void);
 
 #ifdef __cplusplus

@@ -127,3 +127,35 @@ def test_conservation_sums_match_oracle(name, policy):
     else:
         for key in ("E", "D", "boundary", "faces"):
             np.testing.assert_array_equal(got[key], want[key])
+
+
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+def test_device_resident_calls_give_the_host_calls_bits(policy):
+    """upload_device / download_device / residual_device on torch tensors of torch's stream, with
+    no host synchronisation in between, give the bits of upload / download / residual."""
+    import torch
+
+    case = cases.tets_case(5)
+    host = DeviceStepper(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
+    host.upload(case.u)
+    host.step(cfl=0.5)
+    want_u = host.download()
+    want_r = host.residual(want_u)
+
+    dev = DeviceStepper(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
+    n = case.mesh.n_cells
+    u = torch.from_numpy(np.asarray(case.u, dtype=np.float64).copy()).cuda()
+    # work queued on torch's stream before the upload must land first
+    u.mul_(2.0).mul_(0.5)
+    dev.upload_device(u)
+    dev.step(cfl=0.5)
+    out = torch.empty(5 * n, dtype=torch.float64, device="cuda")
+    dev.download_device(out)
+    res = torch.empty_like(out)
+    dev.residual_device(out, res)
+    got_u = out.cpu().numpy()
+    got_r = res.cpu().numpy()
+    np.testing.assert_array_equal(got_u, want_u)
+    np.testing.assert_array_equal(got_r, want_r)
+    with pytest.raises(ValueError):
+        dev.upload_device(u.float())
