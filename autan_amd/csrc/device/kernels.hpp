@@ -87,6 +87,11 @@ struct Dev {
   const int* fcase;
   const int* fcbeg;
   const int* cflags;
+  // Launches over the active cases' blocks only: grid position p (after the XCD mapping) takes
+  // logical cell block cblk[p] and face block fblk[p], -1 padding the grid to a multiple of the
+  // XCD count; null: position p is block p.
+  const int* cblk;
+  const int* fblk;
   int ncase;
   int nslot;  // exact-sum slots per case
   // Block-local faces: the faces of the cells of logical block b are the slots sbeg[b] ..
@@ -128,6 +133,18 @@ __device__ __forceinline__ int xcd_block() {
   return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
 }
 __device__ __forceinline__ int xcd_index() { return xcd_block() * kBlock + threadIdx.x; }
+
+// The logical cell block and face block of this workgroup (-1: none).
+template <class C>
+__device__ __forceinline__ int cell_block(const Dev<C>& d) {
+  const int p = xcd_block();
+  return d.cblk != nullptr ? d.cblk[p] : p;
+}
+template <class C>
+__device__ __forceinline__ int face_block(const Dev<C>& d) {
+  const int p = xcd_block();
+  return d.fblk != nullptr ? d.fblk[p] : p;
+}
 
 template <class C>
 __device__ __forceinline__ int block_case(const Dev<C>& d, int blk) {
@@ -234,7 +251,8 @@ __device__ __forceinline__ W5<C> primitive_of(const double* __restrict__ u, int 
 
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_prim(Dev<C> d, const double* __restrict__ u) {
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int c = blk * kBlock + threadIdx.x;
   if (c >= d.nc) return;
   const int cs = block_case(d, blk);
@@ -268,7 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_bstate(Dev<C> d) {
 // ties keeping the first entry. Without the limiter phi = 1 (second order unlimited).
 template <class C, bool LIMITER>
 __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d) {
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int c = blk * kBlock + threadIdx.x;
   if (c >= d.nc) return;
   const int cs = block_case(d, blk);
@@ -356,7 +375,8 @@ template <class C, bool LIMITER, int OCC = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_grad_lds(Dev<C> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   C* tab = reinterpret_cast<C*>(smem);
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int cs = block_case(d, blk);
   if (!case_active(d, cs) || !case_second(d, cs)) return;
   const Params<C>& pa = d.ptab[cs];
@@ -824,7 +844,8 @@ template <class C, bool PRIM, bool CHECK>
 __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, double a, double b, const double* __restrict__ u,
                                                const double* stage_in, double* stage_out, int use_hkey, double hvalue,
                                                int local) {
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int cs = block_case(d, blk);
   if (!case_active(d, cs)) return;
   const Params<C>& pa = d.ptab[cs];
@@ -863,7 +884,8 @@ __global__ __launch_bounds__(kBlock) void k_rk(Dev<C> d, double a, double b, con
 // a second-order evaluation).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_zero_rec(Dev<C> d) {
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int cs = block_case(d, blk);
   if (!case_active(d, cs)) return;
   const int c = blk * kBlock + threadIdx.x;
@@ -877,7 +899,8 @@ __global__ __launch_bounds__(kBlock) void k_zero_rec(Dev<C> d) {
 // DeviceStepper::evaluate's validity checks alone.
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_check(Dev<C> d, const double* __restrict__ u) {
-  const int blk = xcd_block();
+  const int blk = cell_block(d);
+  if (blk < 0) return;
   const int cs = block_case(d, blk);
   if (!case_active(d, cs)) return;
   const Params<C>& pa = d.ptab[cs];
@@ -1005,7 +1028,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
   const int lf_bytes = (kComp * ms * static_cast<int>(sizeof(C)) + 15) & ~15;
   long long* acc = reinterpret_cast<long long*>(smem + lf_bytes);
   __shared__ unsigned long long wave_key[kBlock / kWave];
-  const int blk = xcd_block();
+  const int blk = face_block(d);
+  if (blk < 0) return;
   const int cs = d.fcase != nullptr ? d.fcase[blk] : 0;
   if (!case_active(d, cs)) return;
   const Params<C>& pa = d.ptab[cs];

@@ -192,6 +192,12 @@ class Stepper final : public StepperBase {
     check(hipHostMalloc(reinterpret_cast<void**>(&host_small_), sizeof(Small) * std::size_t(ncase_), hipHostMallocDefault),
           "hipHostMalloc");
     check(hipMemsetAsync(rec_.get(), 0, rec_.size() * sizeof(C), stream_), "hipMemsetAsync");
+    if (ncase_ > 1) {
+      compact_ = true;
+      cblk_ = Buffer<int>(std::size_t(grid(nc_)));
+      fblk_ = Buffer<int>(std::size_t(xcd_grid(face_blocks_)));
+      build_lists();
+    }
     check(hipStreamSynchronize(stream_), "create");
   }
 
@@ -264,7 +270,7 @@ class Stepper final : public StepperBase {
     clear();
     launch_prim(u_.get());
     residual_from_prim(true, 1.0, false);
-    launch("check", [&] { k_check<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), u_.get()); });
+    launch("check", [&] { k_check<C><<<cell_grid(), kBlock, 0, stream_>>>(dev(), u_.get()); });
     finish(residuals, invalid);
   }
 
@@ -500,6 +506,7 @@ class Stepper final : public StepperBase {
       cflags_host_[std::size_t(k)] = flags[k];
     }
     cflags_.upload(cflags_host_.data(), cflags_host_.size(), stream_);
+    if (build_lists()) drop_graphs();
     check(hipStreamSynchronize(stream_), "set_flags");
   }
 
@@ -514,7 +521,7 @@ class Stepper final : public StepperBase {
       clear();
       launch_prim(u_.get());
       residual_from_prim(true, 1.0, false);
-      launch("zero_rec", [&] { k_zero_rec<C><<<grid(nc_), kBlock, 0, stream_>>>(dev()); });
+      launch("zero_rec", [&] { k_zero_rec<C><<<cell_grid(), kBlock, 0, stream_>>>(dev()); });
     } catch (...) {
       eval_mode_ = false;
       throw;
@@ -677,6 +684,8 @@ class Stepper final : public StepperBase {
     d.fcase = ncase_ > 1 ? fcase_.get() : nullptr;
     d.fcbeg = fcbeg_.get();
     d.cflags = ncase_ > 1 ? (eval_mode_ ? eflags_.get() : cflags_.get()) : nullptr;
+    d.cblk = compact_ && !eval_mode_ ? cblk_.get() : nullptr;
+    d.fblk = compact_ && !eval_mode_ ? fblk_.get() : nullptr;
     if (eval_mode_) d.small = small_eval_.get();
     d.ncase = ncase_;
     d.nslot = nslot_;
@@ -880,6 +889,7 @@ class Stepper final : public StepperBase {
     }
     if (ncase_ == 1) fcbeg[1] = xcd_grid(face_blocks_);
     fcbeg_.upload(fcbeg, stream_);
+    fcbeg_host_ = fcbeg;
     if (ncase_ > 1) fcase_.upload(fcase, stream_);
     max_slots_ = max_slots;
     fused_ = fits && faces_lds(true) <= 65536;
@@ -1169,8 +1179,36 @@ class Stepper final : public StepperBase {
     });
   }
 
+  // The active cases' blocks (a batch): cblk_ and fblk_ list them, padded with -1 to the
+  // XCD-aware grid, and the cell and face launches cover only them. True when the lists changed.
+  bool build_lists() {
+    if (!compact_) return false;
+    std::vector<int> cl, fl;
+    for (int k = 0; k < ncase_; ++k) {
+      if (!(cflags_host_[std::size_t(k)] & 1)) continue;
+      for (int b = static_cast<int>(case_first_[std::size_t(k)] / kBlock); b < blocks(case_first_[std::size_t(k) + 1]);
+           ++b) {
+        cl.push_back(b);
+      }
+      for (int b = fcbeg_host_[std::size_t(k)]; b < fcbeg_host_[std::size_t(k) + 1]; ++b) fl.push_back(b);
+    }
+    cl.resize(std::size_t(std::max(kXcds, xcd_grid(static_cast<int>(cl.size())))), -1);
+    fl.resize(std::size_t(std::max(kXcds, xcd_grid(static_cast<int>(fl.size())))), -1);
+    if (cl.size() > cblk_.size() || fl.size() > fblk_.size()) throw std::logic_error("block lists");
+    if (cl == cblk_host_ && fl == fblk_host_) return false;
+    cblk_host_ = std::move(cl);
+    fblk_host_ = std::move(fl);
+    cblk_.upload(cblk_host_.data(), cblk_host_.size(), stream_);
+    fblk_.upload(fblk_host_.data(), fblk_host_.size(), stream_);
+    return true;
+  }
+
+  // Launch grids of the cell and face kernels.
+  int cell_grid() const { return compact_ && !eval_mode_ ? static_cast<int>(cblk_host_.size()) : grid(nc_); }
+  int face_grid() const { return compact_ && !eval_mode_ ? static_cast<int>(fblk_host_.size()) : xcd_grid(face_blocks_); }
+
   void launch_prim(const double* u) {
-    launch("prim", [&] { k_prim<C><<<grid(nc_), kBlock, 0, stream_>>>(dev(), u); });
+    launch("prim", [&] { k_prim<C><<<cell_grid(), kBlock, 0, stream_>>>(dev(), u); });
   }
 
   // Impl::cell_loops and Impl::faces from the primitives already stored. `first` (the step's
@@ -1181,15 +1219,15 @@ class Stepper final : public StepperBase {
     if (second_) {
       const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
       if (staged_ && sc_.limiter && grad_occ_ == 8) {
-        launch("grad_limiter", [&] { k_grad_lds<C, true, 8><<<grid(nc_), kBlock, lds, stream_>>>(d); });
+        launch("grad_limiter", [&] { k_grad_lds<C, true, 8><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (staged_ && sc_.limiter) {
-        launch("grad_limiter", [&] { k_grad_lds<C, true><<<grid(nc_), kBlock, lds, stream_>>>(d); });
+        launch("grad_limiter", [&] { k_grad_lds<C, true><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (staged_) {
-        launch("grad", [&] { k_grad_lds<C, false><<<grid(nc_), kBlock, lds, stream_>>>(d); });
+        launch("grad", [&] { k_grad_lds<C, false><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (sc_.limiter) {
-        launch("grad_limiter", [&] { k_grad<C, true><<<grid(nc_), kBlock, 0, stream_>>>(d); });
+        launch("grad_limiter", [&] { k_grad<C, true><<<cell_grid(), kBlock, 0, stream_>>>(d); });
       } else {
-        launch("grad", [&] { k_grad<C, false><<<grid(nc_), kBlock, 0, stream_>>>(d); });
+        launch("grad", [&] { k_grad<C, false><<<cell_grid(), kBlock, 0, stream_>>>(d); });
       }
     }
     if (fused_) {
@@ -1213,7 +1251,7 @@ class Stepper final : public StepperBase {
 
   template <int FLUX, bool PLANAR, bool SECOND>
   void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
-    const int g = xcd_grid(face_blocks_);
+    const int g = face_grid();
     if constexpr (std::is_same_v<C, float>) {
       if (face_occ_ == 8 && !stage_faces_) {
         if (first) {
@@ -1322,12 +1360,12 @@ class Stepper final : public StepperBase {
       const Dev<C> d = dev();
       if (s < 2) {
         launch("rk_prim", [&] {
-          k_rk<C, true, false><<<grid(nc_), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
+          k_rk<C, true, false><<<cell_grid(), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
                                                                      stage_.get(), use_hkey, dt, use_local ? 1 : 0);
         });
       } else {
         launch("rk_check", [&] {
-          k_rk<C, false, true><<<grid(nc_), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
+          k_rk<C, false, true><<<cell_grid(), kBlock, 0, stream_>>>(d, w[s][0], w[s][1], u_.get(), in,
                                                                      stage_.get(), use_hkey, dt, use_local ? 1 : 0);
         });
       }
@@ -1429,6 +1467,10 @@ class Stepper final : public StepperBase {
   const double* graph_u_[2] = {nullptr, nullptr};
   Buffer<Params<C>> ptab_;
   Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
+  std::vector<int> fcbeg_host_;
+  bool compact_ = false;
+  Buffer<int> cblk_, fblk_;
+  std::vector<int> cblk_host_, fblk_host_;
   Buffer<int2> lr_;
   Buffer<int> bowner_, cfo_, inc_, slen_, sidx_, bkind_;
   Buffer<long long> soff_;
