@@ -246,35 +246,20 @@ class DeviceStepper:
         self._call(self.lib.mxeuler_residual, _p(u.ctypes.data), _p(res.ctypes.data))
         return res
 
-    def upload_device(self, u, stream: int | None = None) -> None:
-        """The state from device memory (a 5 x n FP64 torch tensor on the stepper's GPU), ordered
-        against `stream` (a raw HIP stream handle; default: torch's current stream)."""
-        self._call(self.lib.mxeuler_upload_device, _p(self._device_state(u).data_ptr()), _p(self._stream(stream)))
+    def upload_device(self, u: int, stream: int = 0) -> None:
+        """The state from device memory: `u` the address of 5 x n FP64 values on the stepper's GPU,
+        allocated through the HIP runtime this library links (/opt/rocm's libamdhip64); the copy
+        is ordered after the work `stream` (a raw hipStream_t of that runtime, 0: the null stream)
+        holds, and `stream`'s later work after it."""
+        self._call(self.lib.mxeuler_upload_device, _p(u), _p(stream))
 
-    def download_device(self, u, stream: int | None = None) -> None:
-        """The state into device memory (a 5 x n FP64 torch tensor), ordered as upload_device."""
-        self._call(self.lib.mxeuler_download_device, _p(self._device_state(u).data_ptr()), _p(self._stream(stream)))
+    def download_device(self, u: int, stream: int = 0) -> None:
+        """The state into device memory, ordered as upload_device."""
+        self._call(self.lib.mxeuler_download_device, _p(u), _p(stream))
 
-    def residual_device(self, u, res, stream: int | None = None) -> None:
-        """R(u) into res, both device memory (5 x n FP64 torch tensors), ordered as upload_device."""
-        self._call(self.lib.mxeuler_residual_device, _p(self._device_state(u).data_ptr()),
-                   _p(self._device_state(res).data_ptr()), _p(self._stream(stream)))
-
-    def _device_state(self, t):
-        import torch
-
-        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
-                and t.numel() == 5 * self.n):
-            raise ValueError(f"expected a contiguous float64 device tensor of {5 * self.n} values")
-        return t
-
-    @staticmethod
-    def _stream(stream):
-        if stream is not None:
-            return stream
-        import torch
-
-        return torch.cuda.current_stream().cuda_stream
+    def residual_device(self, u: int, res: int, stream: int = 0) -> None:
+        """R(u) into res, both device addresses of 5 x n FP64 values, ordered as upload_device."""
+        self._call(self.lib.mxeuler_residual_device, _p(u), _p(res), _p(stream))
 
     def step(self, cfl: float = 0.0, dt: float | None = None, local: bool = False) -> Result:
         r = (ctypes.c_double * 5)()

@@ -423,6 +423,7 @@ def main() -> int:
     ap.add_argument("--tune", default="", help="kernel variants, key=value,... (face_stage, graph, grad_occupancy, "
                     "face_occupancy); set after the autotuner")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
+    ap.add_argument("--no-extras", action="store_true", help="bump: skip the steady run, conservation and fp64 check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
@@ -477,7 +478,7 @@ def main() -> int:
             d = np.abs(u_gpu - u_cpu)
             extras["parity"] = {"steps": steps, "max_abs_diff": float(d.max()), "bitwise": bool(np.array_equal(u_gpu, u_cpu))}
     elif args.workload == "bump":
-        extras = bump_extras(dev, w, args)
+        extras = {} if args.no_extras else bump_extras(dev, w, args)
         if not args.no_cpu:
             log("CPU baseline (oracle) on a lattice of half the size per axis")
             nx, ny, nz = size

@@ -131,9 +131,10 @@ def test_conservation_sums_match_oracle(name, policy):
 
 @pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
 def test_device_resident_calls_give_the_host_calls_bits(policy):
-    """upload_device / download_device / residual_device on torch tensors of torch's stream, with
-    no host synchronisation in between, give the bits of upload / download / residual."""
-    import torch
+    """upload_device / download_device / residual_device on device arrays, ordered against a
+    caller's stream with no host synchronisation in between, give the bits of upload / download /
+    residual; the null stream orders them too."""
+    from hipmem import DeviceArray, Stream
 
     case = cases.tets_case(5)
     host = DeviceStepper(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
@@ -143,19 +144,23 @@ def test_device_resident_calls_give_the_host_calls_bits(policy):
     want_r = host.residual(want_u)
 
     dev = DeviceStepper(case.mesh, case.scheme, policy, case.bc_kind, case.bc, case.source)
-    n = case.mesh.n_cells
-    u = torch.from_numpy(np.asarray(case.u, dtype=np.float64).copy()).cuda()
-    # work queued on torch's stream before the upload must land first
-    u.mul_(2.0).mul_(0.5)
-    dev.upload_device(u)
-    dev.step(cfl=0.5)
-    out = torch.empty(5 * n, dtype=torch.float64, device="cuda")
-    dev.download_device(out)
-    res = torch.empty_like(out)
-    dev.residual_device(out, res)
-    got_u = out.cpu().numpy()
-    got_r = res.cpu().numpy()
-    np.testing.assert_array_equal(got_u, want_u)
-    np.testing.assert_array_equal(got_r, want_r)
-    with pytest.raises(ValueError):
-        dev.upload_device(u.float())
+    n5 = 5 * case.mesh.n_cells
+    src = np.ascontiguousarray(case.u, dtype=np.float64).copy()
+    u, out, res, out0 = DeviceArray(n5), DeviceArray(n5), DeviceArray(n5), DeviceArray(n5)
+    s = Stream()
+    try:
+        # the state reaches u on the caller's stream; the upload must wait for it
+        u.copy_from_async(src, s.handle)
+        dev.upload_device(u.ptr, s.handle)
+        dev.step(cfl=0.5)
+        dev.download_device(out.ptr, s.handle)
+        dev.residual_device(out.ptr, res.ptr, s.handle)
+        s.synchronize()
+        np.testing.assert_array_equal(out.numpy(), want_u)
+        np.testing.assert_array_equal(res.numpy(), want_r)
+        dev.download_device(out0.ptr)  # the null stream
+        np.testing.assert_array_equal(out0.numpy(), want_u)
+    finally:
+        for a in (u, out, res, out0):
+            a.free()
+        s.destroy()
