@@ -120,7 +120,7 @@ struct Dev {
 
 constexpr int kNormSlots = 64;  // exact-sum slots of one case
 constexpr int kCaseSlots = 8;   // per case when several cases run together
-constexpr int kMaxSlots = 2 * kBlock;  // faces per face block
+constexpr int kMaxSlots = 2 * kBlock;  // faces per face block in FP64 (twice that in FP32)
 
 // Workgroups are handed to the eight XCDs in turn (workgroup b to XCD b mod 8), each XCD with its
 // own L2. Remapped so that XCD k takes a contiguous run of logical blocks: the cells an XCD works
@@ -1011,7 +1011,7 @@ __global__ void k_clear(Small* small, int ncase, long long* nslots) {
 }
 
 // The faces and boundaries loops fused with their gather, one block of at most 256 consecutive
-// cells and at most kMaxSlots faces per workgroup (two full rounds of its threads). Phase 1:
+// cells and at most 2 kMaxSlots faces (FP32; kMaxSlots in FP64) per workgroup. Phase 1:
 // every face of the block's cells (its interior faces once, the faces it
 // shares with another block here and there too) evaluates its flux into LDS. Phase 2: each cell
 // adds its faces' fluxes in its cell_faces order, interior faces then boundary faces, each set

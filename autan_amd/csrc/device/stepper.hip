@@ -391,10 +391,18 @@ class Stepper final : public StepperBase {
       use_graph_ = value != 0;
       return;
     }
-    if (key == "grad_occupancy" || key == "face_occupancy") {
-      if (value != 0 && value != 8) throw std::invalid_argument(key + ": 0 or 8");
+    if (key == "grad_occupancy") {
+      if (value != 0 && value != 7 && value != 8) throw std::invalid_argument("grad_occupancy: 0, 7 or 8");
       drop_graphs();
-      (key == "grad_occupancy" ? grad_occ_ : face_occ_) = value;
+      grad_occ_ = value;
+      return;
+    }
+    if (key == "face_occupancy") {
+      if (value != 0 && value != 5 && value != 8) throw std::invalid_argument("face_occupancy: 0, 5 or 8");
+      if (value != 0 && !std::is_same_v<C, float>) throw std::invalid_argument("face_occupancy: needs fp32+acc64");
+      if (value == 5 && planar_) throw std::invalid_argument("face_occupancy: 5 is built for non-planar meshes only");
+      drop_graphs();
+      face_occ_ = value;
       return;
     }
     throw std::invalid_argument("unknown tuning key '" + key + "'");
@@ -431,10 +439,12 @@ class Stepper final : public StepperBase {
     double best = std::numeric_limits<double>::infinity();
     int best_stage = 0, best_graph = 0;
     int best_go = 0, best_fo = 0;
-    for (int vi = 0; vi < 16; ++vi) {
-      const int st = vi & 1, gr = (vi >> 1) & 1, go = (vi >> 2) & 1 ? 8 : 0, fo = (vi >> 3) & 1 ? 8 : 0;
+    constexpr int kGradOcc[3] = {0, 7, 8}, kFaceOcc[3] = {0, 5, 8};
+    for (int vi = 0; vi < 36; ++vi) {
+      const int st = vi & 1, gr = (vi >> 1) & 1, go = kGradOcc[(vi >> 2) % 3], fo = kFaceOcc[(vi >> 2) / 3];
       if (st && !can_stage) continue;
       if (fo && (st || !std::is_same_v<C, float>)) continue;
+      if (fo == 5 && planar_) continue;
       {
         tune("face_stage", st);
         tune("graph", gr);
@@ -830,7 +840,7 @@ class Stepper final : public StepperBase {
   }
 
   // The face blocks of the fused face kernel: consecutive cells, at most kBlock of them and at
-  // most kMaxSlots distinct faces, so that phase 1 runs two full rounds of the workgroup. For each
+  // most max_block_faces() distinct faces. For each
   // block its faces ascending, and for every incidence the LDS slot of its face with the cell's
   // side and whether the face is a boundary face.
   void build_slots(const mxeuler_topology& t, const std::vector<int>& cfo) {
@@ -853,7 +863,7 @@ class Stepper final : public StepperBase {
       while (c < cend && c - c0 < kBlock) {
         int fresh = 0;
         for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) fresh += mark[std::size_t(t.cell_faces[k])] != b;
-        if (static_cast<int>(faces.size()) + fresh > kMaxSlots && c > c0) break;
+        if (static_cast<int>(faces.size()) + fresh > max_block_faces() && c > c0) break;
         for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) {
           const int f = t.cell_faces[k];
           if (mark[std::size_t(f)] != b) {
@@ -899,6 +909,10 @@ class Stepper final : public StepperBase {
     islot_.upload(islot, stream_);
     sface_host_ = std::move(sface);
   }
+
+  // Faces per face block: four rounds of the workgroup in FP32 (a block then holds a full 256
+  // cells on hexahedral meshes), two in FP64 (whose fluxes take twice the LDS).
+  static constexpr int max_block_faces() { return std::is_same_v<C, float> ? 2 * kMaxSlots : kMaxSlots; }
 
   // Dynamic LDS of the fused face kernel: the face fluxes (and lambdas), the exact-sum words of
   // the first residual, the staged cells.
@@ -1220,6 +1234,8 @@ class Stepper final : public StepperBase {
       const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
       if (staged_ && sc_.limiter && grad_occ_ == 8) {
         launch("grad_limiter", [&] { k_grad_lds<C, true, 8><<<cell_grid(), kBlock, lds, stream_>>>(d); });
+      } else if (staged_ && sc_.limiter && grad_occ_ == 7) {
+        launch("grad_limiter", [&] { k_grad_lds<C, true, 7><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (staged_ && sc_.limiter) {
         launch("grad_limiter", [&] { k_grad_lds<C, true><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (staged_) {
@@ -1262,6 +1278,18 @@ class Stepper final : public StepperBase {
                                                                                                      nullptr);
         }
         return;
+      }
+      if constexpr (!PLANAR) {
+        if (face_occ_ == 5 && !stage_faces_) {
+          if (first) {
+            k_faces<C, FLUX, PLANAR, SECOND, true, false, 5><<<g, kBlock, faces_lds(true), stream_>>>(
+                d, cfl, local ? 1 : 0, partial_.get());
+          } else {
+            k_faces<C, FLUX, PLANAR, SECOND, false, false, 5><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0,
+                                                                                                       nullptr);
+          }
+          return;
+        }
       }
       if (stage_faces_) {
         if (first) {
@@ -1483,7 +1511,9 @@ class Stepper final : public StepperBase {
   Buffer<int> cbeg_, sbeg_;
   std::vector<int> sface_host_;  // the faces of every slot, until the slot records are built
   bool stage_faces_ = false;     // fp32: the face kernel stages its block's cells in LDS
-  int grad_occ_ = 0, face_occ_ = 0;  // 8: the gradient / face kernel built for 8 waves per SIMD
+  // The gradient / face kernel built for at least this many waves per SIMD (0: the compiler's
+  // choice); gradient 7 or 8, face 5 (non-planar meshes) or 8.
+  int grad_occ_ = 0, face_occ_ = 0;
   Buffer<C> srec_, smom_;
   int face_blocks_ = 0;
   Buffer<int> tbeg_, tent_, lens_;
