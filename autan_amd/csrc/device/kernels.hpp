@@ -836,6 +836,20 @@ __device__ __forceinline__ void flag_least(int* slot, bool flagged, int c) {
   if ((threadIdx.x % kWave) == 0 && v != INT32_MAX) atomicMin(slot, v);
 }
 
+// An SSP-RK3 stage done by the fused face kernel as each cell's residual is gathered (mode 1: then
+// the new stage's primitives into prim_out, a second buffer since other blocks still read this
+// block's cells; mode 2, the last stage: the validity of the new state and of the residual; 0: no
+// update, the residual is stored). The arithmetic is k_rk's.
+template <class C>
+struct RkStage {
+  const double* u;   // the step's state
+  const double* in;  // the stage's input (u on the first stage)
+  double* out;       // the new stage (may be `in`: each cell reads its own values first)
+  C* prim_out;
+  double a, b, hvalue;
+  int use_hkey, local, mode;
+};
+
 // One SSP-RK3 stage (physics/src/device_stepper.cuh:92-103): next = stage - (h / V) R,
 // stage = a u + b next; on the first stage `stage` is u itself. Then either the primitives of the
 // new stage for the next residual, or (last stage) the validity of the new state and residual.
@@ -1020,7 +1034,7 @@ __global__ void k_clear(Small* small, int ncase, long long* nslots) {
 // spectral radius, the cell's time step and the block's least one, and the exact sums of R^2.
 template <class C, int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE, int OCC = 0>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_faces(
-    Dev<C> d, double cfl, int store_dt, unsigned long long* partial) {
+    Dev<C> d, double cfl, int store_dt, unsigned long long* partial, RkStage<C> rk) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int kComp = FIRST ? 7 : 5;
   const int ms = d.max_slots;
@@ -1100,15 +1114,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
     for (int q = 0; q < 5; ++q) {
       const C sq = d.source != nullptr ? d.source[static_cast<long long>(q) * d.nc + c] : C(0);
       r[q] = (static_cast<double>(nv * sq) + a[q]) + b[q];
-      d.res[static_cast<long long>(q) * d.nc + c] = r[q];
+      if (rk.mode == 0) d.res[static_cast<long long>(q) * d.nc + c] = r[q];
     }
   }
+  double dtc = 0.0;
   if constexpr (FIRST) {
     unsigned long long key = ~0ULL;
     if (active) {
       const C lam = static_cast<C>(lam_acc);
       d.lam[c] = lam;
-      const double dtc = (cfl * d.vol64[c]) / static_cast<double>(lam);
+      dtc = (cfl * d.vol64[c]) / static_cast<double>(lam);
       if (store_dt) d.dt[c] = dtc;
       if (!__builtin_isnan(dtc)) key = order_key(dtc);
     }
@@ -1136,6 +1151,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
     for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) {
       const long long v = acc[t];
       if (v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(out + t), static_cast<unsigned long long>(v));
+    }
+  }
+  if (rk.mode != 0) {
+    bool bad_state = false, bad_res = false;
+    if (active) {
+      double un[5];
+      if (c >= pa.cell_end) {
+        // A filler cell of a batch: its state stays.
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const long long i = static_cast<long long>(k) * d.nc + c;
+          un[k] = rk.u[i];
+          rk.out[i] = un[k];
+        }
+      } else {
+        const double h = rk.local ? (FIRST ? dtc : d.dt[c]) : (rk.use_hkey ? key_value(d.small[cs].hkey) : rk.hvalue);
+        const double hv = h / d.vol64[c];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const long long i = static_cast<long long>(k) * d.nc + c;
+          const double next = rk.in[i] - hv * r[k];
+          un[k] = rk.a * rk.u[i] + rk.b * next;
+          rk.out[i] = un[k];
+          bad_res = bad_res || !__builtin_isfinite(r[k]);
+        }
+      }
+      if (rk.mode == 1) {
+        store_w(rk.prim_out, c, primitive_of(un, 1, 0, pa));
+      } else {
+        bad_state = invalid_state(un, 1, 0, pa);
+      }
+    }
+    if (rk.mode == 2) {
+      const int cl = c - static_cast<int>(pa.cell0);
+      flag_least(&d.small[cs].invalid_state, bad_state, cl);
+      flag_least(&d.small[cs].invalid_residual, bad_res, cl);
     }
   }
 }

@@ -181,6 +181,7 @@ class Stepper final : public StepperBase {
     stage_ = Buffer<double>(5 * std::size_t(nc_));
     res_ = Buffer<double>(5 * std::size_t(nc_));
     prim_ = Buffer<C>(8 * std::size_t(nc_));
+    prim2_ = Buffer<C>(8 * std::size_t(nc_));
     prim_b_ = Buffer<C>(8 * std::size_t(std::max(nb_, 1)));
     rec_ = Buffer<C>(20 * std::size_t(nc_));
     fflux_ = Buffer<C>(8 * std::size_t(nf_));
@@ -313,8 +314,9 @@ class Stepper final : public StepperBase {
   }
 
   void arrays(double* prim, double* grad, double* phi, double* prim_b) override {
-    std::vector<C> p(prim_.size()), r(rec_.size()), pb(prim_b_.size());
-    prim_.download(p.data(), p.size(), stream_);
+    const Buffer<C>& pc = pcur_ ? prim2_ : prim_;
+    std::vector<C> p(pc.size()), r(rec_.size()), pb(prim_b_.size());
+    pc.download(p.data(), p.size(), stream_);
     rec_.download(r.data(), r.size(), stream_);
     prim_b_.download(pb.data(), pb.size(), stream_);
     check(hipStreamSynchronize(stream_), "arrays");
@@ -681,7 +683,7 @@ class Stepper final : public StepperBase {
     d.bkind = bkind_.get();
     d.bc = bc_.get();
     d.source = source_.get();
-    d.prim = prim_.get();
+    d.prim = pcur_ ? prim2_.get() : prim_.get();
     d.prim_b = prim_b_.get();
     d.rec = rec_.get();
     d.fflux = fflux_.get();
@@ -1247,7 +1249,9 @@ class Stepper final : public StepperBase {
       }
     }
     if (fused_) {
-      launch(first ? "faces_first" : "faces", [&] { launch_faces(d, first, cfl, local); });
+      static constexpr const char* kNames[2][3] = {{"faces", "faces_rk", "faces_rk_check"},
+                                                   {"faces_first", "faces_first_rk", "faces_first_rk_check"}};
+      launch(kNames[first ? 1 : 0][rk_.mode], [&] { launch_faces(d, first, cfl, local); });
       if (first) {
         launch("step_reduce", [&] {
           const Dev<C> dd = dev();
@@ -1265,48 +1269,30 @@ class Stepper final : public StepperBase {
     }
   }
 
-  template <int FLUX, bool PLANAR, bool SECOND>
-  void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
-    const int g = face_grid();
+  template <int FLUX, bool PLANAR, bool SECOND, bool FIRST, bool STAGE, int OCC>
+  void launch_faces_k(const Dev<C>& d, double cfl, bool local) {
+    k_faces<C, FLUX, PLANAR, SECOND, FIRST, STAGE, OCC><<<face_grid(), kBlock, faces_lds(FIRST, STAGE), stream_>>>(
+        d, cfl, FIRST && local ? 1 : 0, FIRST ? partial_.get() : nullptr, rk_);
+  }
+
+  template <int FLUX, bool PLANAR, bool SECOND, bool FIRST>
+  void launch_faces_v(const Dev<C>& d, double cfl, bool local) {
     if constexpr (std::is_same_v<C, float>) {
-      if (face_occ_ == 8 && !stage_faces_) {
-        if (first) {
-          k_faces<C, FLUX, PLANAR, SECOND, true, false, 8><<<g, kBlock, faces_lds(true), stream_>>>(
-              d, cfl, local ? 1 : 0, partial_.get());
-        } else {
-          k_faces<C, FLUX, PLANAR, SECOND, false, false, 8><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0,
-                                                                                                     nullptr);
-        }
-        return;
-      }
+      if (stage_faces_) return launch_faces_k<FLUX, PLANAR, SECOND, FIRST, true, 0>(d, cfl, local);
+      if (face_occ_ == 8) return launch_faces_k<FLUX, PLANAR, SECOND, FIRST, false, 8>(d, cfl, local);
       if constexpr (!PLANAR) {
-        if (face_occ_ == 5 && !stage_faces_) {
-          if (first) {
-            k_faces<C, FLUX, PLANAR, SECOND, true, false, 5><<<g, kBlock, faces_lds(true), stream_>>>(
-                d, cfl, local ? 1 : 0, partial_.get());
-          } else {
-            k_faces<C, FLUX, PLANAR, SECOND, false, false, 5><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0,
-                                                                                                       nullptr);
-          }
-          return;
-        }
-      }
-      if (stage_faces_) {
-        if (first) {
-          k_faces<C, FLUX, PLANAR, SECOND, true, true><<<g, kBlock, faces_lds(true, true), stream_>>>(
-              d, cfl, local ? 1 : 0, partial_.get());
-        } else {
-          k_faces<C, FLUX, PLANAR, SECOND, false, true><<<g, kBlock, faces_lds(false, true), stream_>>>(d, cfl, 0,
-                                                                                                       nullptr);
-        }
-        return;
+        if (face_occ_ == 5) return launch_faces_k<FLUX, PLANAR, SECOND, FIRST, false, 5>(d, cfl, local);
       }
     }
+    launch_faces_k<FLUX, PLANAR, SECOND, FIRST, false, 0>(d, cfl, local);
+  }
+
+  template <int FLUX, bool PLANAR, bool SECOND>
+  void launch_faces_t(const Dev<C>& d, bool first, double cfl, bool local) {
     if (first) {
-      k_faces<C, FLUX, PLANAR, SECOND, true, false><<<g, kBlock, faces_lds(true), stream_>>>(d, cfl, local ? 1 : 0,
-                                                                                          partial_.get());
+      launch_faces_v<FLUX, PLANAR, SECOND, true>(d, cfl, local);
     } else {
-      k_faces<C, FLUX, PLANAR, SECOND, false, false><<<g, kBlock, faces_lds(false), stream_>>>(d, cfl, 0, nullptr);
+      launch_faces_v<FLUX, PLANAR, SECOND, false>(d, cfl, local);
     }
   }
 
@@ -1376,15 +1362,36 @@ class Stepper final : public StepperBase {
   // CudaStepper::step (physics/src/device_stepper.cuh:313-345) without a host round trip.
   void enqueue_step(double cfl, bool has_dt, double dt, bool local) {
     clear();
+    pcur_ = 0;
     launch_prim(u_.get());
     const bool use_local = !has_dt && local;
-    residual_from_prim(true, has_dt ? 1.0 : cfl, use_local);
     const int use_hkey = has_dt ? 0 : 1;
     // Shu and Osher (1988), eq. 2.18.
     static constexpr double w[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
     for (int s = 0; s < 3; ++s) {
-      if (s > 0) residual_from_prim();
       const double* in = s == 0 ? u_.get() : stage_.get();
+      // The fused face kernel updates the stage as it gathers each cell's residual, unless the
+      // step is the least over all cells and not yet known (the first stage of a global step).
+      const bool fuse = fused_ && (s > 0 || has_dt || use_local);
+      if (fuse) {
+        rk_ = RkStage<C>{u_.get(), in, stage_.get(), pcur_ ? prim_.get() : prim2_.get(), w[s][0], w[s][1], dt,
+                         use_hkey, use_local ? 1 : 0, s < 2 ? 1 : 2};
+      }
+      try {
+        if (s == 0) {
+          residual_from_prim(true, has_dt ? 1.0 : cfl, use_local);
+        } else {
+          residual_from_prim();
+        }
+      } catch (...) {
+        rk_ = RkStage<C>{};
+        throw;
+      }
+      rk_ = RkStage<C>{};
+      if (fuse) {
+        if (s < 2) pcur_ ^= 1;
+        continue;
+      }
       const Dev<C> d = dev();
       if (s < 2) {
         launch("rk_prim", [&] {
@@ -1506,6 +1513,11 @@ class Stepper final : public StepperBase {
   Buffer<double> vol64_;
   Buffer<double> u_, stage_, res_, dt_;
   Buffer<C> prim_, prim_b_, rec_, fflux_, lam_;
+  // A second primitive buffer: a stage updated inside the face kernel writes its primitives there
+  // while other blocks still read the current ones; pcur_ names the current one.
+  Buffer<C> prim2_;
+  int pcur_ = 0;
+  RkStage<C> rk_{};  // the stage the next face launch updates (mode 0: none)
   Buffer<Small> small_;
   Buffer<unsigned long long> partial_;
   Buffer<int> cbeg_, sbeg_;
