@@ -838,8 +838,9 @@ __device__ __forceinline__ void flag_least(int* slot, bool flagged, int c) {
 
 // An SSP-RK3 stage done by the fused face kernel as each cell's residual is gathered (mode 1: then
 // the new stage's primitives into prim_out, a second buffer since other blocks still read this
-// block's cells; mode 2, the last stage: the validity of the new state and of the residual; 0: no
-// update, the residual is stored). The arithmetic is k_rk's.
+// block's cells; mode 2, the last stage: the validity of the new state and of the residual, and
+// the new state's primitives into prim_out unless it is null (the next step's first primitives);
+// 0: no update, the residual is stored). The arithmetic is k_rk's.
 template <class C>
 struct RkStage {
   const double* u;   // the step's state
@@ -1181,6 +1182,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
         store_w(rk.prim_out, c, primitive_of(un, 1, 0, pa));
       } else {
         bad_state = invalid_state(un, 1, 0, pa);
+        if (rk.prim_out != nullptr) store_w(rk.prim_out, c, primitive_of(un, 1, 0, pa));
       }
     }
     if (rk.mode == 2) {

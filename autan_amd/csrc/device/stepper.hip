@@ -212,6 +212,7 @@ class Stepper final : public StepperBase {
   }
 
   void upload(const double* u) override {
+    prim_fresh_ = false;
     u_.upload(u, 5 * std::size_t(nc_), stream_);
     check(hipStreamSynchronize(stream_), "upload");
   }
@@ -222,6 +223,7 @@ class Stepper final : public StepperBase {
   }
 
   void residual(const double* u, double* res) override {
+    prim_fresh_ = false;  // the primitive buffer now holds u's
     stage_.upload(u, 5 * std::size_t(nc_), stream_);
     launch_prim(stage_.get());
     residual_from_prim();
@@ -232,6 +234,7 @@ class Stepper final : public StepperBase {
   // Device-resident variants: the arrays are device memory, the work is ordered after what the
   // caller's stream holds and before what it is given next (events, no host wait).
   void upload_device(const double* u, hipStream_t caller) override {
+    prim_fresh_ = false;
     join(caller);
     check(hipMemcpyAsync(u_.get(), u, 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
           "hipMemcpyAsync D2D");
@@ -246,6 +249,7 @@ class Stepper final : public StepperBase {
   }
 
   void residual_device(const double* u, double* res, hipStream_t caller) override {
+    prim_fresh_ = false;
     join(caller);
     check(hipMemcpyAsync(stage_.get(), u, 5 * std::size_t(nc_) * sizeof(double), hipMemcpyDeviceToDevice, stream_),
           "hipMemcpyAsync D2D");
@@ -433,6 +437,7 @@ class Stepper final : public StepperBase {
       }
     };
     auto restore = [&] {
+      prim_fresh_ = false;
       check(hipMemcpyAsync(u_.get(), saved.get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_),
             "hipMemcpyAsync");
       check(hipStreamSynchronize(stream_), "autotune");
@@ -501,6 +506,8 @@ class Stepper final : public StepperBase {
     }
     for (int k = 0; k < ncase_; ++k) {
       const int old = cflags_host_[std::size_t(k)];
+      // A case stepped again: its primitives were left behind while it was frozen.
+      if (!(old & 1) && (flags[k] & 1)) prim_fresh_ = false;
       const std::size_t first = std::size_t(case_first_[std::size_t(k)]);
       const std::size_t range = std::size_t(case_first_[std::size_t(k) + 1]) - first;
       if ((old & 1) && !(flags[k] & 1)) {
@@ -1361,9 +1368,13 @@ class Stepper final : public StepperBase {
 
   // CudaStepper::step (physics/src/device_stepper.cuh:313-345) without a host round trip.
   void enqueue_step(double cfl, bool has_dt, double dt, bool local) {
+    const bool fresh = prim_fresh_;
+    prim_fresh_ = false;
     clear();
-    pcur_ = 0;
-    launch_prim(u_.get());
+    if (!fresh) {
+      pcur_ = 0;
+      launch_prim(u_.get());
+    }
     const bool use_local = !has_dt && local;
     const int use_hkey = has_dt ? 0 : 1;
     // Shu and Osher (1988), eq. 2.18.
@@ -1374,6 +1385,7 @@ class Stepper final : public StepperBase {
       // step is the least over all cells and not yet known (the first stage of a global step).
       const bool fuse = fused_ && (s > 0 || has_dt || use_local);
       if (fuse) {
+        // The last stage writes the new state's primitives too: the next step starts from them.
         rk_ = RkStage<C>{u_.get(), in, stage_.get(), pcur_ ? prim_.get() : prim2_.get(), w[s][0], w[s][1], dt,
                          use_hkey, use_local ? 1 : 0, s < 2 ? 1 : 2};
       }
@@ -1389,7 +1401,8 @@ class Stepper final : public StepperBase {
       }
       rk_ = RkStage<C>{};
       if (fuse) {
-        if (s < 2) pcur_ ^= 1;
+        pcur_ ^= 1;
+        if (s == 2) prim_fresh_ = true;
         continue;
       }
       const Dev<C> d = dev();
@@ -1408,43 +1421,63 @@ class Stepper final : public StepperBase {
     std::swap(u_, stage_);
   }
 
-  // A step's launches recorded once as a HIP graph and replayed: two graphs, one for each way the
-  // state and stage buffers alternate. Recorded again when the CFL number, the time stepping or a
-  // kernel variant changes, or a buffer the kernels read is replaced.
+  // A step's launches recorded once as a HIP graph and replayed. A graph is keyed by where the step
+  // starts: the state buffer, the primitive buffer it reads and whether the last step left them
+  // (the first step after an upload computes them), so steady stepping alternates between two
+  // graphs. Recorded again when the CFL number, the time stepping or a kernel variant changes, or
+  // a buffer the kernels read is replaced.
   void graph_step(double cfl, bool local) {
     if (!graphs_valid_ || graph_cfl_ != cfl || graph_local_ != local) {
       drop_graphs();
-      for (int p = 0; p < 2; ++p) {
-        graph_u_[p] = u_.get();
-        check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        hipGraph_t g = nullptr;
-        try {
-          enqueue_step(cfl, false, 0.0, local);
-        } catch (...) {
-          (void)hipStreamEndCapture(stream_, &g);
-          if (g != nullptr) (void)hipGraphDestroy(g);
-          throw;
-        }
-        check(hipStreamEndCapture(stream_, &g), "hipStreamEndCapture");
-        const hipError_t e = hipGraphInstantiate(&graph_exec_[p], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        check(e, "hipGraphInstantiate");
-      }
       graphs_valid_ = true;
       graph_cfl_ = cfl;
       graph_local_ = local;
     }
-    const int p = graph_u_[0] == u_.get() ? 0 : 1;
-    if (graph_u_[p] != u_.get()) throw std::logic_error("graph_step: state buffer not recorded");
-    check(hipGraphLaunch(graph_exec_[p], stream_), "hipGraphLaunch");
+    const GraphEntry* g = nullptr;
+    for (const auto& e : graphs_) {
+      if (e.u == u_.get() && e.pcur == pcur_ && e.fresh == prim_fresh_) g = &e;
+    }
+    if (g == nullptr) {
+      if (graphs_.size() >= 4) {
+        drop_graphs();
+        graphs_valid_ = true;
+      }
+      GraphEntry e{u_.get(), pcur_, prim_fresh_, nullptr, 0, false};
+      // Recorded from the current state; the host's view of the buffers is put back afterwards,
+      // since the recording ran nothing.
+      check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      hipGraph_t gr = nullptr;
+      try {
+        enqueue_step(cfl, false, 0.0, local);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &gr);
+        if (gr != nullptr) (void)hipGraphDestroy(gr);
+        prim_fresh_ = false;
+        throw;
+      }
+      std::swap(u_, stage_);
+      e.end_pcur = pcur_;
+      e.end_fresh = prim_fresh_;
+      pcur_ = e.pcur;
+      prim_fresh_ = e.fresh;
+      check(hipStreamEndCapture(stream_, &gr), "hipStreamEndCapture");
+      const hipError_t err = hipGraphInstantiate(&e.exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      check(err, "hipGraphInstantiate");
+      graphs_.push_back(e);
+      g = &graphs_.back();
+    }
+    check(hipGraphLaunch(g->exec, stream_), "hipGraphLaunch");
+    pcur_ = g->end_pcur;
+    prim_fresh_ = g->end_fresh;
     std::swap(u_, stage_);
   }
 
   void drop_graphs() {
-    for (auto& e : graph_exec_) {
-      if (e != nullptr) (void)hipGraphExecDestroy(e);
-      e = nullptr;
+    for (auto& e : graphs_) {
+      if (e.exec != nullptr) (void)hipGraphExecDestroy(e.exec);
     }
+    graphs_.clear();
     graphs_valid_ = false;
   }
 
@@ -1498,8 +1531,18 @@ class Stepper final : public StepperBase {
   bool eval_mode_ = false;
   bool use_graph_ = false, graphs_valid_ = false, graph_local_ = false;
   double graph_cfl_ = 0.0;
-  hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
-  const double* graph_u_[2] = {nullptr, nullptr};
+  struct GraphEntry {
+    const double* u;  // the state buffer the step starts from
+    int pcur;         // the primitive buffer it reads
+    bool fresh;       // whether those primitives were left by the step before
+    hipGraphExec_t exec;
+    int end_pcur;
+    bool end_fresh;
+  };
+  std::vector<GraphEntry> graphs_;
+  // The primitive buffer pcur_ holds the primitives of the state u_ (the last step's last stage
+  // wrote them), so the next step needs no k_prim.
+  bool prim_fresh_ = false;
   Buffer<Params<C>> ptab_;
   Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
   std::vector<int> fcbeg_host_;

@@ -98,12 +98,12 @@ def kernel_bytes(c: Counts, policy: str = "fp32+acc64", local: bool = False) -> 
     out["faces"] = out["flux"] - F * 5 * s + out["gather"] - F * 5 * s
     out["faces_first"] = out["faces"] + N * (r + s + (r if local else 0))
     # The face kernel that also updates the stage (no residual written): u and the stage read, the
-    # new stage written, the FP64 volume and local dt read; then the new stage's primitives, or
-    # (the last stage) nothing more. On the first stage the stage read is u itself and the dt is
-    # the kernel's own.
+    # new stage written, the FP64 volume and local dt read; then the new stage's primitives (on the
+    # last stage, the new state's: the next step's first). On the first stage the stage read is u
+    # itself and the dt is the kernel's own.
     rk_io = N * (5 * r + 5 * r + 5 * r + r + (r if local else 0))
     out["faces_rk"] = out["faces"] - N * 5 * r + rk_io + N * 5 * s
-    out["faces_rk_check"] = out["faces"] - N * 5 * r + rk_io
+    out["faces_rk_check"] = out["faces"] - N * 5 * r + rk_io + N * 5 * s  # and the next step's primitives
     out["faces_first_rk"] = out["faces_first"] - N * 5 * r + N * (5 * r + 5 * r) + N * 5 * s
     return out
 
