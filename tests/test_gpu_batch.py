@@ -71,6 +71,30 @@ def test_inactive_cases_keep_their_state():
 
 
 @pytest.mark.parametrize("graph", [0, 1])
+def test_a_case_stepped_again_continues_from_its_frozen_state(graph):
+    """A case frozen over an odd number of steps (the buffers the others step through change
+    parity) and then stepped again gives the bits of the same case stepped alone without the pause."""
+    cs = make_cases()
+    b = batch.BatchStepper(cs, "fp32+acc64")
+    b.tune("graph", graph)
+    b.upload()
+    alone = solo(cs[1], "fp32+acc64")
+    alone.upload(cs[1].u)
+    b.step(cfl=0.8, local=True)
+    alone.step(cfl=0.8, local=True)
+    b.set_flags([3, 2, 3])
+    for _ in range(3):
+        b.step(cfl=0.8, local=True)
+    b.set_flags([3, 3, 3])
+    for _ in range(2):
+        r, _, inv = b.step(cfl=0.8, local=True)
+        s = alone.step(cfl=0.8, local=True)
+        np.testing.assert_array_equal(r[1], s.residuals)
+        assert inv[1] is None
+    np.testing.assert_array_equal(b.download()[1], alone.download())
+
+
+@pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("start", ["second", "first"])
 def test_batch_steady_matches_each_solo_run(start, graph):
     cs = make_cases()
