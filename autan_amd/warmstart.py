@@ -164,3 +164,47 @@ def warm_states(net, specs: list[sweep.Spec], policy="fp32+acc64", flux="hllc"):
         out.append(conserved(w))
         seconds += time.perf_counter() - t0
     return out, seconds
+
+
+def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_case: int = 4096, batch: int = 4096,
+                  lr: tuple[float, float] = (1e-3, 1e-5), seed: int = 0, log=print):
+    """The network trained on the training cases' states: each case run `iterations` steady
+    iterations from the freestream (cases that diverge left out), `per_case` random cells of each
+    kept, then `epochs` of Adam with a cosine learning rate from lr[0] to lr[1]. Returns the network
+    and what it took."""
+    from autan_amd.mlp import MLP
+
+    collect = TrainingSet(per_case=per_case, seed=seed)
+    _, t_gen = solve(specs, max_iterations=iterations, residual_orders=8.0, sample=collect, log=log)
+    x, t = collect.arrays()
+    log(f"training set: {x.shape[0]} cells of {len(specs) - collect.skipped} cases ({collect.skipped} diverged, "
+        f"left out; {t_gen:.1f} s of solves)")
+    net = MLP(7, 256, 4, 5, seed=seed)
+    net.fit_normalization(x, t)
+    t_train = 0.0
+    first = loss = float("nan")
+    for ep in range(epochs):
+        rate = lr[1] + (lr[0] - lr[1]) * 0.5 * (1.0 + math.cos(math.pi * ep / epochs))
+        loss, sec = net.train(x, t, epochs=1, batch=batch, lr=rate, seed=seed + ep)
+        if ep == 0:
+            first = loss
+        t_train += sec
+        if ep % 10 == 0 or ep == epochs - 1:
+            log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
+    return net, {"cells": int(x.shape[0]), "diverged_left_out": collect.skipped, "solve_s": round(t_gen, 1),
+                 "train_s": round(t_train, 2), "first_loss": first, "loss": loss}
+
+
+def compare(net, specs: list[sweep.Spec], orders: float, max_iterations: int, log=print) -> dict:
+    """The held-out cases solved from the freestream (first-order start) and from the network's state
+    (second order from the first step), the latter both against its own first residual and against
+    the freestream run's (the absolute threshold a warm start from another run is held to)."""
+    free, t_free = solve(specs, max_iterations=max_iterations, residual_orders=orders, log=log)
+    warm_u, t_infer = warm_states(net, specs)
+    warm, t_warm = solve(specs, states=warm_u, max_iterations=max_iterations, residual_orders=orders,
+                         start_order="second", log=log)
+    refs = [f["result"].reference for f in free]
+    warm_abs, t_warm_abs = solve(specs, states=warm_u, references=refs, max_iterations=max_iterations,
+                                 residual_orders=orders, start_order="second", log=log)
+    return {"free": free, "warm": warm, "warm_abs": warm_abs, "free_s": t_free, "warm_s": t_warm,
+            "warm_abs_s": t_warm_abs, "inference_s": t_infer}

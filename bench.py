@@ -315,40 +315,18 @@ def warmstart_main(args) -> int:
     import numpy as np
 
     from autan_amd import sweep, warmstart
-    from autan_amd.mlp import MLP
 
     t0 = time.time()
     train_specs = sweep.specs(args.train_cases, seed=11, cells=(2e4, 1e5))
     test_specs = sweep.specs(args.test_cases, seed=12)
-    collect = warmstart.TrainingSet()
-    _, t_gen = warmstart.solve(train_specs, max_iterations=args.train_iterations, residual_orders=8.0, sample=collect,
-                               log=log)
-    x, t = collect.arrays()
-    log(f"training set: {collect.skipped} diverged cases left out")
-    log(f"training set: {x.shape[0]} cells of {len(train_specs)} cases ({t_gen:.1f} s of solves)")
-    net = MLP(7, 256, 4, 5, seed=0)
-    net.fit_normalization(x, t)
-    t_train = 0.0
-    for ep in range(args.epochs):
-        # cosine decay from 1e-3 to 1e-5
-        lr = 1e-5 + (1e-3 - 1e-5) * 0.5 * (1.0 + math.cos(math.pi * ep / args.epochs))
-        loss, sec = net.train(x, t, epochs=1, batch=4096, lr=lr, seed=ep)
-        t_train += sec
-        if ep % 10 == 0 or ep == args.epochs - 1:
-            log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
+    net, train = warmstart.train_network(train_specs, args.train_iterations, args.epochs, log=log)
     fwd_ms = net.time_forward(1 << 18, 5)
     tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
-    log(f"network: loss {loss:.3e} after {args.epochs} epochs ({t_train:.1f} s); forward {tflops:.1f} TFLOP/s")
-    free, t_free = warmstart.solve(test_specs, max_iterations=args.max_iterations, residual_orders=args.test_orders,
-                                   log=log)
-    warm_u, t_infer = warmstart.warm_states(net, test_specs)
-    # A network's state is already smooth: its runs start at second order.
-    warm, t_warm = warmstart.solve(test_specs, states=warm_u, max_iterations=args.max_iterations,
-                                   residual_orders=args.test_orders, start_order="second", log=log)
-    refs = [f["result"].reference for f in free]
-    warm_abs, t_warm_abs = warmstart.solve(test_specs, states=warm_u, references=refs,
-                                           max_iterations=args.max_iterations, residual_orders=args.test_orders,
-                                           start_order="second", log=log)
+    log(f"network: loss {train['loss']:.3e} after {args.epochs} epochs ({train['train_s']:.1f} s); forward "
+        f"{tflops:.1f} TFLOP/s")
+    cmp = warmstart.compare(net, test_specs, args.test_orders, args.max_iterations, log=log)
+    free, warm, warm_abs = cmp["free"], cmp["warm"], cmp["warm_abs"]
+    t_free, t_warm, t_warm_abs, t_infer = cmp["free_s"], cmp["warm_s"], cmp["warm_abs_s"], cmp["inference_s"]
     rows = []
     for s, f, w, wa in zip(test_specs, free, warm, warm_abs):
         ff = f["loads"][0]
@@ -381,9 +359,7 @@ def warmstart_main(args) -> int:
             f"held-out channels (seed 12, 2e4-5e5 cells) solved to {args.test_orders:g} orders below the freestream "
             f"run's first residual",
         },
-        "train": {"cells": int(x.shape[0]), "diverged_left_out": collect.skipped, "solve_s": round(t_gen, 1),
-                  "train_s": round(t_train, 2),
-                  "loss": loss, "forward_TFLOPs": round(tflops, 1), "forward_ms_262144": round(fwd_ms, 3)},
+        "train": {**train, "forward_TFLOPs": round(tflops, 1), "forward_ms_262144": round(fwd_ms, 3)},
         "test": {"freestream_s": round(t_free, 2), "warm_s_own_reference": round(t_warm, 2),
                  "warm_s_freestream_reference": round(t_warm_abs, 2), "inference_s": round(t_infer, 2),
                  "iterations_freestream": it_free, "iterations_warm": it_warm,
