@@ -33,9 +33,12 @@ inline void check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw std::runtime_error(std::string("E1101: ") + what + ": " + hipGetErrorString(e));
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative through the logistic s(z) = 1 / (1 + e^-z), with the hardware exponential
+// and reciprocal (about 1 ulp each; the activations are rounded to BF16 right after).
+__device__ __forceinline__ float logistic(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * logistic(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = logistic(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -232,6 +235,106 @@ __global__ void k_out(const float* __restrict__ y, int n, int d, int width, cons
   out[i] = y[r * width + c] * scale[c] + mean[c];
 }
 
+// The layers of a network as k_infer reads them: weight and bias offsets into the parameter arrays.
+constexpr int kMaxLayers = 18;
+struct LayerTable {
+  long long w[kMaxLayers], b[kMaxLayers];
+  int nl;
+};
+
+// The whole forward of 64 samples per workgroup (inference): the normalised inputs, then every
+// layer's activations, stay in one 64 x (hidden + 8) BF16 LDS tile; the weights stream from L2.
+// Each of the four waves computes 16 CT output columns of the 64 rows as 4 x CT matrix-core tiles,
+// K in steps of 32 in the order of k_gemm, and the epilogues are k_gemm's (bias + SiLU, then bias
+// and the output scaling of k_out): the outputs are the layer-by-layer path's bits, without its
+// HBM round trip of every layer's activations.
+template <int CT>
+__global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x, int n, int d_in, int d_out,
+                                                    const float* __restrict__ norm, const bf16* __restrict__ wb,
+                                                    const float* __restrict__ params, LayerTable L,
+                                                    float* __restrict__ y) {
+  constexpr int H = 64 * CT, ld = H + kPad, kIn = 32, kOut = 16;
+  __shared__ __align__(16) bf16 act[64 * ld];
+  const int m0 = blockIdx.x * 64;
+  const int t = threadIdx.x, lane = t % 64, wave = t / 64;
+  const int row = lane % 16, kq = 8 * (lane / 16);
+  for (int i = t; i < 64 * kIn; i += kThreads) {
+    const int r = i / kIn, c = i % kIn;
+    const long long m = m0 + r;
+    act[r * ld + c] = static_cast<bf16>(m < n && c < d_in ? (x[m * d_in + c] - norm[c]) / norm[32 + c] : 0.0f);
+  }
+  for (int l = 0; l < L.nl; ++l) {
+    const int K = l == 0 ? kIn : H;
+    const bf16* W = wb + L.w[l];
+    const float* bias = params + L.b[l];
+    __syncthreads();  // the layer's input is complete
+    if (l < L.nl - 1) {
+      f32x4 acc[4][CT];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < CT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+      const int nb = wave * 16 * CT;
+      const bf16* wrow[CT];
+#pragma unroll
+      for (int j = 0; j < CT; ++j) wrow[j] = W + static_cast<long long>(nb + 16 * j + row) * K + kq;
+      // The next K step's weights are in flight while this step's products run.
+      bf16x8 b_next[CT];
+#pragma unroll
+      for (int j = 0; j < CT; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wrow[j]);
+      for (int k0 = 0; k0 < K; k0 += kTileK) {
+        bf16x8 a[4], b[CT];
+#pragma unroll
+        for (int j = 0; j < CT; ++j) b[j] = b_next[j];
+        if (k0 + kTileK < K) {
+#pragma unroll
+          for (int j = 0; j < CT; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + k0 + kTileK);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&act[(16 * i + row) * ld + k0 + kq]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+          for (int j = 0; j < CT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // every wave has read the layer's input: the tile takes its output
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < CT; ++j) {
+          const int nn = nb + 16 * j + lane % 16;
+          const float bn = bias[nn];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int mr = 16 * i + 4 * (lane / 16) + r;
+            act[mr * ld + nn] = static_cast<bf16>(silu(acc[i][j][r] + bn));
+          }
+        }
+      }
+    } else {
+      // The padded output layer (16 columns): wave w takes rows 16 w .. 16 w + 15.
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int k0 = 0; k0 < K; k0 += kTileK) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[(16 * wave + row) * ld + k0 + kq]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + static_cast<long long>(row) * K + k0 + kq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+      }
+      const int nn = lane % 16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long long m = m0 + 16 * wave + 4 * (lane / 16) + r;
+        if (m < n && nn < d_out) {
+          const float z = acc[r] + bias[nn];
+          y[m * d_out + nn] = z * norm[96 + nn] + norm[64 + nn];
+        }
+      }
+      (void)kOut;
+    }
+  }
+}
+
 template <class T>
 class Buf {
  public:
@@ -355,10 +458,7 @@ class Net {
       const int m = static_cast<int>(std::min(chunk, n - at));
       check(hipMemcpyAsync(xd.get(), x + at * d_in_, std::size_t(m) * d_in_ * sizeof(float), hipMemcpyHostToDevice, s_),
             "hipMemcpyAsync");
-      forward(xd.get(), m);
-      k_out<<<cdiv(static_cast<long long>(m) * d_out_, 256), 256, 0, s_>>>(z_.back().get(), m, d_out_, kOutPad,
-                                                                           norm_.get() + 64, norm_.get() + 96, yd.get());
-      check(hipGetLastError(), "k_out");
+      infer(xd.get(), m, yd.get());
       check(hipMemcpyAsync(y + at * d_out_, yd.get(), std::size_t(m) * d_out_ * sizeof(float), hipMemcpyDeviceToHost, s_),
             "hipMemcpyAsync");
       check(hipStreamSynchronize(s_), "predict");
@@ -436,17 +536,46 @@ class Net {
     check(hipStreamSynchronize(s_), "adam");
   }
 
-  // Milliseconds of one forward of n samples (device-resident inputs, HIP events) and the
-  // multiply-add count per sample of the real layers.
+  // Inference: n device-resident samples to n x outputs in physical units, by the fused forward
+  // (hidden widths up to 256) or, with fused_ off or wider layers, the training path's
+  // layer-by-layer products; the same bits either way.
+  void infer(const float* x, int n, float* y) {
+    const int h = w_[1];
+    if (fused_ && h <= 256 && w_.size() - 1 <= std::size_t(kMaxLayers)) {
+      LayerTable L{};
+      L.nl = layers();
+      for (int l = 0; l < L.nl; ++l) {
+        L.w[l] = woff_[std::size_t(l)];
+        L.b[l] = boff_[std::size_t(l)];
+      }
+      const int g = cdiv(n, 64);
+      switch (h / 64) {
+        case 1: k_infer<1><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
+        case 2: k_infer<2><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
+        case 3: k_infer<3><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
+        default: k_infer<4><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
+      }
+      check(hipGetLastError(), "k_infer");
+      return;
+    }
+    forward(x, n);
+    k_out<<<cdiv(static_cast<long long>(n) * d_out_, 256), 256, 0, s_>>>(z_.back().get(), n, d_out_, kOutPad,
+                                                                         norm_.get() + 64, norm_.get() + 96, y);
+    check(hipGetLastError(), "k_out");
+  }
+
+  void set_fused(bool on) { fused_ = on; }
+
+  // Milliseconds of one inference of n samples (device-resident inputs and outputs, HIP events).
   double time_forward(int n, int repeats) {
-    Buf<float> xd(std::size_t(n) * d_in_);
+    Buf<float> xd(std::size_t(n) * d_in_), yd(std::size_t(n) * d_out_);
     check(hipMemsetAsync(xd.get(), 0, std::size_t(n) * d_in_ * sizeof(float), s_), "hipMemset");
-    forward(xd.get(), n);
+    infer(xd.get(), n, yd.get());
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
     check(hipEventRecord(e0, s_), "hipEventRecord");
-    for (int r = 0; r < repeats; ++r) forward(xd.get(), n);
+    for (int r = 0; r < repeats; ++r) infer(xd.get(), n, yd.get());
     check(hipEventRecord(e1, s_), "hipEventRecord");
     check(hipEventSynchronize(e1), "hipEventSynchronize");
     float ms = 0.0f;
@@ -561,6 +690,7 @@ class Net {
   std::vector<Buf<float>> z_;
   int cap_ = 0;
   long long step_ = 0;
+  bool fused_ = true;
   Buf<float> data_x_, data_t_;
   long long n_data_ = 0;
 };
@@ -661,7 +791,14 @@ int mxmlp_train(mxmlp* h, int32_t epochs, int32_t batch, float lr, uint64_t seed
 }
 
 int mxmlp_time_forward(mxmlp* h, int32_t n, int32_t repeats, double* ms, char* err, size_t err_len) {
-  return mxm::guarded(err, err_len, [&] { *ms = h->net->time_forward(n, repeats); });
+  return mxm::guarded(err, err_len, [&] {
+    if (n < 1 || repeats < 1) throw std::invalid_argument("n >= 1 and repeats >= 1");
+    *ms = h->net->time_forward(n, repeats);
+  });
+}
+
+int mxmlp_set_fused(mxmlp* h, int32_t on, char* err, size_t err_len) {
+  return mxm::guarded(err, err_len, [&] { h->net->set_fused(on != 0); });
 }
 
 }  // extern "C"

@@ -115,7 +115,12 @@ class MLP:
                    ctypes.c_uint64(seed), ctypes.byref(loss), ctypes.byref(sec))
         return loss.value, sec.value
 
+    def set_fused(self, on: bool) -> None:
+        """Inference by the fused all-layer kernel (default) or the layer-by-layer products; same bits."""
+        self._call(self.lib.mxmlp_set_fused, ctypes.c_int32(1 if on else 0))
+
     def time_forward(self, n: int, repeats: int = 10) -> float:
+        """Milliseconds of one inference of n device-resident samples."""
         ms = ctypes.c_double()
         self._call(self.lib.mxmlp_time_forward, ctypes.c_int32(n), ctypes.c_int32(repeats), ctypes.byref(ms))
         return ms.value

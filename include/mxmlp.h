@@ -39,8 +39,12 @@ int mxmlp_adam_step(mxmlp* h, float lr, char* err, size_t err_len);
 int mxmlp_set_data(mxmlp* h, const float* x, const float* t, int64_t n, char* err, size_t err_len);
 int mxmlp_train(mxmlp* h, int32_t epochs, int32_t batch, float lr, uint64_t seed, float* loss, double* seconds,
                 char* err, size_t err_len);
-/* Milliseconds of one forward of n device-resident samples (average of repeats). */
+/* Milliseconds of one inference of n device-resident samples (average of repeats). */
 int mxmlp_time_forward(mxmlp* h, int32_t n, int32_t repeats, double* ms, char* err, size_t err_len);
+/* Inference (mxmlp_predict, mxmlp_time_forward) by the fused forward, one kernel over all layers (1,
+ * the default; hidden widths up to 256), or by the training path's layer-by-layer products (0). The
+ * outputs are the same bits. */
+int mxmlp_set_fused(mxmlp* h, int32_t on, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

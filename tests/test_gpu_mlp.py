@@ -69,3 +69,20 @@ def test_training_lowers_the_loss():
     l1, _ = m.gradients(x[:4096], t[:4096])
     assert l1 < 0.05 * l0, (l0, l1)
     assert sec > 0
+
+
+@pytest.mark.parametrize("hidden,n", [(256, 333), (256, 70000), (128, 1000), (64, 65)])
+def test_fused_inference_gives_the_layer_by_layer_bits(hidden, n):
+    """The one-kernel forward (every layer's activations kept in LDS) and the training path's
+    layer-by-layer products give the same outputs, bit for bit, ragged tiles included."""
+    m = MLP(7, hidden, 4, 5, seed=3)
+    x, t = data(2048, 9)
+    m.fit_normalization(x, t)
+    m.train(x, t, epochs=2, batch=256, lr=1e-3, seed=0)  # weights away from their initial values
+    xs, _ = data(n, 4)
+    fused = m.predict(xs)
+    m.set_fused(False)
+    layered = m.predict(xs)
+    m.set_fused(True)
+    np.testing.assert_array_equal(fused, layered)
+    assert m.time_forward(4096, 2) > 0.0
