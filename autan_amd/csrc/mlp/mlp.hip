@@ -202,13 +202,49 @@ __global__ void k_colsum(const bf16* __restrict__ dz, int n, int width, float* _
   atomicAdd(g + c, s);
 }
 
+// A training step's counters, kept on the device so that a recorded step replays unchanged: the
+// batch of the epoch, Adam's step and its bias corrections 1 - b^step.
+struct StepState {
+  long long batch;
+  long long step;
+  float c1, c2;
+};
+
+// Adam's next step (one thread): the step count and its bias corrections; the next batch.
+__global__ void k_next_step(StepState* st, float b1, float b2) {
+  st->batch += 1;
+  st->step += 1;
+  const float t = static_cast<float>(st->step);
+  st->c1 = 1.0f - powf(b1, t);
+  st->c2 = 1.0f - powf(b2, t);
+}
+
+__global__ void k_set_batch(StepState* st, long long b) { st->batch = b; }
+
+// Batch `st->batch` of the shuffled order (n samples from idx[batch n]) into bx, bt.
+__global__ void k_gather(const long long* __restrict__ idx, const StepState* __restrict__ st, int n,
+                         const float* __restrict__ x, int dx, const float* __restrict__ t, int dt, float* __restrict__ bx,
+                         float* __restrict__ bt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long r = idx[st->batch * n + i];
+  for (int c = 0; c < dx; ++c) bx[static_cast<long long>(i) * dx + c] = x[r * dx + c];
+  for (int c = 0; c < dt; ++c) bt[static_cast<long long>(i) * dt + c] = t[r * dt + c];
+}
+
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Adam (Kingma and Ba 2015) on the FP32 master weights, then their BF16 copy. `mask` marks the
-// entries that exist (padding stays zero).
+// entries that exist (padding stays zero); the bias corrections are the step state's.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1, float* __restrict__ m2,
                        const unsigned char* __restrict__ mask, long long n, float lr, float b1, float b2, float eps,
-                       float c1, float c2, bf16* __restrict__ wb) {
+                       const StepState* __restrict__ st, bf16* __restrict__ wb) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float c1 = st->c1, c2 = st->c2;
   if (mask[i]) {
     const float gi = g[i];
     const float a = b1 * m1[i] + (1.0f - b1) * gi;
@@ -413,6 +449,8 @@ class Net {
     check(hipMemcpy(mask_.get(), mask.data(), np_, hipMemcpyHostToDevice), "hipMemcpy");
     check(hipMemset(m1_.get(), 0, np_ * sizeof(float)), "hipMemset");
     check(hipMemset(m2_.get(), 0, np_ * sizeof(float)), "hipMemset");
+    state_ = Buf<StepState>(1);
+    check(hipMemset(state_.get(), 0, sizeof(StepState)), "hipMemset");
     refresh_bf16();
     std::vector<float> ones(32, 1.0f), zeros(32, 0.0f);
     norm_ = Buf<float>(4 * 32);
@@ -500,6 +538,13 @@ class Net {
     std::vector<long long> order(static_cast<std::size_t>(n_data_));
     std::mt19937_64 rng(seed);
     float last = 0.0f;
+    hipGraphExec_t step_graph = nullptr;
+    struct GraphGuard {
+      hipGraphExec_t* g;
+      ~GraphGuard() {
+        if (*g != nullptr) (void)hipGraphExecDestroy(*g);
+      }
+    } guard{&step_graph};
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
@@ -511,11 +556,30 @@ class Net {
             "hipMemcpyAsync");
       ensure(batch);
       check(hipMemsetAsync(loss_sum_.get(), 0, sizeof(double), s_), "hipMemsetAsync");
-      for (long long b = 0; b < nb; ++b) {
-        gather(idx.get() + b * batch, batch, bx.get(), bt.get());
-        backward(bx.get(), bt.get(), batch);
-        adam(lr);
+      k_set_batch<<<1, 1, 0, s_>>>(state_.get(), 0);
+      check(hipGetLastError(), "k_set_batch");
+      if (step_graph == nullptr) {
+        // One step (gather of the state's batch, forward, loss, backward, Adam, next batch)
+        // recorded once and replayed for every batch: its launches cost one graph launch.
+        check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try {
+          k_gather<<<cdiv(batch, 256), 256, 0, s_>>>(idx.get(), state_.get(), batch, data_x_.get(), d_in_,
+                                                     data_t_.get(), d_out_, bx.get(), bt.get());
+          check(hipGetLastError(), "k_gather");
+          backward(bx.get(), bt.get(), batch);
+          adam(lr);
+        } catch (...) {
+          (void)hipStreamEndCapture(s_, &g);
+          if (g != nullptr) (void)hipGraphDestroy(g);
+          throw;
+        }
+        check(hipStreamEndCapture(s_, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check(e, "hipGraphInstantiate");
       }
+      for (long long b = 0; b < nb; ++b) check(hipGraphLaunch(step_graph, s_), "hipGraphLaunch");
       double sum = 0.0;
       check(hipMemcpyAsync(&sum, loss_sum_.get(), sizeof(double), hipMemcpyDeviceToHost, s_), "hipMemcpyAsync");
       check(hipStreamSynchronize(s_), "train");
@@ -639,7 +703,6 @@ class Net {
     }
   }
 
-  void gather(const long long* idx, int n, float* bx, float* bt);
 
   // Forward and backward of one batch: gradients into grads_, the squared errors added to
   // loss_sum_ (cleared by the caller). No host synchronisation.
@@ -650,7 +713,8 @@ class Net {
     k_loss<<<blocks, 256, 0, s_>>>(z_.back().get(), t, n, d_out_, kOutPad, norm_.get() + 64, norm_.get() + 96,
                                   dz_.back().get(), loss_sum_.get());
     check(hipGetLastError(), "k_loss");
-    check(hipMemsetAsync(grads_.get(), 0, np_ * sizeof(float), s_), "hipMemsetAsync");
+    k_fill<<<cdiv(np_, 256), 256, 0, s_>>>(grads_.get(), np_, 0.0f);  // a kernel: the step may be recorded
+    check(hipGetLastError(), "k_fill");
     const int splits = std::max(1, std::min(32, n / 512));
     for (int l = nl - 1; l >= 0; --l) {
       const int K = w_[std::size_t(l)], N = w_[std::size_t(l) + 1];
@@ -669,11 +733,11 @@ class Net {
   }
 
   void adam(float lr) {
-    ++step_;
     const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = 1.0f - std::pow(b1, static_cast<float>(step_)), c2 = 1.0f - std::pow(b2, static_cast<float>(step_));
+    k_next_step<<<1, 1, 0, s_>>>(state_.get(), b1, b2);
+    check(hipGetLastError(), "k_next_step");
     k_adam<<<cdiv(np_, 256), 256, 0, s_>>>(params_.get(), grads_.get(), m1_.get(), m2_.get(), mask_.get(), np_, lr, b1,
-                                            b2, 1e-8f, c1, c2, wb_.get());
+                                            b2, 1e-8f, state_.get(), wb_.get());
     check(hipGetLastError(), "k_adam");
   }
 
@@ -683,31 +747,17 @@ class Net {
   std::vector<long long> woff_, boff_;
   long long np_ = 0;
   Buf<float> params_, grads_, m1_, m2_, norm_;
+  Buf<StepState> state_;
   Buf<double> loss_sum_;
   Buf<bf16> wb_;
   Buf<unsigned char> mask_;
   std::vector<Buf<bf16>> acts_, dz_;
   std::vector<Buf<float>> z_;
   int cap_ = 0;
-  long long step_ = 0;
   bool fused_ = true;
   Buf<float> data_x_, data_t_;
   long long n_data_ = 0;
 };
-
-__global__ void k_gather(const long long* __restrict__ idx, int n, const float* __restrict__ x, int dx,
-                         const float* __restrict__ t, int dt, float* __restrict__ bx, float* __restrict__ bt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long long r = idx[i];
-  for (int c = 0; c < dx; ++c) bx[static_cast<long long>(i) * dx + c] = x[r * dx + c];
-  for (int c = 0; c < dt; ++c) bt[static_cast<long long>(i) * dt + c] = t[r * dt + c];
-}
-
-void Net::gather(const long long* idx, int n, float* bx, float* bt) {
-  k_gather<<<cdiv(n, 256), 256, 0, s_>>>(idx, n, data_x_.get(), d_in_, data_t_.get(), d_out_, bx, bt);
-  check(hipGetLastError(), "k_gather");
-}
 
 int fail(const std::exception& e, char* err, size_t len) {
   if (err != nullptr && len > 0) {
