@@ -318,7 +318,10 @@ class Stepper final : public StepperBase {
   }
 
   void arrays(double* prim, double* grad, double* phi, double* prim_b) override {
-    const Buffer<C>& pc = pcur_ ? prim2_ : prim_;
+    // The last residual's primitives: after a step that left the next step's in pcur_, the other
+    // buffer's.
+    const int last = prim_fresh_ ? pcur_ ^ 1 : pcur_;
+    const Buffer<C>& pc = last ? prim2_ : prim_;
     std::vector<C> p(pc.size()), r(rec_.size()), pb(prim_b_.size());
     pc.download(p.data(), p.size(), stream_);
     rec_.download(r.data(), r.size(), stream_);
