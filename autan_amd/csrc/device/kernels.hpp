@@ -664,6 +664,38 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, int f, C* lam = null
                                                    PLANAR ? nullptr : d.moment + f, d.nf, lam);
 }
 
+// The fused face kernel's slot records (face_core's 12 values) of the faces sface[0 .. ns), built
+// from the per-face arrays as face_flux assembles them, and their moments (non-planar meshes).
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_slot_records(Dev<C> d, const int* __restrict__ sface, long long ns,
+                                                         C* __restrict__ srec, C* __restrict__ smom) {
+  const long long sl = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (sl >= ns) return;
+  const int f = sface[sl];
+  C g[12];
+  load_n<C, 8>(d.fgeo + 8LL * f, g);
+  if (f < d.ni) {
+    const int2 lr = d.lr[f];
+    C rr[4];
+    load_n<C, 4>(d.frr + 4LL * f, rr);
+    g[7] = index_bits<C>(lr.x);
+    g[8] = rr[0];
+    g[9] = rr[1];
+    g[10] = rr[2];
+    g[11] = index_bits<C>(lr.y);
+  } else {
+    const int b = f - d.ni;
+    g[7] = index_bits<C>(d.bowner[b]);
+    g[8] = g[9] = g[10] = C(0);
+    g[11] = index_bits<C>(-b - 1);
+  }
+  store_n<C, 12>(srec + 12 * sl, g);
+  if (smom != nullptr) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) smom[9 * sl + q] = d.moment[static_cast<long long>(q) * d.nf + f];
+  }
+}
+
 // The face loops' fluxes stored per face for the per-cell gather (the store half of the
 // reference's store_gather), for meshes whose blocks have too many faces for the fused kernel.
 template <class C, int FLUX, bool PLANAR, bool SECOND>

@@ -395,6 +395,22 @@ class Stepper final : public StepperBase {
       stage_faces_ = value != 0;
       return;
     }
+    if (key == "face_block") {
+      if (value != 256 && value != 512 && value != 1024) throw std::invalid_argument("face_block: 256, 512 or 1024");
+      if (!fused_) throw std::invalid_argument("face_block: needs the fused face kernel");
+      if (value == block_faces_) return;
+      drop_graphs();
+      const int old = block_faces_;
+      block_faces_ = value;
+      rebuild_blocks();
+      if (!fused_ || (stage_faces_ && faces_lds(true, true) > 65536)) {
+        block_faces_ = old;
+        rebuild_blocks();
+        throw std::invalid_argument("face_block: blocks of " + std::to_string(value) +
+                                    " faces do not fit the fused face kernel here");
+      }
+      return;
+    }
     if (key == "graph") {
       drop_graphs();
       use_graph_ = value != 0;
@@ -427,7 +443,6 @@ class Stepper final : public StepperBase {
     const std::size_t n = 5 * std::size_t(nc_);
     Buffer<double> saved(n);
     check(hipMemcpyAsync(saved.get(), u_.get(), n * sizeof(double), hipMemcpyDeviceToDevice, stream_), "hipMemcpyAsync");
-    const bool can_stage = std::is_same_v<C, float> && fused_ && faces_lds(true, true) <= 65536;
     const std::size_t nk = static_cast<std::size_t>(ncase_);
     std::vector<double> r(kSums * nk), ts(nk);
     std::vector<int> inv(nk);
@@ -448,14 +463,25 @@ class Stepper final : public StepperBase {
     std::string report;
     double best = std::numeric_limits<double>::infinity();
     int best_stage = 0, best_graph = 0;
-    int best_go = 0, best_fo = 0;
+    int best_go = 0, best_fo = 0, best_fb = block_faces_;
     constexpr int kGradOcc[3] = {0, 7, 8}, kFaceOcc[3] = {0, 5, 8};
-    for (int vi = 0; vi < 36; ++vi) {
-      const int st = vi & 1, gr = (vi >> 1) & 1, go = kGradOcc[(vi >> 2) % 3], fo = kFaceOcc[(vi >> 2) / 3];
-      if (st && !can_stage) continue;
-      if (fo && (st || !std::is_same_v<C, float>)) continue;
-      if (fo == 5 && planar_) continue;
-      {
+    std::vector<int> sizes{block_faces_};
+    if (std::is_same_v<C, float> && fused_) sizes = {1024, 512};
+    for (const int fb : sizes) {
+      if (fb != block_faces_) {
+        tune("face_stage", 0);
+        try {
+          tune("face_block", fb);
+        } catch (const std::invalid_argument&) {
+          continue;  // blocks of this size do not fit here
+        }
+      }
+      const bool can_stage = std::is_same_v<C, float> && fused_ && faces_lds(true, true) <= 65536;
+      for (int vi = 0; vi < 36; ++vi) {
+        const int st = vi & 1, gr = (vi >> 1) & 1, go = kGradOcc[(vi >> 2) % 3], fo = kFaceOcc[(vi >> 2) / 3];
+        if (st && !can_stage) continue;
+        if (fo && (st || !std::is_same_v<C, float>)) continue;
+        if (fo == 5 && planar_) continue;
         tune("face_stage", st);
         tune("graph", gr);
         tune("grad_occupancy", go);
@@ -467,11 +493,13 @@ class Stepper final : public StepperBase {
         for (int k = 0; k < steps; ++k) one();
         const double ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-        const std::string name = "face_stage=" + std::to_string(st) + ",graph=" + std::to_string(gr) +
-                                 ",grad_occupancy=" + std::to_string(go) + ",face_occupancy=" + std::to_string(fo);
+        const std::string name = "face_block=" + std::to_string(block_faces_) + ",face_stage=" + std::to_string(st) +
+                                 ",graph=" + std::to_string(gr) + ",grad_occupancy=" + std::to_string(go) +
+                                 ",face_occupancy=" + std::to_string(fo);
         report += name + ":" + std::to_string(ms) + "ms;";
         if (ms < best) {
           best = ms;
+          best_fb = block_faces_;
           best_stage = st;
           best_graph = gr;
           best_go = go;
@@ -479,13 +507,16 @@ class Stepper final : public StepperBase {
         }
       }
     }
+    tune("face_stage", 0);
+    if (best_fb != block_faces_) tune("face_block", best_fb);
     tune("face_stage", best_stage);
     tune("graph", best_graph);
     tune("grad_occupancy", best_go);
     tune("face_occupancy", best_fo);
     restore();
-    return "best face_stage=" + std::to_string(best_stage) + ",graph=" + std::to_string(best_graph) +
-           ",grad_occupancy=" + std::to_string(best_go) + ",face_occupancy=" + std::to_string(best_fo) + ";" + report;
+    return "best face_block=" + std::to_string(best_fb) + ",face_stage=" + std::to_string(best_stage) +
+           ",graph=" + std::to_string(best_graph) + ",grad_occupancy=" + std::to_string(best_go) +
+           ",face_occupancy=" + std::to_string(best_fo) + ";" + report;
   }
 
   // One SSP-RK3 step of every active case: norms, time step and invalid cell per case.
@@ -847,7 +878,11 @@ class Stepper final : public StepperBase {
     bowner_.upload(bowner, stream_);
     cfo_.upload(cfo, stream_);
     inc_.upload(inc, stream_);
-    build_slots(t, cfo);
+    // Kept for rebuilding the face blocks at another size (the face_block tuning key).
+    cell_faces_host_.assign(t.cell_faces, t.cell_faces + cfo.back());
+    owner_host_.assign(t.owner, t.owner + nf_);
+    build_slots(cell_faces_host_.data(), owner_host_.data(), cfo);
+    cfo_host_ = std::move(cfo);
     check(hipStreamSynchronize(stream_), "topology");
   }
 
@@ -855,7 +890,7 @@ class Stepper final : public StepperBase {
   // most max_block_faces() distinct faces. For each
   // block its faces ascending, and for every incidence the LDS slot of its face with the cell's
   // side and whether the face is a boundary face.
-  void build_slots(const mxeuler_topology& t, const std::vector<int>& cfo) {
+  void build_slots(const int* cell_faces, const int* owner, const std::vector<int>& cfo) {
     std::vector<int> cbeg{0}, sbeg{0}, sface;
     sface.reserve(std::size_t(cfo.back()) / 2 + std::size_t(nc_));
     std::vector<unsigned short> islot(std::size_t(std::max(cfo.back(), 1)), 0);
@@ -874,10 +909,10 @@ class Stepper final : public StepperBase {
       fcase.push_back(cs);
       while (c < cend && c - c0 < kBlock) {
         int fresh = 0;
-        for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) fresh += mark[std::size_t(t.cell_faces[k])] != b;
+        for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) fresh += mark[std::size_t(cell_faces[k])] != b;
         if (static_cast<int>(faces.size()) + fresh > max_block_faces() && c > c0) break;
         for (int k = cfo[std::size_t(c)]; k < cfo[std::size_t(c) + 1]; ++k) {
-          const int f = t.cell_faces[k];
+          const int f = cell_faces[k];
           if (mark[std::size_t(f)] != b) {
             mark[std::size_t(f)] = b;
             faces.push_back(f);
@@ -890,9 +925,9 @@ class Stepper final : public StepperBase {
       if (faces.size() > 0x3fff) fits = false;
       for (int cc = c0; cc < c && fits; ++cc) {
         for (int k = cfo[std::size_t(cc)]; k < cfo[std::size_t(cc) + 1]; ++k) {
-          const int f = t.cell_faces[k];
+          const int f = cell_faces[k];
           const auto slot = static_cast<unsigned>(std::lower_bound(faces.begin(), faces.end(), f) - faces.begin());
-          const bool right = f < ni_ && t.owner[f] != cc;
+          const bool right = f < ni_ && owner[f] != cc;
           const bool boundary = f >= ni_;
           islot[std::size_t(k)] = static_cast<unsigned short>(slot | (right ? 0x4000u : 0u) | (boundary ? 0x8000u : 0u));
         }
@@ -922,9 +957,7 @@ class Stepper final : public StepperBase {
     sface_host_ = std::move(sface);
   }
 
-  // Faces per face block: four rounds of the workgroup in FP32 (a block then holds a full 256
-  // cells on hexahedral meshes), two in FP64 (whose fluxes take twice the LDS).
-  static constexpr int max_block_faces() { return std::is_same_v<C, float> ? 2 * kMaxSlots : kMaxSlots; }
+  int max_block_faces() const { return block_faces_; }
 
   // Dynamic LDS of the fused face kernel: the face fluxes (and lambdas), the exact-sum words of
   // the first residual, the staged cells.
@@ -1091,7 +1124,7 @@ class Stepper final : public StepperBase {
     for (std::size_t c = 0; c < nc; ++c) vol[c] = static_cast<C>(g.cell_volume[c]);
     vol_.upload(vol, stream_);
     vol64_.upload(vol64, stream_);
-    if (fused_) build_slot_records(t, g);
+    if (fused_) build_slot_records();
     // FP64 x_f - x_c of the boundary faces, for the wall loads.
     std::vector<double> rb(3 * std::max<std::size_t>(nf - ni, 1), 0.0);
     for (std::size_t f = ni; f < nf; ++f) {
@@ -1121,45 +1154,41 @@ class Stepper final : public StepperBase {
 
   // The fused face kernel's per-slot face records (face_core's 12 values) and moments, in slot
   // order: each block reads its faces' geometry as one contiguous run.
-  void build_slot_records(const mxeuler_topology& t, const mxeuler_geometry& g) {
-    const std::size_t nf = std::size_t(nf_), ni = std::size_t(ni_);
+  // The slot records of the faces build_slots listed, built on the device from the per-face
+  // arrays (fgeo, frr, lr, bowner, moment: uploaded before).
+  void build_slot_records() {
     const std::size_t ns = std::max<std::size_t>(sface_host_.size(), 1);
-    auto bits = [](int i) {
-      C v;
-      if constexpr (sizeof(C) == 4) {
-        const int32_t x = i;
-        std::memcpy(&v, &x, 4);
-      } else {
-        const int64_t x = i;
-        std::memcpy(&v, &x, 8);
-      }
-      return v;
-    };
-    std::vector<C> rec(12 * ns, C(0));
-    for (std::size_t sl = 0; sl < sface_host_.size(); ++sl) {
-      const auto f = std::size_t(sface_host_[sl]);
-      C* r = rec.data() + 12 * sl;
-      for (std::size_t u = 0; u < 3; ++u) {
-        r[u] = static_cast<C>(g.face_normal[u * nf + f]);
-        r[4 + u] = static_cast<C>(g.r_left[u * nf + f]);
-        r[8 + u] = f < ni ? static_cast<C>(g.r_right[u * nf + f]) : C(0);
-      }
-      r[3] = static_cast<C>(g.face_area[f]);
-      r[7] = bits(t.owner[f]);
-      r[11] = bits(f < ni ? t.neighbour[f] : -static_cast<int>(f - ni) - 1);
-    }
-    srec_.upload(rec, stream_);
-    if (!planar_) {
-      std::vector<C> m(9 * ns, C(0));
-      for (std::size_t sl = 0; sl < sface_host_.size(); ++sl) {
-        const auto f = std::size_t(sface_host_[sl]);
-        for (std::size_t q = 0; q < 9; ++q) m[9 * sl + q] = static_cast<C>(g.face_moment[q * nf + f]);
-      }
-      smom_.upload(m, stream_);
+    Buffer<int> sface(ns);
+    sface.upload(sface_host_.data(), sface_host_.size(), stream_);
+    srec_ = Buffer<C>(12 * ns);
+    smom_ = planar_ ? Buffer<C>() : Buffer<C>(9 * ns);
+    const long long n = static_cast<long long>(sface_host_.size());
+    if (n > 0) {
+      k_slot_records<C><<<static_cast<int>((n + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
+          dev(), sface.get(), n, srec_.get(), planar_ ? nullptr : smom_.get());
+      check(hipGetLastError(), "k_slot_records");
     }
     check(hipStreamSynchronize(stream_), "slot records");
     sface_host_.clear();
     sface_host_.shrink_to_fit();
+  }
+
+  // The face blocks rebuilt at the current block_faces_ (the topology kept on the host), with their
+  // records, the batch's block lists and the per-block partials sized for them.
+  void rebuild_blocks() {
+    check(hipStreamSynchronize(stream_), "rebuild");
+    build_slots(cell_faces_host_.data(), owner_host_.data(), cfo_host_);
+    if (!fused_) return;
+    build_slot_records();
+    const std::size_t need = std::size_t(std::max(grid(nc_), xcd_grid(face_blocks_)));
+    if (partial_.size() < need) partial_ = Buffer<unsigned long long>(need);
+    if (compact_) {
+      fblk_ = Buffer<int>(std::size_t(xcd_grid(face_blocks_)));
+      cblk_host_.clear();
+      fblk_host_.clear();
+      build_lists();
+      check(hipStreamSynchronize(stream_), "rebuild");
+    }
   }
 
   void set_boundary(const double* bc_kind, const double* bc) {
@@ -1549,6 +1578,10 @@ class Stepper final : public StepperBase {
   Buffer<Params<C>> ptab_;
   Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
   std::vector<int> fcbeg_host_;
+  std::vector<int> cfo_host_, cell_faces_host_, owner_host_;
+  // Faces per face block: four rounds of the workgroup in FP32 (a full 256 cells of hexahedra per
+  // block), two in FP64 (whose fluxes take twice the LDS); the face_block tuning key changes it.
+  int block_faces_ = std::is_same_v<C, float> ? 2 * kMaxSlots : kMaxSlots;
   bool compact_ = false;
   Buffer<int> cblk_, fblk_;
   std::vector<int> cblk_host_, fblk_host_;
