@@ -120,7 +120,7 @@ def test_autotune_restores_the_state_and_keeps_the_bits():
     b = batch.BatchStepper(cs, "fp32+acc64")
     b.upload()
     report = b.autotune(steps=2)
-    assert report.startswith("best face_stage=")
+    assert report.startswith("best face_block=")
     for got, c in zip(b.download(), cs):
         np.testing.assert_array_equal(got, c.u)
     r, h, inv = b.step(cfl=0.8, local=True)
