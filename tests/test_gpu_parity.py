@@ -201,6 +201,25 @@ def test_invalid_cell_is_the_least_index():
     assert e.invalid == min(bad) == ora.first_invalid(u)
 
 
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+@pytest.mark.parametrize("kind", ["fixed", "local"])
+def test_step_that_breaks_the_state_reports_the_oracles_cell(kind, policy):
+    """A step too large for the state (a fixed step far above the CFL limit, or local steps at a
+    CFL number of 40) leaves cells with a negative density or pressure: the step's own check (inside
+    the last stage's face kernel) reports the least such cell, as the oracle's step does."""
+    case = CASES["tets-hllc2"]()
+    dev, ora = build(case, policy)
+    dev.upload(case.u)
+    if kind == "fixed":
+        r = dev.step(dt=2.0)
+        _, _, _, inv = ora.step(case.u, dt=2.0)
+    else:
+        r = dev.step(cfl=40.0, local=True)
+        _, _, _, inv = ora.step(case.u, cfl=40.0, local=True)
+    assert inv is not None
+    assert r.invalid == inv
+
+
 def test_second_order_switch_matches_oracle():
     case = CASES["tets-hllc2"]()
     dev, ora = build(case, "fp64")
