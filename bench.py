@@ -372,6 +372,50 @@ def warmstart_main(args) -> int:
     return 0
 
 
+def mlp_main(args) -> int:
+    """The warm-start network's kernels alone: the fused BF16 inference on device-resident samples
+    (against the dense BF16 matrix-core peak) and one training epoch (device-resident data, the
+    recorded step replayed per batch), on synthetic inputs of the network's shape."""
+    import numpy as np
+
+    from autan_amd.mlp import MLP
+
+    rng = np.random.default_rng(0)
+    net = MLP(7, 256, 4, 5, seed=0)
+    n_train = 1 << 21
+    x = rng.uniform(-1, 1, size=(n_train, 7)).astype(np.float32)
+    t = np.stack([np.sin(2 * x[:, 0]) + x[:, 1], np.cos(x[:, 2] * x[:, 3]), x[:, 4] ** 2, x[:, 5] - x[:, 6],
+                  np.tanh(x[:, 0] + x[:, 6])], axis=1).astype(np.float32)
+    net.fit_normalization(x, t)
+    infer = {}
+    for n in (1 << 18, 1 << 20):
+        ms = net.time_forward(n, 20)
+        infer[str(n)] = {"ms": round(ms, 4), "TFLOP/s": round(net.flops_per_sample() * n / (ms * 1e-3) / 1e12, 1)}
+        log(f"inference of {n} samples: {ms:.3f} ms, {infer[str(n)]['TFLOP/s']} TFLOP/s")
+    net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=0)  # records the step
+    loss, sec = net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=1)
+    train_tf = 3 * net.flops_per_sample() * (n_train // 4096) * 4096 / sec / 1e12
+    log(f"training epoch of {n_train} samples: {sec:.3f} s, {train_tf:.1f} TFLOP/s effective, loss {loss:.3e}")
+    big = infer[str(1 << 20)]
+    line = {
+        "metric": "BF16 inference TFLOP/s (real products) of the warm-start network",
+        "value": big["TFLOP/s"],
+        "unit": "TFLOP/s",
+        "higher_is_better": True,
+        "n_gpus": 1,
+        "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic",
+        "config": {"workload": "MLP 7-256x4-5 SiLU, fused inference of 2^20 samples; training epoch of 2^21 "
+                               "samples at batch 4096"},
+        "roofline": {"bound": "mfma", "kernel": "k_infer", "achieved": big["TFLOP/s"], "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(big["TFLOP/s"] / 2500.0, 4), "traffic": None},
+        "inference": infer,
+        "training_epoch": {"seconds": round(sec, 3), "TFLOP/s_effective": round(train_tf, 1), "loss": loss},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def mms_main(args) -> int:
     """Observed order of accuracy on the manufactured solution (Kuhn tetrahedra at the listed
     levels; steady discrete solutions on the device, errors at the centroids)."""
@@ -408,7 +452,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (tets 20, bump 20, vortex 200)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="tets", choices=["tets", "vortex", "bump", "sweep", "warmstart", "mms"])
+    ap.add_argument("--workload", default="tets",
+                    choices=["tets", "vortex", "bump", "sweep", "warmstart", "mms", "mlp"])
     ap.add_argument("--levels", default="64,96,128", help="mms: tetrahedra levels n (n^3 x 6 cells)")
     ap.add_argument("--policies", default="fp64,fp32+acc64", help="mms: policies")
     ap.add_argument("--unlimited", action="store_true", help="mms: no limiter (the design order)")
@@ -446,6 +491,8 @@ def main() -> int:
         return warmstart_main(args)
     if args.workload == "mms":
         return mms_main(args)
+    if args.workload == "mlp":
+        return mlp_main(args)
     defaults = {
         "tets": ("fp32+acc64", "hllc", 20),
         "vortex": ("fp64", "roe", 200),
