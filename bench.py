@@ -392,10 +392,13 @@ def mlp_main(args) -> int:
         ms = net.time_forward(n, 20)
         infer[str(n)] = {"ms": round(ms, 4), "TFLOP/s": round(net.flops_per_sample() * n / (ms * 1e-3) / 1e12, 1)}
         log(f"inference of {n} samples: {ms:.3f} ms, {infer[str(n)]['TFLOP/s']} TFLOP/s")
-    net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=0)  # records the step
-    loss, sec = net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=1)
-    train_tf = 3 * net.flops_per_sample() * (n_train // 4096) * 4096 / sec / 1e12
-    log(f"training epoch of {n_train} samples: {sec:.3f} s, {train_tf:.1f} TFLOP/s effective, loss {loss:.3e}")
+    if args.no_train:
+        sec = loss = train_tf = None
+    else:
+        net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=0)  # records the step
+        loss, sec = net.train(x, t, epochs=1, batch=4096, lr=1e-3, seed=1)
+        train_tf = 3 * net.flops_per_sample() * (n_train // 4096) * 4096 / sec / 1e12
+        log(f"training epoch of {n_train} samples: {sec:.3f} s, {train_tf:.1f} TFLOP/s effective, loss {loss:.3e}")
     big = infer[str(1 << 20)]
     line = {
         "metric": "BF16 inference TFLOP/s (real products) of the warm-start network",
@@ -410,7 +413,8 @@ def mlp_main(args) -> int:
         "roofline": {"bound": "mfma", "kernel": "k_infer", "achieved": big["TFLOP/s"], "peak": 2500.0,
                      "unit": "TFLOP/s", "frac": round(big["TFLOP/s"] / 2500.0, 4), "traffic": None},
         "inference": infer,
-        "training_epoch": {"seconds": round(sec, 3), "TFLOP/s_effective": round(train_tf, 1), "loss": loss},
+        "training_epoch": None if sec is None else {"seconds": round(sec, 3), "TFLOP/s_effective": round(train_tf, 1),
+                                                    "loss": loss},
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -455,6 +459,8 @@ def main() -> int:
     ap.add_argument("--workload", default="tets",
                     choices=["tets", "vortex", "bump", "sweep", "warmstart", "mms", "mlp"])
     ap.add_argument("--levels", default="64,96,128", help="mms: tetrahedra levels n (n^3 x 6 cells)")
+    ap.add_argument("--no-train", action="store_true", help="mlp: inference only (the training step is a recorded "
+                    "HIP graph, which kernel tracing does not follow here)")
     ap.add_argument("--policies", default="fp64,fp32+acc64", help="mms: policies")
     ap.add_argument("--unlimited", action="store_true", help="mms: no limiter (the design order)")
     ap.add_argument("--residual-orders", type=float, default=8.0, help="mms: relative residual drop of the steady runs")
