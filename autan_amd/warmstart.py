@@ -1,7 +1,7 @@
 """The learned warm start of the bump sweep: a network maps a cell (centroid, distance to the bump
 wall) and its case (bump height, Mach number, flow angle) to an initial state (rho, u, v, w, p). It
 is trained on converged states that the batch stepper computes for cases of the sweep's
-generator, and judged on held-out cases solved to the same tolerance from the freestream and from
+generator, and evaluated on held-out cases solved to the same tolerance from the freestream and from
 the network's state: every run still converges under the plain solver's rules, so the network
 only changes where the iterations start.
 """

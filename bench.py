@@ -488,8 +488,8 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true", help="bump: skip the steady run, conservation and fp64 check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box here has one, and this build has no multi-GPU path "
-              "(the sharding design is in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the stepper holds a whole mesh on one device and has no "
+              "multi-GPU path. Run with --gpus 1.", file=sys.stderr)
         return 2
     if args.workload == "sweep":
         return sweep_main(args)
