@@ -1520,6 +1520,12 @@ class Stepper final : public StepperBase {
     check(hipStreamSynchronize(stream_), "step");
     for (int cs = 0; cs < ncase_; ++cs) {
       const Small& sm = host_small_[cs];
+      if (ncase_ > 1 && !(cflags_host_[std::size_t(cs)] & 1)) {
+        // A frozen case: nothing was added to its cleared words (its norms are 0, no invalid cell).
+        for (int k = 0; k < kSums; ++k) residuals[kSums * cs + k] = 0.0;
+        invalid[cs] = -1;
+        continue;
+      }
       for (int k = 0; k < kSums; ++k) {
         residuals[kSums * cs + k] = std::sqrt(exact_value(sm.words[k]) / double(case_cells_[std::size_t(cs)]));
       }
