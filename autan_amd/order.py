@@ -44,7 +44,7 @@ def fit_order(h, e) -> tuple[float, float]:
 
 
 def level(n: int, policy: str = "fp32+acc64", flux: str = "hllc", limiter: bool = True,
-          max_iterations: int = 20000, residual_orders: float = 8.0, log=print) -> dict:
+          max_iterations: int = 20000, residual_orders: float = 8.0, log=print, chunk: int = 4000) -> dict:
     """One level: the Kuhn tetrahedra n^3 x 6 of the benchmark, its steady discrete solution and its
     errors against the manufactured solution at the centroids."""
     w = workloads.tets(n, policy, flux)
@@ -52,22 +52,32 @@ def level(n: int, policy: str = "fp32+acc64", flux: str = "hllc", limiter: bool 
     m = w.mesh
     dev = DeviceStepper(m, w.scheme, w.policy, w.bc_kind, w.bc, w.source)
     dev.upload(w.u)
+    # In chunks, each dropping from the first chunk's iteration-1 residuals, so a long run reports
+    # its progress as it goes.
     t0 = time.perf_counter()
-    run = steady.solve(dev, max_iterations=max_iterations, residual_orders=residual_orders, start_order="second",
-                       level_rule=False)
+    run = steady.solve(dev, max_iterations=min(chunk, max_iterations), residual_orders=residual_orders,
+                       start_order="second", level_rule=False)
+    reference = run.reference
+    iterations = run.iterations
+    while run.status == steady.BUDGET and iterations < max_iterations:
+        log(f"order: n={n} {policy}: {iterations} iterations, residual drop "
+            f"{[round(-math.log10(a / b), 2) if a > 0 and b > 0 else None for a, b in zip(run.residuals, reference)]}")
+        run = steady.solve(dev, max_iterations=min(chunk, max_iterations - iterations), residual_orders=residual_orders,
+                           start_order="second", level_rule=False, reference=reference)
+        iterations += run.iterations
     sec = time.perf_counter() - t0
     u = dev.download()
     vol = m.arrays["cell_volume"]
     e = errors(vol, u, w.u)
-    drop = [float(a / b) if b > 0 and a > 0 else None for a, b in zip(run.residuals, run.reference)]
-    log(f"order: n={n} {policy} {flux} limiter={limiter}: {run.iterations} iterations in {sec:.1f} s, "
+    drop = [float(a / b) if b > 0 and a > 0 else None for a, b in zip(run.residuals, reference)]
+    log(f"order: n={n} {policy} {flux} limiter={limiter}: {iterations} iterations in {sec:.1f} s, "
         f"status {run.status}, errors {e}")
     return {
         "n": n,
         "cells": m.n_cells,
         "h": m.n_cells ** (-1.0 / 3.0),
         "policy": policy,
-        "iterations": int(run.iterations),
+        "iterations": int(iterations),
         "status": ["converged", "budget", "diverged"][run.status],
         "seconds": round(sec, 2),
         "residual_drop_orders": [None if d is None else round(-math.log10(d), 2) for d in drop],
