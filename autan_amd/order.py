@@ -35,7 +35,9 @@ def errors(volume: np.ndarray, u: np.ndarray, exact: np.ndarray) -> np.ndarray:
 
 def fit_order(h, e) -> tuple[float, float]:
     """The least-squares slope of log e against log h, and the order of the finest pair (the two
-    smallest h)."""
+    smallest h); at least two levels."""
+    if len(h) < 2 or len(h) != len(e):
+        raise ValueError("an order needs errors on two levels at least")
     lh, le = np.log(np.asarray(h, dtype=np.float64)), np.log(np.asarray(e, dtype=np.float64))
     slope = float(np.polyfit(lh, le, 1)[0])
     i = np.argsort(lh)[:2]
@@ -94,10 +96,11 @@ def study(levels=(64, 96, 128), policies=("fp64", "fp32+acc64"), flux: str = "hl
     for pol in policies:
         rows = [level(n, pol, flux, limiter, max_iterations, residual_orders, log) for n in levels]
         out["levels"][pol] = rows
-        out["orders"][pol] = {
-            k: [round(x, 3) for x in fit_order([r["h"] for r in rows], [r["errors"][k] for r in rows])]
-            for k in CONSERVED
-        }
+        if len(rows) >= 2:  # an order needs two levels at least
+            out["orders"][pol] = {
+                k: [round(x, 3) for x in fit_order([r["h"] for r in rows], [r["errors"][k] for r in rows])]
+                for k in CONSERVED
+            }
     if "fp64" in policies and "fp32+acc64" in policies:
         for r64, r32 in zip(out["levels"]["fp64"], out["levels"]["fp32+acc64"]):
             m = workloads.tets(r64["n"], "fp64", flux).mesh

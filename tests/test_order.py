@@ -32,3 +32,8 @@ def test_finest_pair_is_the_two_smallest_h():
     e = [0.09, 0.01 * 1.5, 0.04]  # the finest pair (0.1, 0.2) has order log(0.04/0.015)/log 2
     _, pair = order.fit_order(h, e)
     assert pair == pytest.approx(np.log(0.04 / 0.015) / np.log(2.0))
+
+
+def test_one_level_has_no_order():
+    with pytest.raises(ValueError):
+        order.fit_order([0.1], [1e-3])
