@@ -776,6 +776,33 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// A 64-bit integer moved across lanes by a DPP control (zero where the source lane is out of the
+// row, or the destination row is masked off), as two 32-bit halves.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ long long dpp_move(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, ROWS, 0xf, true);
+  return static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
+// The wave's sum of a 64-bit integer, in registers: prefix sums within each 16-lane row (row_shr
+// 1, 2, 4, 8), then each row's total carried into the rows after it (row_bcast 15 into rows 1 and
+// 3, row_bcast 31 into rows 2 and 3); lane 63 then holds the sum, read by every lane. Integer
+// sums: the same words as any other order.
+__device__ __forceinline__ long long wave_sum_rows(long long v) {
+  v += dpp_move<0x111, 0xf>(v);
+  v += dpp_move<0x112, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), kWave - 1);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), kWave - 1);
+  return static_cast<long long>((static_cast<unsigned long long>(static_cast<unsigned int>(hi)) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
 // spectral_radius (kernels/euler.py:1091-1112), then local_time_step (dt_c = CFL V / lambda,
 // physics/src/device_stepper.cuh:83-89) and each block's least dt_c as an ordered key in
 // partial[block]; k_min_key reduces those exactly (no same-address atomics from every wave).
@@ -1000,7 +1027,7 @@ __device__ __forceinline__ void exact_add_wave(long long* acc, double t, bool ac
   const int last = wave_max(finite ? i : -1);
   for (int j = first; j <= last + 2; ++j) {
     const int dj = j - i;
-    const long long v = wave_sum(finite && dj >= 0 && dj < 3 ? part[dj] : 0LL);
+    const long long v = wave_sum_rows(finite && dj >= 0 && dj < 3 ? part[dj] : 0LL);
     if ((threadIdx.x % kWave) == 0 && v != 0) atomicAdd(reinterpret_cast<unsigned long long*>(&acc[j]), static_cast<unsigned long long>(v));
   }
 }
