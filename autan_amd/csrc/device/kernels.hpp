@@ -786,6 +786,25 @@ __device__ __forceinline__ long long dpp_move(long long v) {
                                 static_cast<unsigned int>(lo));
 }
 
+// The wave's least or greatest 32-bit integer, in registers, by the same row shifts and
+// broadcasts (lanes without a source keep `identity`).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move32(int v, int identity) {
+  return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWS, 0xf, false);
+}
+template <bool MIN>
+__device__ __forceinline__ int wave_extreme_rows(int v) {
+  constexpr int id = MIN ? INT32_MAX : INT32_MIN;
+  auto pick = [](int a, int b) { return MIN ? (b < a ? b : a) : (b > a ? b : a); };
+  v = pick(v, dpp_move32<0x111, 0xf>(v, id));
+  v = pick(v, dpp_move32<0x112, 0xf>(v, id));
+  v = pick(v, dpp_move32<0x114, 0xf>(v, id));
+  v = pick(v, dpp_move32<0x118, 0xf>(v, id));
+  v = pick(v, dpp_move32<0x142, 0xa>(v, id));
+  v = pick(v, dpp_move32<0x143, 0xc>(v, id));
+  return __builtin_amdgcn_readlane(v, kWave - 1);
+}
+
 // The wave's sum of a 64-bit integer, in registers: prefix sums within each 16-lane row (row_shr
 // 1, 2, 4, 8), then each row's total carried into the rows after it (row_bcast 15 into rows 1 and
 // 3, row_bcast 31 into rows 2 and 3); lane 63 then holds the sum, read by every lane. Integer
@@ -1023,8 +1042,8 @@ __device__ __forceinline__ void exact_add_wave(long long* acc, double t, bool ac
       part[2] = -part[2];
     }
   }
-  const int first = wave_min(i);
-  const int last = wave_max(finite ? i : -1);
+  const int first = wave_extreme_rows<true>(i);
+  const int last = wave_extreme_rows<false>(finite ? i : -1);
   for (int j = first; j <= last + 2; ++j) {
     const int dj = j - i;
     const long long v = wave_sum_rows(finite && dj >= 0 && dj < 3 ? part[dj] : 0LL);
