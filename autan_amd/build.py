@@ -4,6 +4,9 @@
     libmxeuler.so  HIP kernels and the C ABI, hipcc for gfx950
     libmxmlp.so    the warm-start network (BF16 matrix-core kernels), hipcc for gfx950
     oracle/_build/liboracle.so   the CPU oracle (test infrastructure), g++ -O3 -ffp-contract=off
+    tests/_build/adapter_check   a test program of the C++ adapter include/mxeuler.hpp, g++
+    oracle/_ref/json_dump        the reference runtime's JSON library as a checker (oracle/build_ref.sh;
+                                 built only where the reference tree is)
 
 Every build passes -ffp-contract=off, as the reference builds host and device code
 (CMakeLists.txt:45-55), so that no product is fused into an FMA.
@@ -72,6 +75,20 @@ def build_oracle() -> Path:
     return out
 
 
+def build_adapter_check() -> Path:
+    """The C++ adapter's test program, linked against the in-tree libmxeuler.so."""
+    out = ROOT / "tests" / "_build" / "adapter_check"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-Werror", "-I", ROOT / "include", "-o", out,
+          ROOT / "tests" / "cpp" / "adapter_check.cpp", f"-L{PKG}", "-l:libmxeuler.so",
+          "-Wl,-rpath,$ORIGIN/../../autan_amd"])
+    return out
+
+
+def build_ref() -> None:
+    _run(["sh", ROOT / "oracle" / "build_ref.sh"])
+
+
 def build_all() -> None:
     if shutil.which("g++") is None:
         raise RuntimeError("g++ not found")
@@ -79,9 +96,12 @@ def build_all() -> None:
     build_oracle()
     build_device()
     build_mlp()
+    build_adapter_check()
+    build_ref()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "mlp": build_mlp, "all": build_all}[w]()
+        {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "mlp": build_mlp,
+         "adapter": build_adapter_check, "ref": build_ref, "all": build_all}[w]()

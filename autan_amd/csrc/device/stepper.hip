@@ -1639,12 +1639,14 @@ struct mxeuler_stepper {
 
 namespace {
 
+// 1: the input was refused (std::invalid_argument, as the reference's stepper throws for it);
+// 2: a device or runtime failure (std::runtime_error and the rest).
 int fail(const std::exception& e, char* err, size_t err_len) {
   if (err != nullptr && err_len > 0) {
     std::strncpy(err, e.what(), err_len - 1);
     err[err_len - 1] = '\0';
   }
-  return 1;
+  return dynamic_cast<const std::invalid_argument*>(&e) != nullptr ? 1 : 2;
 }
 
 template <class F>

@@ -11,10 +11,12 @@
  * the boundary as FP64 structure-of-arrays, component k of cell c at k * n_cells + c.
  *
  * Conventions (the reference's C ABI precedent, physics/mms/capi.cpp:126-347): every call
- * returning int returns 0 on success, 1 on error with a message in err[0 .. err_len) that starts
- * with the reference's error code where one exists (E1101 for a device failure). Handles are
- * opaque. Calls are synchronous: each returns once its results are on the host. One handle is
- * driven by one host thread.
+ * returning int returns 0 on success and otherwise puts a message in err[0 .. err_len) that starts
+ * with the reference's error code where one exists (E1101 for a device failure). The value tells
+ * the two exception kinds of the reference's stepper apart: 1 when the input is refused
+ * (std::invalid_argument), 2 for a device or runtime failure (std::runtime_error); mxeuler.hpp
+ * rethrows them as those exceptions. Handles are opaque. Calls are synchronous: each returns once
+ * its results are on the host. One handle is driven by one host thread.
  */
 #ifndef MXEULER_H
 #define MXEULER_H
