@@ -8,12 +8,13 @@ only changes where the iterations start.
 
 from __future__ import annotations
 
+import hashlib
 import math
 import time
 
 import numpy as np
 
-from autan_amd import batch, sweep
+from autan_amd import batch, initial_state, sweep
 
 GAMMA = 1.4
 
@@ -77,9 +78,9 @@ def primitive(u: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
 
 
 def conserved(w: np.ndarray, gamma: float = GAMMA) -> np.ndarray:
-    """5 x n conserved state of n x 5 primitives."""
-    rho, u, v, ww, p = (np.asarray(w, dtype=np.float64)[:, k] for k in range(5))
-    return np.stack([rho, rho * u, rho * v, rho * ww, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + ww * ww)]).reshape(-1)
+    """5 x n conserved state (flat) of n x 5 primitives, as the reference's runtime builds a start
+    state from an initial-state file."""
+    return initial_state.conserved(w, gamma).reshape(-1)
 
 
 def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=20000, residual_orders=8.0,
@@ -158,12 +159,38 @@ def warm_states(net, specs: list[sweep.Spec], policy="fp32+acc64", flux="hllc"):
     for s in specs:
         c = sweep.build(s, policy, flux)
         t0 = time.perf_counter()
-        w = net.predict(features(c.mesh, s)).astype(np.float64)
-        w[:, 0] = np.maximum(w[:, 0], 0.05)
-        w[:, 4] = np.maximum(w[:, 4], 0.05 / GAMMA)
-        out.append(conserved(w))
+        out.append(conserved(predicted(net, c.mesh, s)))
         seconds += time.perf_counter() - t0
     return out, seconds
+
+
+def predicted(net, mesh, spec: sweep.Spec) -> np.ndarray:
+    """n x 5 primitives (rho, u, v, w, p) the network gives the mesh's cells, density and pressure
+    kept positive."""
+    w = net.predict(features(mesh, spec)).astype(np.float64)
+    w[:, 0] = np.maximum(w[:, 0], 0.05)
+    w[:, 4] = np.maximum(w[:, 4], 0.05 / GAMMA)
+    return w
+
+
+def mesh_digest(mesh) -> str:
+    """SHA-256 of a generated mesh's points and faces (the reference hashes a mesh's file; a mesh
+    built in memory has none)."""
+    h = hashlib.sha256()
+    for name in ("x", "y", "z", "face_node_offsets", "face_nodes", "owner", "neighbour"):
+        h.update(np.ascontiguousarray(mesh.arrays[name]).tobytes())
+    return h.hexdigest()
+
+
+def write_initial_state(path, net, spec: sweep.Spec, policy="fp32+acc64", flux="hllc") -> tuple[str, object]:
+    """The network's state of a sweep case as an autan.initial-state/1 file (one row per cell at its
+    centroid), the warm start's interchange with the reference's SDK and runtime. Returns the
+    file's SHA-256 and the case."""
+    c = sweep.build(spec, policy, flux)
+    digest = initial_state.write(path, oracle=f"autan_amd warm start: h={spec.h!r} M={spec.mach!r} alpha={spec.alpha!r}",
+                                 mesh_sha256=mesh_digest(c.mesh), centroids=c.mesh.centroids(),
+                                 values=predicted(net, c.mesh, spec))
+    return digest, c
 
 
 def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_case: int = 4096, batch: int = 4096,

@@ -28,3 +28,22 @@ def test_the_trained_network_saves_iterations():
     assert np.all(np.isfinite([w["result"].residuals for w in cmp["warm_abs"]]))
     print(f"iterations to the threshold: freestream {it_free}, warm {it_warm} ({it_free / it_warm:.2f}x)")
     assert it_warm < 0.8 * it_free, (it_warm, it_free)
+
+
+def test_the_warm_start_travels_as_an_initial_state_file(tmp_path):
+    """The network's state of a case written as an autan.initial-state/1 file, read back with the
+    runtime's checks and paired with the cells by centroid, is the warm start's state bit for bit."""
+    from autan_amd import initial_state
+    from autan_amd.mlp import MLP
+
+    net = MLP(7, 256, 4, 5, seed=1)
+    net.set_normalization(np.zeros(7), np.ones(7), np.array([1.0, 0.5, 0.0, 0.0, 1 / 1.4]), np.full(5, 0.05))
+    spec = sweep.specs(1, seed=5, cells=CELLS)[0]
+    digest, case = warmstart.write_initial_state(tmp_path / "state.json", net, spec)
+    st = initial_state.read(tmp_path / "state.json", sha256=digest, mesh_sha256=warmstart.mesh_digest(case.mesh))
+    m = case.mesh
+    u = initial_state.start_state(st, m.centroids(), np.column_stack([m.x, m.y, m.z]), warmstart.GAMMA)
+    states, _ = warmstart.warm_states(net, [spec])
+    net.close()
+    assert u.shape == states[0].shape
+    assert np.array_equal(u.view(np.uint64), np.asarray(states[0], dtype=np.float64).view(np.uint64))
