@@ -71,6 +71,10 @@ int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_le
   return guarded(err, err_len, [&] { make(mx::read_polymesh(dir), out); });
 }
 
+int mxmesh_read_gmsh(const char* path, mxmesh** out, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] { make(mx::read_gmsh(path), out); });
+}
+
 void mxmesh_destroy(mxmesh* m) { delete m; }
 
 int mxmesh_renumber_rcm(mxmesh* m, char* err, size_t err_len) {

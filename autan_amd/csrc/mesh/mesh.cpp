@@ -2,6 +2,8 @@
 #include "mesh.hpp"
 
 #include <algorithm>
+#include <cctype>
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -9,6 +11,7 @@
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace mx {
 
@@ -651,6 +654,311 @@ Mesh read_polymesh(const std::string& dir) {
     f.zone.push_back(i < nei.size() ? -1 : zone[i]);
   }
   return assemble(static_cast<index_t>(n_cells), std::move(x), std::move(y), std::move(z), std::move(f));
+}
+
+// Gmsh MSH 4.1 (ASCII) ------------------------------------------------------------------------------
+
+namespace {
+
+// Whitespace-separated tokens of a text; a "quoted string" is one token.
+class Tokens {
+ public:
+  explicit Tokens(std::string text) : s_(std::move(text)) {}
+  bool done() {
+    skip();
+    return p_ >= s_.size();
+  }
+  std::string word() {
+    skip();
+    if (p_ >= s_.size()) throw std::runtime_error("E0132: gmsh: the file ends inside a section");
+    if (s_[p_] == '"') {
+      const std::size_t q = s_.find('"', p_ + 1);
+      if (q == std::string::npos) throw std::runtime_error("E0132: gmsh: an unterminated string");
+      std::string w = s_.substr(p_ + 1, q - p_ - 1);
+      p_ = q + 1;
+      return w;
+    }
+    const std::size_t b = p_;
+    while (p_ < s_.size() && !std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
+    return s_.substr(b, p_ - b);
+  }
+  long long integer() {
+    const std::string w = word();
+    char* end = nullptr;
+    errno = 0;
+    const long long v = std::strtoll(w.c_str(), &end, 10);
+    if (end == w.c_str() || *end != '\0' || errno != 0) {
+      throw std::runtime_error("E0132: gmsh: expected an integer, found '" + w + "'");
+    }
+    return v;
+  }
+  // A count or tag that must be >= lo and small enough to index with.
+  std::size_t size(long long lo = 0) {
+    const long long v = integer();
+    if (v < lo || v > INT32_MAX) throw std::runtime_error("E0132: gmsh: a count or tag out of range");
+    return static_cast<std::size_t>(v);
+  }
+  double real() {
+    const std::string w = word();
+    char* end = nullptr;
+    const double v = std::strtod(w.c_str(), &end);
+    if (end == w.c_str() || *end != '\0') throw std::runtime_error("E0132: gmsh: expected a number, found '" + w + "'");
+    return v;
+  }
+  void expect(const char* w) {
+    if (word() != w) throw std::runtime_error(std::string("E0132: gmsh: expected ") + w);
+  }
+
+ private:
+  void skip() {
+    while (p_ < s_.size() && std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
+  }
+  std::string s_;
+  std::size_t p_ = 0;
+};
+
+// Linear element types of the format: node count, dimension, and for volume elements the faces of
+// an element in the format's node ordering, outward for a positively oriented element
+// (count, then that many local nodes).
+struct Shape {
+  int nodes, dim, n_faces;
+  int face[6][5];
+};
+
+bool shape_of(long long type, Shape& out) {
+  static const Shape tet{4, 3, 4, {{3, 0, 2, 1}, {3, 0, 1, 3}, {3, 1, 2, 3}, {3, 0, 3, 2}}};
+  static const Shape hex{
+      8, 3, 6, {{4, 0, 3, 2, 1}, {4, 4, 5, 6, 7}, {4, 0, 1, 5, 4}, {4, 2, 3, 7, 6}, {4, 0, 4, 7, 3}, {4, 1, 2, 6, 5}}};
+  static const Shape prism{6, 3, 5, {{3, 0, 2, 1}, {3, 3, 4, 5}, {4, 0, 1, 4, 3}, {4, 0, 3, 5, 2}, {4, 1, 2, 5, 4}}};
+  static const Shape pyramid{5, 3, 5, {{4, 0, 3, 2, 1}, {3, 0, 1, 4}, {3, 1, 2, 4}, {3, 2, 3, 4}, {3, 3, 0, 4}}};
+  switch (type) {
+    case 15: out = Shape{1, 0, 0, {}}; return true;  // point
+    case 1: out = Shape{2, 1, 0, {}}; return true;   // line
+    case 2: out = Shape{3, 2, 0, {}}; return true;   // triangle
+    case 3: out = Shape{4, 2, 0, {}}; return true;   // quadrangle
+    case 4: out = tet; return true;
+    case 5: out = hex; return true;
+    case 6: out = prism; return true;
+    case 7: out = pyramid; return true;
+    default: return false;
+  }
+}
+
+// A boundary face known by its sorted nodes (-1 padded).
+using FaceKey = std::array<index_t, 4>;
+
+FaceKey face_key(const index_t* v, std::size_t count) {
+  FaceKey k{-1, -1, -1, -1};
+  for (std::size_t i = 0; i < count && i < 4; ++i) k[i] = v[i];
+  std::sort(k.begin(), k.begin() + static_cast<std::ptrdiff_t>(std::min<std::size_t>(count, 4)));
+  return k;
+}
+
+}  // namespace
+
+Mesh read_gmsh(const std::string& path) {
+  std::ifstream probe(path, std::ios::binary);
+  if (!probe) throw std::runtime_error("E0130: cannot open " + path);
+  probe.close();
+  Tokens t(slurp(path));
+  if (t.done() || t.word() != "$MeshFormat") throw std::runtime_error("E0131: gmsh: no $MeshFormat at the start");
+  const std::string version = t.word();
+  if (version != "4.1") throw std::runtime_error("E0133: gmsh: MSH " + version + " is not read (MSH 4.1 only)");
+  if (t.integer() != 0) throw std::runtime_error("E0133: gmsh: binary MSH is not read (ASCII only)");
+  (void)t.integer();  // data size
+  t.expect("$EndMeshFormat");
+
+  std::vector<int> surface_groups;                // 2-D physical tags
+  std::map<std::pair<int, long long>, int> physical;  // (dim, entity) -> its physical tag, 0: none
+  std::vector<double> x, y, z;
+  std::unordered_map<long long, index_t> node_of;  // node tag -> index
+  std::vector<index_t> cell_nodes;
+  std::vector<const Shape*> cell_shape;
+  std::vector<std::pair<FaceKey, int>> labelled;  // boundary elements and their physical tags
+  static Shape shapes[8];
+  bool have_nodes = false, have_elements = false;
+
+  while (!t.done()) {
+    const std::string section = t.word();
+    if (section == "$PhysicalNames") {
+      const std::size_t n = t.size();
+      for (std::size_t i = 0; i < n; ++i) {
+        const long long dim = t.integer();
+        const long long tag = t.integer();
+        (void)t.word();  // the name
+        if (tag <= 0) throw std::runtime_error("E0132: gmsh: a physical tag must be positive");
+        if (dim == 2) surface_groups.push_back(static_cast<int>(tag));
+      }
+      t.expect("$EndPhysicalNames");
+    } else if (section == "$Entities") {
+      std::size_t count[4];
+      for (auto& c : count) c = t.size();
+      for (int dim = 0; dim < 4; ++dim) {
+        for (std::size_t i = 0; i < count[dim]; ++i) {
+          const long long tag = t.integer();
+          for (int k = 0; k < (dim == 0 ? 3 : 6); ++k) (void)t.real();
+          const std::size_t np = t.size();
+          if (np > 1) throw std::runtime_error("E0135: gmsh: an entity in two physical groups");
+          const int ph = np == 1 ? static_cast<int>(t.integer()) : 0;
+          if (np == 1 && ph <= 0) throw std::runtime_error("E0132: gmsh: a physical tag must be positive");
+          physical[{dim, tag}] = ph;
+          if (dim == 2 && ph != 0) surface_groups.push_back(ph);
+          if (dim > 0) {
+            const std::size_t nb = t.size();
+            for (std::size_t k = 0; k < nb; ++k) (void)t.integer();
+          }
+        }
+      }
+      t.expect("$EndEntities");
+    } else if (section == "$Nodes") {
+      const std::size_t blocks = t.size();
+      const std::size_t n = t.size();
+      (void)t.integer();
+      (void)t.integer();
+      x.reserve(n);
+      y.reserve(n);
+      z.reserve(n);
+      node_of.reserve(n);
+      for (std::size_t b = 0; b < blocks; ++b) {
+        const long long dim = t.integer();
+        (void)t.integer();  // entity
+        const long long parametric = t.integer();
+        const std::size_t m = t.size();
+        if (x.size() + m > n) throw std::runtime_error("E0132: gmsh: $Nodes blocks hold more nodes than declared");
+        std::vector<long long> tags(m);
+        for (auto& g : tags) g = t.integer();
+        for (std::size_t i = 0; i < m; ++i) {
+          if (!node_of.emplace(tags[i], static_cast<index_t>(x.size())).second) {
+            throw std::runtime_error("E0132: gmsh: node tag " + std::to_string(tags[i]) + " is defined twice");
+          }
+          x.push_back(t.real());
+          y.push_back(t.real());
+          z.push_back(t.real());
+          for (long long k = 0; parametric != 0 && k < dim; ++k) (void)t.real();
+        }
+      }
+      if (x.size() != n) throw std::runtime_error("E0132: gmsh: $Nodes count does not match its blocks");
+      t.expect("$EndNodes");
+      have_nodes = true;
+    } else if (section == "$Elements") {
+      if (!have_nodes) throw std::runtime_error("E0132: gmsh: $Elements before $Nodes");
+      const std::size_t blocks = t.size();
+      const std::size_t n = t.size();
+      (void)t.integer();
+      (void)t.integer();
+      std::size_t read = 0;
+      std::vector<index_t> v;
+      for (std::size_t b = 0; b < blocks; ++b) {
+        const long long dim = t.integer();
+        const long long entity = t.integer();
+        const long long type = t.integer();
+        const std::size_t m = t.size();
+        read += m;
+        if (read > n) throw std::runtime_error("E0132: gmsh: $Elements blocks hold more elements than declared");
+        Shape sh;
+        if (!shape_of(type, sh)) {
+          throw std::runtime_error("E0133: gmsh: element type " + std::to_string(type) + " (linear elements only)");
+        }
+        if (sh.dim != dim) throw std::runtime_error("E0132: gmsh: an element type does not match its entity's dimension");
+        const Shape* kept = nullptr;
+        if (dim == 3) {
+          shapes[type] = sh;
+          kept = &shapes[type];
+        }
+        const auto it = physical.find({static_cast<int>(dim), entity});
+        const int ph = it == physical.end() ? 0 : it->second;
+        v.resize(static_cast<std::size_t>(sh.nodes));
+        for (std::size_t e = 0; e < m; ++e) {
+          (void)t.integer();  // element tag
+          for (auto& q : v) {
+            const long long g = t.integer();
+            const auto f = node_of.find(g);
+            if (f == node_of.end()) throw std::runtime_error("E0132: gmsh: an element refers to undefined node " + std::to_string(g));
+            q = f->second;
+          }
+          if (dim == 3) {
+            cell_nodes.insert(cell_nodes.end(), v.begin(), v.end());
+            cell_shape.push_back(kept);
+          } else if (dim == 2 && ph != 0) {
+            labelled.emplace_back(face_key(v.data(), v.size()), ph);
+          }
+        }
+      }
+      if (read != n) throw std::runtime_error("E0132: gmsh: $Elements count does not match its blocks");
+      t.expect("$EndElements");
+      have_elements = true;
+    } else if (section == "$Periodic") {
+      throw std::runtime_error("E0133: gmsh: periodic links are not read");
+    } else if (section.size() > 1 && section[0] == '$') {
+      // A section this reader does not use ($Comments, $NodeData, ...): skipped to its end.
+      const std::string end = "$End" + section.substr(1);
+      while (t.word() != end) {
+      }
+    } else {
+      throw std::runtime_error("E0131: gmsh: '" + section + "' where a section should start");
+    }
+  }
+  if (!have_elements || cell_shape.empty()) throw std::runtime_error("E0132: gmsh: no volume elements");
+
+  // Zones: the 2-D physical groups in increasing tag order, then one zone for boundary faces of no group.
+  std::sort(surface_groups.begin(), surface_groups.end());
+  surface_groups.erase(std::unique(surface_groups.begin(), surface_groups.end()), surface_groups.end());
+  std::map<FaceKey, int> zone_of_face;
+  for (const auto& [key, ph] : labelled) {
+    const int zone = static_cast<int>(std::lower_bound(surface_groups.begin(), surface_groups.end(), ph) -
+                                      surface_groups.begin());
+    zone_of_face[key] = zone;
+  }
+  const int unlabelled = static_cast<int>(surface_groups.size());
+
+  CellFaces cf;
+  const index_t n_cells = static_cast<index_t>(cell_shape.size());
+  std::size_t base = 0;
+  for (index_t c = 0; c < n_cells; ++c) {
+    const Shape& sh = *cell_shape[at(c)];
+    const index_t* v = &cell_nodes[base];
+    // Orientation: the outward-flux sum of the faces' Newell normals against the node average is
+    // positive for a positively oriented element; an inverted element has its faces reversed.
+    double cx = 0, cy = 0, cz = 0;
+    for (int i = 0; i < sh.nodes; ++i) {
+      cx += x[at(v[i])];
+      cy += y[at(v[i])];
+      cz += z[at(v[i])];
+    }
+    cx /= sh.nodes;
+    cy /= sh.nodes;
+    cz /= sh.nodes;
+    double flux = 0.0;
+    for (int f = 0; f < sh.n_faces; ++f) {
+      const int k = sh.face[f][0];
+      double nx = 0, ny = 0, nz = 0, fx = 0, fy = 0, fz = 0;
+      for (int i = 0; i < k; ++i) {
+        const std::size_t a = at(v[sh.face[f][1 + i]]), b = at(v[sh.face[f][1 + (i + 1) % k]]);
+        nx += (y[a] - y[b]) * (z[a] + z[b]);
+        ny += (z[a] - z[b]) * (x[a] + x[b]);
+        nz += (x[a] - x[b]) * (y[a] + y[b]);
+        fx += x[a];
+        fy += y[a];
+        fz += z[a];
+      }
+      flux += nx * (fx / k - cx) + ny * (fy / k - cy) + nz * (fz / k - cz);
+    }
+    const bool inverted = flux < 0.0;
+    for (int f = 0; f < sh.n_faces; ++f) {
+      const int k = sh.face[f][0];
+      for (int i = 0; i < k; ++i) cf.nodes.push_back(v[sh.face[f][1 + (inverted ? k - 1 - i : i)]]);
+      cf.off.push_back(static_cast<offset_t>(cf.nodes.size()));
+      cf.cell.push_back(c);
+    }
+    base += static_cast<std::size_t>(sh.nodes);
+  }
+  auto zone_of = [&](const index_t* v, std::size_t count) {
+    const auto it = zone_of_face.find(face_key(v, count));
+    return it == zone_of_face.end() ? unlabelled : it->second;
+  };
+  RawFaces faces = match_faces(cf, zone_of);
+  cf = CellFaces();
+  return assemble(n_cells, std::move(x), std::move(y), std::move(z), std::move(faces));
 }
 
 // Renumbering ------------------------------------------------------------------------------------

@@ -1,0 +1,191 @@
+"""Gmsh MSH 4.1 ASCII meshes (mxmesh_read_gmsh, io/src/gmsh.cpp) against the same meshes read from
+polyMesh: the reference's test meshes in both formats (tests/data/mesh/<name>/, copied under
+tests/golden/meshes/), which its own test reads to the same mesh (tests/io/test_readers.cpp:30-74);
+a hand-written file from the format's specification (test_readers.cpp:207-229); the refusals of
+test_readers.cpp:230-270."""
+
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import cases
+from autan_amd import mesh as meshes
+from autan_amd.stepper import freestream_scheme
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "meshes"
+NAMES = ("hex", "hybrid", "tet-gmsh")
+
+
+def polymesh_zones(name: str) -> list[str]:
+    """Patch names of a polyMesh boundary file, in file order (the reader's zone order)."""
+    text = (GOLDEN / name / "boundary").read_text()
+    body = text[text.index("(", text.index("}") + 1) + 1 :]
+    return re.findall(r"^\s*([A-Za-z_][\w.-]*)\s*\n\s*\{", body, flags=re.M)
+
+
+def gmsh_zones(path: Path) -> list[str]:
+    """Names of the 2-D physical groups by increasing tag (the reader's zone order)."""
+    text = path.read_text()
+    body = text[text.index("$PhysicalNames") : text.index("$EndPhysicalNames")].splitlines()[2:]
+    groups = [(int(t), n.strip('"')) for d, t, n in (line.split(None, 2) for line in body) if d == "2"]
+    return [n for _, n in sorted(groups)]
+
+
+def by_centroid(points: np.ndarray) -> np.ndarray:
+    r = np.round(points, 9)
+    return np.lexsort((r[:, 2], r[:, 1], r[:, 0]))
+
+
+def read_both(name: str):
+    return meshes.read_gmsh(GOLDEN / "gmsh" / f"{name}.ascii.msh"), meshes.read_polymesh(GOLDEN / name)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_every_format_reads_to_the_same_mesh(name):
+    g, p = read_both(name)
+    assert (g.n_cells, g.n_faces, g.n_interior) == (p.n_cells, p.n_faces, p.n_interior)
+    og, op = by_centroid(g.centroids()), by_centroid(p.centroids())
+    np.testing.assert_allclose(g.centroids()[og], p.centroids()[op], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(g.cell_volume[og], p.cell_volume[op], rtol=1e-12)
+    zg = gmsh_zones(GOLDEN / "gmsh" / f"{name}.ascii.msh")
+    zp = polymesh_zones(name)
+    assert sorted(zg) == sorted(zp) and len(zp) == 6
+    xg, xp = g.face_centroids()[g.n_interior :], p.face_centroids()[p.n_interior :]
+    ag = g.face_area[g.n_interior :]
+    ap = p.face_area[p.n_interior :]
+    for zone in zp:
+        sg = g.face_zone == zg.index(zone)
+        sp = p.face_zone == zp.index(zone)
+        assert sg.sum() == sp.sum() > 0, zone
+        fg, fp = by_centroid(xg[sg]), by_centroid(xp[sp])
+        np.testing.assert_allclose(xg[sg][fg], xp[sp][fp], rtol=0, atol=1e-13)
+        np.testing.assert_allclose(ag[sg][fg], ap[sp][fp], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_the_residual_is_the_same_on_both_readings(name):
+    """The oracle's second-order residual of one smooth field on both readings, cell by cell: the
+    same geometry, stencils and weights up to summation order."""
+    from oracle.oracle import Oracle
+
+    out = []
+    for m in read_both(name):
+        kinds = np.zeros(m.n_boundary, dtype=np.int32)
+        meshes.set_boundary_kinds(m, kinds)
+        sc = freestream_scheme(1.0, (0.4, 0.1, -0.05), 1 / 1.4, flux="roe", second_order=True, limiter=True)
+        bc = cases.state_faces_data(m, lambda x: cases._prim(cases.smooth_state(x)))
+        u = cases.smooth_state(m.centroids())
+        r = Oracle(m, sc, "fp64", kinds.astype(np.float64), bc).residual(u.reshape(-1)).reshape(5, -1)
+        out.append(r[:, by_centroid(m.centroids())])
+    scale = np.abs(out[1]).max(axis=1, keepdims=True)
+    assert np.all(scale > 0)
+    np.testing.assert_allclose(out[0] / scale, out[1] / scale, rtol=0, atol=1e-12)
+
+
+# A unit tetrahedron written by hand from the MSH 4.1 specification: its base (z = 0) in physical
+# surface 7 "base", the other three faces in 8 "sides", the volume in 9 "solid-block".
+TET = """$MeshFormat
+4.1 0 8
+$EndMeshFormat
+$PhysicalNames
+3
+2 7 "base"
+2 8 "sides"
+3 9 "solid-block"
+$EndPhysicalNames
+$Entities
+0 0 2 1
+1 0 0 0 1 1 0 1 7 0
+2 0 0 0 1 1 1 1 8 0
+1 0 0 0 1 1 1 1 9 2 1 2
+$EndEntities
+$Nodes
+1 4 1 4
+3 1 0 4
+1
+2
+3
+4
+0 0 0
+1 0 0
+0 1 0
+0 0 1
+$EndNodes
+$Elements
+3 5 1 6
+2 1 2 1
+1 1 3 2
+2 2 2 3
+2 1 2 4
+3 2 3 4
+4 1 4 3
+3 1 4 1
+6 NODES
+$EndElements
+"""
+
+
+def write(tmp_path: Path, text: str) -> Path:
+    p = tmp_path / "m.msh"
+    p.write_text(text)
+    return p
+
+
+@pytest.mark.parametrize("nodes", ["1 2 3 4", "1 3 2 4"])
+def test_a_hand_written_tetrahedron(tmp_path, nodes):
+    """One cell of volume 1/6 in either node orientation, its base alone in zone 0 and the three
+    other faces in zone 1."""
+    m = meshes.read_gmsh(write(tmp_path, TET.replace("6 NODES", f"6 {nodes}")))
+    assert (m.n_cells, m.n_faces, m.n_interior) == (1, 4, 0)
+    assert abs(m.cell_volume[0] - 1 / 6) < 1e-15
+    assert sorted(m.face_zone.tolist()) == [0, 1, 1, 1]
+    base = m.face_centroids()[m.face_zone == 0]
+    np.testing.assert_allclose(base, [[1 / 3, 1 / 3, 0.0]], atol=1e-15)
+    normal = m.face_normal.reshape(3, -1)[:, m.face_zone == 0].ravel()
+    np.testing.assert_allclose(normal, [0.0, 0.0, -1.0], atol=1e-15)  # out of the cell
+
+
+@pytest.mark.parametrize("change,code", [
+    (("4.1 0 8", "4.1 1 8"), "E0133"),
+    (("4.1 0 8", "2.2 0 8"), "E0133"),
+    (("3 1 4 1", "3 1 11 1"), "E0133"),
+    (("$EndElements\n", "$EndElements\n$Periodic\n0\n$EndPeriodic\n"), "E0133"),
+    (("2 0 0 0 1 1 1 1 8 0", "2 0 0 0 1 1 1 2 7 8 0"), "E0135"),
+    (("$MeshFormat", "$Format"), "E0131"),
+    (("6 NODES", "6 1 2 3 99"), "E0132"),
+])
+def test_refusals(tmp_path, change, code):
+    text = TET.replace(*change).replace("6 NODES", "6 1 2 3 4")
+    with pytest.raises(RuntimeError, match=code):
+        meshes.read_gmsh(write(tmp_path, text))
+
+
+def test_a_missing_file_is_e0130(tmp_path):
+    with pytest.raises(RuntimeError, match="E0130"):
+        meshes.read_gmsh(tmp_path / "none.msh")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+def test_the_device_residual_on_a_gmsh_mesh_is_the_oracles(policy):
+    """The hybrid mesh (tetrahedra, hexahedra, prisms, pyramids) read from Gmsh, every boundary a
+    `state` face: the device residual equals the oracle's bit for bit."""
+    from autan_amd.stepper import DeviceStepper
+    from oracle.oracle import Oracle
+
+    m = meshes.read_gmsh(GOLDEN / "gmsh" / "hybrid.ascii.msh", order="rcm")
+    kinds = np.zeros(m.n_boundary, dtype=np.int32)
+    meshes.set_boundary_kinds(m, kinds)
+    sc = freestream_scheme(1.0, (0.4, 0.1, -0.05), 1 / 1.4, flux="hllc", second_order=True, limiter=True)
+    bc = cases.state_faces_data(m, lambda x: cases._prim(cases.smooth_state(x)))
+    u = cases.smooth_state(m.centroids()).reshape(-1)
+    print(f"gmsh hybrid {policy}", flush=True)
+    dev = DeviceStepper(m, sc, policy, kinds.astype(np.float64), bc)
+    got = dev.residual(u)
+    dev.close()
+    want = Oracle(m, sc, policy, kinds.astype(np.float64), bc).residual(u)
+    assert np.array_equal(np.asarray(got).reshape(-1).view(np.uint64), np.asarray(want).reshape(-1).view(np.uint64))
