@@ -775,7 +775,7 @@ Mesh read_gmsh(const std::string& path) {
   std::vector<index_t> cell_nodes;
   std::vector<const Shape*> cell_shape;
   std::vector<std::pair<FaceKey, int>> labelled;  // boundary elements and their physical tags
-  static Shape shapes[8];
+  std::array<Shape, 8> shapes{};                  // the volume shapes met, by element type
   bool have_nodes = false, have_elements = false;
 
   while (!t.done()) {
@@ -862,8 +862,8 @@ Mesh read_gmsh(const std::string& path) {
         if (sh.dim != dim) throw std::runtime_error("E0132: gmsh: an element type does not match its entity's dimension");
         const Shape* kept = nullptr;
         if (dim == 3) {
-          shapes[type] = sh;
-          kept = &shapes[type];
+          shapes[static_cast<std::size_t>(type)] = sh;
+          kept = &shapes[static_cast<std::size_t>(type)];
         }
         const auto it = physical.find({static_cast<int>(dim), entity});
         const int ph = it == physical.end() ? 0 : it->second;
