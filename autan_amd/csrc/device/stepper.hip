@@ -21,6 +21,7 @@
 
 #include "../../../include/mxeuler.h"
 #include "exact.hpp"
+#include "jacobian.hpp"
 #include "kernels.hpp"
 #include "steady.hpp"
 #include "stream.hpp"
@@ -96,6 +97,8 @@ class StepperBase {
   virtual void upload(const double* u) = 0;
   virtual void download(double* u) = 0;
   virtual void residual(const double* u, double* res) = 0;
+  virtual void residual_and_jacobian(const double* u, double* res, double* dwdu, double* face_jac,
+                                     double* bnd_jac) = 0;
   virtual void upload_device(const double* u, hipStream_t caller) = 0;
   virtual void download_device(double* u, hipStream_t caller) = 0;
   virtual void residual_device(const double* u, double* res, hipStream_t caller) = 0;
@@ -229,6 +232,44 @@ class Stepper final : public StepperBase {
     residual_from_prim();
     res_.download(res, 5 * std::size_t(nc_), stream_);
     check(hipStreamSynchronize(stream_), "residual");
+  }
+
+  // Euler::residual_and_jacobian (physics/src/euler.cpp:983-991): R(u), then from the same
+  // primitives, gradients and limiter the cells' dW/dq and the faces' dF/dW (jacobian.hpp).
+  void residual_and_jacobian(const double* u, double* res, double* dwdu, double* face_jac,
+                             double* bnd_jac) override {
+    if (!planar_) throw std::invalid_argument("residual_and_jacobian: planar meshes only (warp terms are not differentiated)");
+    if (ncase() != 1) throw std::invalid_argument("residual_and_jacobian: one case");
+    Buffer<double> dj(25 * std::size_t(nc_)), fj(50 * std::size_t(std::max(ni_, 1))), bj(25 * std::size_t(std::max(nb_, 1)));
+    residual(u, res);
+    const Dev<C> d = dev();
+    launch("dwdu", [&] { k_dwdu<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, stage_.get(), dj.get()); });
+    launch("face_jacobian", [&] {
+      switch (sc_.flux) {
+        case MXEULER_ROE:
+          launch_face_jacobian<0>(d, fj.get(), bj.get());
+          break;
+        case MXEULER_HLLC:
+          launch_face_jacobian<1>(d, fj.get(), bj.get());
+          break;
+        default:
+          launch_face_jacobian<2>(d, fj.get(), bj.get());
+          break;
+      }
+    });
+    dj.download(dwdu, 25 * std::size_t(nc_), stream_);
+    if (ni_ > 0) fj.download(face_jac, 50 * std::size_t(ni_), stream_);
+    if (nb_ > 0) bj.download(bnd_jac, 25 * std::size_t(nb_), stream_);
+    check(hipStreamSynchronize(stream_), "residual_and_jacobian");
+  }
+
+  template <int FLUX>
+  void launch_face_jacobian(const Dev<C>& d, double* fj, double* bj) {
+    if (second_) {
+      k_face_jacobian<C, FLUX, true><<<blocks(nf_), kBlock, 0, stream_>>>(d, fj, bj);
+    } else {
+      k_face_jacobian<C, FLUX, false><<<blocks(nf_), kBlock, 0, stream_>>>(d, fj, bj);
+    }
   }
 
   // Device-resident variants: the arrays are device memory, the work is ordered after what the
@@ -1696,6 +1737,16 @@ int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len) {
 
 int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len) {
   return guarded(err, err_len, [&] { s->impl->residual(u, res); });
+}
+
+int mxeuler_residual_and_jacobian(mxeuler_stepper* s, const double* u, double* res, double* dwdu, double* face_jac,
+                                  double* bnd_jac, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (u == nullptr || res == nullptr || dwdu == nullptr || face_jac == nullptr || bnd_jac == nullptr) {
+      throw std::invalid_argument("residual_and_jacobian: null array");
+    }
+    s->impl->residual_and_jacobian(u, res, dwdu, face_jac, bnd_jac);
+  });
 }
 
 int mxeuler_upload_device(mxeuler_stepper* s, const double* u, void* stream, char* err, size_t err_len) {

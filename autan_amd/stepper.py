@@ -246,6 +246,22 @@ class DeviceStepper:
         self._call(self.lib.mxeuler_residual, _p(u.ctypes.data), _p(res.ctypes.data))
         return res
 
+    def residual_and_jacobian(self, u: np.ndarray) -> dict[str, np.ndarray]:
+        """Euler::residual_and_jacobian: R(u) (5, n) and the approximate Jacobian's parts in FP64,
+        the reconstruction's gradients and limiter held fixed: dwdu (5, 5, n) dW/dq per cell,
+        face_jac (2, 5, 5, n_interior) dF/dW_L and dF/dW_R per interior face (area-weighted),
+        bnd_jac (5, 5, n_boundary) dF/dW_L per boundary face. Planar meshes."""
+        u = self._state(u)
+        n, ni, nb = self.n, self.mesh.n_interior, self.mesh.n_boundary
+        res = np.empty(5 * n)
+        dwdu = np.empty(25 * n)
+        fj = np.empty(50 * max(ni, 1))
+        bj = np.empty(25 * max(nb, 1))
+        self._call(self.lib.mxeuler_residual_and_jacobian, _p(u.ctypes.data), _p(res.ctypes.data),
+                   _p(dwdu.ctypes.data), _p(fj.ctypes.data), _p(bj.ctypes.data))
+        return {"res": res.reshape(5, n), "dwdu": dwdu.reshape(5, 5, n),
+                "face_jac": fj[: 50 * ni].reshape(2, 5, 5, ni), "bnd_jac": bj[: 25 * nb].reshape(5, 5, nb)}
+
     def upload_device(self, u: int, stream: int = 0) -> None:
         """The state from device memory: `u` the address of 5 x n FP64 values on the stepper's GPU,
         allocated through the HIP runtime this library links (/opt/rocm's libamdhip64); the copy

@@ -101,6 +101,18 @@ int mxeuler_download(mxeuler_stepper* s, double* u, char* err, size_t err_len);
 /* DeviceStepper::residual: R(u) into res (both 5 x n_cells, FP64); the stepper's state is kept. */
 int mxeuler_residual(mxeuler_stepper* s, const double* u, double* res, char* err, size_t err_len);
 
+/* Euler::residual_and_jacobian (physics/src/euler.cpp:983-991; kernels/euler.py:434-440, 854-860,
+ * 1051-1057): R(u) into res as mxeuler_residual, and the approximate Jacobian's parts at u, with the
+ * reconstruction's gradients and limiter held fixed, in FP64:
+ * dwdu (25 x n_cells) dW/dq of each cell, q the stored state, entry (i, j) at (5 i + j) n_cells + c;
+ * face_jac (50 x n_interior) dF/dW_L then dF/dW_R of each interior face (owner to neighbour,
+ * area-weighted), entry (k, i, j) at (25 k + 5 i + j) n_interior + f; bnd_jac (25 x n_boundary)
+ * dF/dW_L of each boundary face (through its boundary state; slip and wall faces the pressure
+ * flux's, as the reference's Jacobian loops take them). The residual's Jacobian is then, per cell, sum over its faces of
+ * +-J_side D_cell (Euler::assemble, physics/src/euler.cpp:993-1053). Planar meshes, one case. */
+int mxeuler_residual_and_jacobian(mxeuler_stepper* s, const double* u, double* res, double* dwdu, double* face_jac,
+                                  double* bnd_jac, char* err, size_t err_len);
+
 /* Device-resident upload, download and residual (the hipGraph path's variant of the three calls
  * above): u and res are device memory on the stepper's GPU (5 x n_cells FP64). The work runs on
  * the stepper's stream after everything `stream` holds at the call, and `stream` receives an
