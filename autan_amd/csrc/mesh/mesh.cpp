@@ -709,12 +709,47 @@ class Tokens {
     if (word() != w) throw std::runtime_error(std::string("E0132: gmsh: expected ") + w);
   }
 
+  // The typed fields of $Entities, $Nodes and $Elements: text tokens, or in a binary file's
+  // sections the raw little-endian values (int for tags, dimensions and types, size_t for counts
+  // and node and element tags, double for coordinates).
+  void set_binary(bool on) { binary_ = on; }
+  long long i32() { return binary_ ? static_cast<long long>(raw<std::int32_t>()) : integer(); }
+  long long u64() {
+    if (!binary_) return integer();
+    const std::uint64_t v = raw<std::uint64_t>();
+    if (v > static_cast<std::uint64_t>(INT64_MAX)) throw std::runtime_error("E0132: gmsh: a count or tag out of range");
+    return static_cast<long long>(v);
+  }
+  std::size_t count(long long lo = 0) {
+    const long long v = u64();
+    if (v < lo || v > INT32_MAX) throw std::runtime_error("E0132: gmsh: a count or tag out of range");
+    return static_cast<std::size_t>(v);
+  }
+  double f64() { return binary_ ? raw<double>() : real(); }
+  // After a binary block: the newline before its end marker.
+  void skip_newline() {
+    if (p_ < s_.size() && s_[p_] == '\n') ++p_;
+  }
+  std::int32_t probe_int() {
+    std::int32_t v = raw<std::int32_t>();
+    return v;
+  }
+
  private:
+  template <class T>
+  T raw() {
+    if (p_ + sizeof(T) > s_.size()) throw std::runtime_error("E0132: gmsh: the file ends inside a section");
+    T v;
+    std::memcpy(&v, s_.data() + p_, sizeof(T));
+    p_ += sizeof(T);
+    return v;
+  }
   void skip() {
     while (p_ < s_.size() && std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
   }
   std::string s_;
   std::size_t p_ = 0;
+  bool binary_ = false;
 };
 
 // Linear element types of the format: node count, dimension, and for volume elements the faces of
@@ -764,8 +799,15 @@ Mesh read_gmsh(const std::string& path) {
   if (t.done() || t.word() != "$MeshFormat") throw std::runtime_error("E0131: gmsh: no $MeshFormat at the start");
   const std::string version = t.word();
   if (version != "4.1") throw std::runtime_error("E0133: gmsh: MSH " + version + " is not read (MSH 4.1 only)");
-  if (t.integer() != 0) throw std::runtime_error("E0133: gmsh: binary MSH is not read (ASCII only)");
-  (void)t.integer();  // data size
+  const long long file_type = t.integer();
+  if (file_type != 0 && file_type != 1) throw std::runtime_error("E0132: gmsh: file type 0 (ASCII) or 1 (binary)");
+  const bool binary = file_type == 1;
+  if (t.integer() != 8) throw std::runtime_error("E0133: gmsh: a data size other than 8");
+  if (binary) {
+    // The integer 1 written in binary after the header line tells the byte order.
+    t.skip_newline();
+    if (t.probe_int() != 1) throw std::runtime_error("E0133: gmsh: a big-endian binary file");
+  }
   t.expect("$EndMeshFormat");
 
   std::vector<int> surface_groups;                // 2-D physical tags
@@ -791,68 +833,76 @@ Mesh read_gmsh(const std::string& path) {
       }
       t.expect("$EndPhysicalNames");
     } else if (section == "$Entities") {
+      t.skip_newline();
+      t.set_binary(binary);
       std::size_t count[4];
-      for (auto& c : count) c = t.size();
+      for (auto& c : count) c = t.count();
       for (int dim = 0; dim < 4; ++dim) {
         for (std::size_t i = 0; i < count[dim]; ++i) {
-          const long long tag = t.integer();
-          for (int k = 0; k < (dim == 0 ? 3 : 6); ++k) (void)t.real();
-          const std::size_t np = t.size();
+          const long long tag = t.i32();
+          for (int k = 0; k < (dim == 0 ? 3 : 6); ++k) (void)t.f64();
+          const std::size_t np = t.count();
           if (np > 1) throw std::runtime_error("E0135: gmsh: an entity in two physical groups");
-          const int ph = np == 1 ? static_cast<int>(t.integer()) : 0;
+          const int ph = np == 1 ? static_cast<int>(t.i32()) : 0;
           if (np == 1 && ph <= 0) throw std::runtime_error("E0132: gmsh: a physical tag must be positive");
           physical[{dim, tag}] = ph;
           if (dim == 2 && ph != 0) surface_groups.push_back(ph);
           if (dim > 0) {
-            const std::size_t nb = t.size();
-            for (std::size_t k = 0; k < nb; ++k) (void)t.integer();
+            const std::size_t nb = t.count();
+            for (std::size_t k = 0; k < nb; ++k) (void)t.i32();
           }
         }
       }
+      t.set_binary(false);
       t.expect("$EndEntities");
     } else if (section == "$Nodes") {
-      const std::size_t blocks = t.size();
-      const std::size_t n = t.size();
-      (void)t.integer();
-      (void)t.integer();
+      t.skip_newline();
+      t.set_binary(binary);
+      const std::size_t blocks = t.count();
+      const std::size_t n = t.count();
+      (void)t.u64();
+      (void)t.u64();
       x.reserve(n);
       y.reserve(n);
       z.reserve(n);
       node_of.reserve(n);
       for (std::size_t b = 0; b < blocks; ++b) {
-        const long long dim = t.integer();
-        (void)t.integer();  // entity
-        const long long parametric = t.integer();
-        const std::size_t m = t.size();
+        const long long dim = t.i32();
+        (void)t.i32();  // entity
+        const long long parametric = t.i32();
+        const std::size_t m = t.count();
         if (x.size() + m > n) throw std::runtime_error("E0132: gmsh: $Nodes blocks hold more nodes than declared");
         std::vector<long long> tags(m);
-        for (auto& g : tags) g = t.integer();
+        for (auto& g : tags) g = t.u64();
         for (std::size_t i = 0; i < m; ++i) {
           if (!node_of.emplace(tags[i], static_cast<index_t>(x.size())).second) {
             throw std::runtime_error("E0132: gmsh: node tag " + std::to_string(tags[i]) + " is defined twice");
           }
-          x.push_back(t.real());
-          y.push_back(t.real());
-          z.push_back(t.real());
-          for (long long k = 0; parametric != 0 && k < dim; ++k) (void)t.real();
+          x.push_back(t.f64());
+          y.push_back(t.f64());
+          z.push_back(t.f64());
+          for (long long k = 0; parametric != 0 && k < dim; ++k) (void)t.f64();
         }
       }
       if (x.size() != n) throw std::runtime_error("E0132: gmsh: $Nodes count does not match its blocks");
+      t.set_binary(false);
       t.expect("$EndNodes");
       have_nodes = true;
     } else if (section == "$Elements") {
       if (!have_nodes) throw std::runtime_error("E0132: gmsh: $Elements before $Nodes");
-      const std::size_t blocks = t.size();
-      const std::size_t n = t.size();
-      (void)t.integer();
-      (void)t.integer();
+      t.skip_newline();
+      t.set_binary(binary);
+      const std::size_t blocks = t.count();
+      const std::size_t n = t.count();
+      (void)t.u64();
+      (void)t.u64();
       std::size_t read = 0;
       std::vector<index_t> v;
       for (std::size_t b = 0; b < blocks; ++b) {
-        const long long dim = t.integer();
-        const long long entity = t.integer();
-        const long long type = t.integer();
-        const std::size_t m = t.size();
+        const long long dim = t.i32();
+        const long long entity = t.i32();
+        const long long type = t.i32();
+        const std::size_t m = t.count();
         read += m;
         if (read > n) throw std::runtime_error("E0132: gmsh: $Elements blocks hold more elements than declared");
         Shape sh;
@@ -869,9 +919,9 @@ Mesh read_gmsh(const std::string& path) {
         const int ph = it == physical.end() ? 0 : it->second;
         v.resize(static_cast<std::size_t>(sh.nodes));
         for (std::size_t e = 0; e < m; ++e) {
-          (void)t.integer();  // element tag
+          (void)t.u64();  // element tag
           for (auto& q : v) {
-            const long long g = t.integer();
+            const long long g = t.u64();
             const auto f = node_of.find(g);
             if (f == node_of.end()) throw std::runtime_error("E0132: gmsh: an element refers to undefined node " + std::to_string(g));
             q = f->second;
@@ -885,6 +935,7 @@ Mesh read_gmsh(const std::string& path) {
         }
       }
       if (read != n) throw std::runtime_error("E0132: gmsh: $Elements count does not match its blocks");
+      t.set_binary(false);
       t.expect("$EndElements");
       have_elements = true;
     } else if (section == "$Periodic") {

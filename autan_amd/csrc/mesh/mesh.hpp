@@ -152,10 +152,10 @@ Mesh bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double
                  int prism_layers, int band_lo, int band_hi);
 /// OpenFOAM polyMesh (ASCII) directory: points, faces, owner, neighbour, boundary.
 Mesh read_polymesh(const std::string& dir);
-/// Gmsh MSH 4.1 ASCII file (io/src/gmsh.cpp): linear tetrahedra, hexahedra, prisms and pyramids;
-/// boundary zones the 2-D physical groups in increasing tag order, then one zone for boundary
-/// faces in no group; cells in file order. Binary files, other versions and periodic links are
-/// refused.
+/// Gmsh MSH 4.1 file, ASCII or little-endian binary (io/src/gmsh.cpp): linear tetrahedra,
+/// hexahedra, prisms and pyramids; boundary zones the 2-D physical groups in increasing tag order,
+/// then one zone for boundary faces in no group; cells in file order. Other versions and periodic
+/// links are refused.
 Mesh read_gmsh(const std::string& path);
 
 }  // namespace mx

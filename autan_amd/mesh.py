@@ -182,7 +182,7 @@ def bump_hybrid(nx, ny, nz, lx=3.0, ly=1.0, lz=1.0, h=0.1, w=0.2, x0=1.5, prism_
 
 
 def read_gmsh(path: str | os.PathLike, order: str | None = None, kinds=None) -> Mesh:
-    """A Gmsh MSH 4.1 ASCII file; boundary zone k is the k-th 2-D physical group by tag."""
+    """A Gmsh MSH 4.1 file (ASCII or binary); boundary zone k is the k-th 2-D physical group by tag."""
     lib = _lib.mesh()
     return _build(lib.mxmesh_read_gmsh, os.fsencode(str(path)), order=order, kinds=kinds)
 

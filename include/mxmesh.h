@@ -37,10 +37,10 @@ int mxmesh_bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, 
                        int prism_layers, int band_lo, int band_hi, mxmesh** out, char* err, size_t err_len);
 /* An ASCII OpenFOAM polyMesh directory. */
 int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_len);
-/* A Gmsh MSH 4.1 ASCII file (io/src/gmsh.cpp; docs/contracts/mesh.md, Readers): linear volume
+/* A Gmsh MSH 4.1 file, ASCII or binary (io/src/gmsh.cpp; docs/contracts/mesh.md, Readers): linear volume
  * elements; boundary zone k is the k-th 2-D physical group by increasing tag, then one zone for
  * boundary faces in no group. Errors carry the reference's reader codes (E0130 missing file, E0131
- * not a mesh file, E0132 malformed, E0133 unsupported: binary, MSH 2.2, higher-order elements,
+ * not a mesh file, E0132 malformed, E0133 unsupported: MSH 2.2, big-endian, higher-order elements,
  * periodic links; E0135 an entity in two physical groups). */
 int mxmesh_read_gmsh(const char* path, mxmesh** out, char* err, size_t err_len);
 void mxmesh_destroy(mxmesh* m);

@@ -1,5 +1,5 @@
-"""Gmsh MSH 4.1 ASCII meshes (mxmesh_read_gmsh, io/src/gmsh.cpp) against the same meshes read from
-polyMesh: the reference's test meshes in both formats (tests/data/mesh/<name>/, copied under
+"""Gmsh MSH 4.1 meshes, ASCII and binary (mxmesh_read_gmsh, io/src/gmsh.cpp), against the same meshes
+read from polyMesh: the reference's test meshes in both formats (tests/data/mesh/<name>/, copied under
 tests/golden/meshes/), which its own test reads to the same mesh (tests/io/test_readers.cpp:30-74);
 a hand-written file from the format's specification (test_readers.cpp:207-229); the refusals of
 test_readers.cpp:230-270."""
@@ -40,13 +40,14 @@ def by_centroid(points: np.ndarray) -> np.ndarray:
     return np.lexsort((r[:, 2], r[:, 1], r[:, 0]))
 
 
-def read_both(name: str):
-    return meshes.read_gmsh(GOLDEN / "gmsh" / f"{name}.ascii.msh"), meshes.read_polymesh(GOLDEN / name)
+def read_both(name: str, encoding: str = "ascii"):
+    return meshes.read_gmsh(GOLDEN / "gmsh" / f"{name}.{encoding}.msh"), meshes.read_polymesh(GOLDEN / name)
 
 
+@pytest.mark.parametrize("encoding", ["ascii", "binary"])
 @pytest.mark.parametrize("name", NAMES)
-def test_every_format_reads_to_the_same_mesh(name):
-    g, p = read_both(name)
+def test_every_format_reads_to_the_same_mesh(name, encoding):
+    g, p = read_both(name, encoding)
     assert (g.n_cells, g.n_faces, g.n_interior) == (p.n_cells, p.n_faces, p.n_interior)
     og, op = by_centroid(g.centroids()), by_centroid(p.centroids())
     np.testing.assert_allclose(g.centroids()[og], p.centroids()[op], rtol=0, atol=1e-13)
@@ -64,6 +65,18 @@ def test_every_format_reads_to_the_same_mesh(name):
         fg, fp = by_centroid(xg[sg]), by_centroid(xp[sp])
         np.testing.assert_allclose(xg[sg][fg], xp[sp][fp], rtol=0, atol=1e-13)
         np.testing.assert_allclose(ag[sg][fg], ap[sp][fp], rtol=1e-12)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_binary_and_ascii_files_hold_the_same_mesh(name):
+    """The same topology and zones; the points equal to the ASCII file's printed digits (Gmsh
+    writes coordinates there with fewer digits than a double round-trips)."""
+    a = meshes.read_gmsh(GOLDEN / "gmsh" / f"{name}.ascii.msh")
+    b = meshes.read_gmsh(GOLDEN / "gmsh" / f"{name}.binary.msh")
+    for k in ("owner", "neighbour", "face_node_offsets", "face_nodes", "face_zone", "cell_face_offsets"):
+        assert np.array_equal(a.arrays[k], b.arrays[k]), k
+    for k in ("x", "y", "z"):
+        np.testing.assert_allclose(a.arrays[k], b.arrays[k], rtol=0, atol=1e-15)
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -150,7 +163,8 @@ def test_a_hand_written_tetrahedron(tmp_path, nodes):
 
 
 @pytest.mark.parametrize("change,code", [
-    (("4.1 0 8", "4.1 1 8"), "E0133"),
+    (("4.1 0 8", "4.1 0 4"), "E0133"),
+    (("4.1 0 8", "4.1 2 8"), "E0132"),
     (("4.1 0 8", "2.2 0 8"), "E0133"),
     (("3 1 4 1", "3 1 11 1"), "E0133"),
     (("$EndElements\n", "$EndElements\n$Periodic\n0\n$EndPeriodic\n"), "E0133"),
