@@ -87,8 +87,9 @@ def _problem(doc) -> str | None:
 def document_bytes(*, oracle: str, mesh_sha256: str, centroids, values) -> bytes:
     """The file's bytes for these rows: the contract's keys in its order, numbers as Python writes
     a float, ", " and ": " separators, ASCII with escapes (py/autan/initial_state.py:104-114)."""
-    c = [[float(x) for x in row] for row in np.asarray(centroids, dtype=np.float64).reshape(-1, 3)]
-    v = [[float(x) for x in row] for row in np.asarray(values, dtype=np.float64)]
+    # Row by row, as the SDK converts them: a row of the wrong length is refused, not reshaped.
+    c = [[float(x) for x in row] for row in centroids]
+    v = [[float(x) for x in row] for row in values]
     doc = {"schema": SCHEMA, "oracle": oracle, "mesh_sha256": mesh_sha256, "variables": list(VARIABLES),
            "cells": len(c), "centroids": c, "values": v}
     why = _problem(doc)

@@ -65,6 +65,13 @@ def test_write_refuses_what_the_contract_refuses(tmp_path, values):
         ist.write(tmp_path / "s.json", oracle="o", mesh_sha256="0" * 64, centroids=centroids, values=values)
 
 
+def test_write_refuses_transposed_rows(tmp_path):
+    centroids = np.zeros((3, 4))  # 3 x n instead of n x 3
+    with pytest.raises(ist.InitialStateError, match="E0508"):
+        ist.write(tmp_path / "s.json", oracle="o", mesh_sha256="0" * 64, centroids=centroids,
+                  values=[[1.2, 30.0, 0.0, 0.0, 1e5]] * 4)
+
+
 def _doc(n=4) -> dict:
     return {"schema": ist.SCHEMA, "oracle": "o", "mesh_sha256": "0" * 64, "variables": list(ist.VARIABLES),
             "cells": n, "centroids": [[0.1 * i, 0.0, 0.0] for i in range(n)],
