@@ -63,3 +63,38 @@ def test_create_validates_before_touching_a_device():
     with pytest.raises(RuntimeError, match="owner out of range"):
         DeviceStepper(bad.mesh, bad.scheme, "fp64", bad.bc_kind, bad.bc)
     assert np.isfinite(case.u).all()
+
+
+def test_the_status_tells_refused_input_from_a_device_failure():
+    """mxeuler_create returns 1 for input it refuses (the reference's std::invalid_argument) and 2
+    for a device failure (std::runtime_error, E1101), here a host without a GPU."""
+    import os
+
+    import cases
+    from autan_amd import stepper as st
+
+    lib = _lib.euler()
+    bad = cases.fixture_case("hex", "roe")
+    bad.mesh.arrays["owner"] = bad.mesh.owner.copy()
+    bad.mesh.arrays["owner"][3] = 10**6
+    calls = []
+    real = lib.mxeuler_create
+
+    def spy(*args):
+        rc = real(*args)
+        calls.append(rc)
+        return rc
+
+    lib.mxeuler_create = spy
+    try:
+        with pytest.raises(RuntimeError, match="owner out of range"):
+            st.DeviceStepper(bad.mesh, bad.scheme, "fp64", bad.bc_kind, bad.bc)
+        if not os.path.exists("/dev/kfd"):
+            good = cases.tets_case(3)
+            with pytest.raises(RuntimeError, match="E1101"):
+                st.DeviceStepper(good.mesh, good.scheme, "fp64", good.bc_kind, good.bc, good.source)
+    finally:
+        lib.mxeuler_create = real
+    assert calls[0] == 1
+    if len(calls) > 1:
+        assert calls[1] == 2
