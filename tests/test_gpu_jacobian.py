@@ -186,3 +186,18 @@ def test_warped_meshes_are_refused():
     with pytest.raises(RuntimeError, match="planar meshes only"):
         dev.residual_and_jacobian(c.u)
     dev.close()
+
+
+def test_the_implicit_solve_converges_in_few_solves():
+    """First order HLLC on tets with sources and `state` boundaries (the Jacobian exact there):
+    pseudo-transient continuation on the device's residual and Jacobian drops every residual 8
+    orders within 60 linear solves."""
+    from autan_amd import implicit
+
+    c = cases.tets_case(4, flux="hllc", second_order=False)
+    print("implicit solve tets 4 hllc first order", flush=True)
+    dev = DeviceStepper(c.mesh, c.scheme, "fp64", c.bc_kind, c.bc, c.source)
+    out = implicit.solve(dev, c.u, max_iterations=60, residual_orders=8.0, cfl0=10.0)
+    dev.close()
+    print(f"{out.iterations} solves, residual drop {np.log10(out.residuals[0] / out.residuals[-1])}", flush=True)
+    assert out.converged, out.residuals[-1] / out.residuals[0]
