@@ -31,6 +31,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace oracle {
@@ -57,6 +58,15 @@ using A = double;  // accumulation and residual type of both policies
 
 // ------------------------------------------------------------------------------------------ math
 // The generator's select semantics (frontend.py:438-442, 486-489).
+// The bodies' square root, exponential and logarithm, called unqualified so that the dual numbers
+// of the Jacobian (below) take their own.
+inline float osqrt(float x) { return std::sqrt(x); }
+inline double osqrt(double x) { return std::sqrt(x); }
+inline float oexp(float x) { return std::exp(x); }
+inline double oexp(double x) { return std::exp(x); }
+inline float olog(float x) { return std::log(x); }
+inline double olog(double x) { return std::log(x); }
+
 template <class C> inline C sabs(C x) { return x >= C(0) ? x : -x; }
 template <class C> inline C smin(C a, C b) { return a <= b ? a : b; }
 template <class C> inline C smax(C a, C b) { return a >= b ? a : b; }
@@ -114,14 +124,14 @@ void roe(const C* wl, const C* wr, const C* n, C g, C pr, C* out) {
   const C rl = wl[0], rr = wr[0];
   const C vl[3] = {wl[1], wl[2], wl[3]}, vr[3] = {wr[1], wr[2], wr[3]};
   const C hl = total_enthalpy(wl, g, pr), hr = total_enthalpy(wr, g, pr);
-  const C s = std::sqrt(rr / rl);
+  const C s = osqrt(rr / rl);
   const C rho = s * rl;
   C v[3];
   for (int i = 0; i < 3; ++i) v[i] = (vl[i] + s * vr[i]) / (C(1.0) + s);
   const C h = (hl + s * hr) / (C(1.0) + s);
   const C vn = dot3(v, n);
   const C q2 = dot3(v, v);
-  const C c = std::sqrt((g - C(1.0)) * (h - C(0.5) * q2));
+  const C c = osqrt((g - C(1.0)) * (h - C(0.5) * q2));
   const C dp = wr[4] - wl[4];
   const C drho = rr - rl;
   const C dv[3] = {vr[0] - vl[0], vr[1] - vl[1], vr[2] - vl[2]};
@@ -178,13 +188,13 @@ void hllc(const C* wl, const C* wr, const C* n, C g, C pr, C* out) {
   const C rl = wl[0], rr = wr[0];
   const C vl[3] = {wl[1], wl[2], wl[3]}, vr[3] = {wr[1], wr[2], wr[3]};
   const C vnl = dot3(vl, n), vnr = dot3(vr, n);
-  const C cl = std::sqrt(g * (wl[4] + pr) / rl);
-  const C cr = std::sqrt(g * (wr[4] + pr) / rr);
-  const C s = std::sqrt(rr / rl);
+  const C cl = osqrt(g * (wl[4] + pr) / rl);
+  const C cr = osqrt(g * (wr[4] + pr) / rr);
+  const C s = osqrt(rr / rl);
   C v[3];
   for (int i = 0; i < 3; ++i) v[i] = (vl[i] + s * vr[i]) / (C(1.0) + s);
   const C h = (total_enthalpy(wl, g, pr) + s * total_enthalpy(wr, g, pr)) / (C(1.0) + s);
-  const C c = std::sqrt((g - C(1.0)) * (h - C(0.5) * dot3(v, v)));
+  const C c = osqrt((g - C(1.0)) * (h - C(0.5) * dot3(v, v)));
   const C vn = dot3(v, n);
   const C sl = smin(vnl - cl, vn - c);
   const C sr = smax(vnr + cr, vn + c);
@@ -244,8 +254,8 @@ void ausm_up(const C* wl, const C* wr, const C* n, C g, C minf, C pr, C* out) {
   const C vl[3] = {wl[1], wl[2], wl[3]}, vr[3] = {wr[1], wr[2], wr[3]};
   const C vnl = dot3(vl, n), vnr = dot3(vr, n);
   const C hl = total_enthalpy(wl, g, pr), hr = total_enthalpy(wr, g, pr);
-  const C csl = std::sqrt(C(2.0) * (g - C(1.0)) / (g + C(1.0)) * hl);
-  const C csr = std::sqrt(C(2.0) * (g - C(1.0)) / (g + C(1.0)) * hr);
+  const C csl = osqrt(C(2.0) * (g - C(1.0)) / (g + C(1.0)) * hl);
+  const C csr = osqrt(C(2.0) * (g - C(1.0)) / (g + C(1.0)) * hr);
   const C al = csl * csl / smax(csl, vnl);
   const C ar = csr * csr / smax(csr, -vnr);
   const C a = smin(al, ar);
@@ -253,7 +263,7 @@ void ausm_up(const C* wl, const C* wr, const C* n, C g, C minf, C pr, C* out) {
   const C mr = vnr / a;
   const C mbar2 = C(0.5) * (vnl * vnl + vnr * vnr) / (a * a);
   const C mo2 = smin(C(1.0), smax(mbar2, minf * minf));
-  const C mo = std::sqrt(mo2);
+  const C mo = osqrt(mo2);
   const C fa = mo * (C(2.0) - mo);
   const C alpha = C(0.1875) * (C(-4.0) + C(5.0) * fa * fa);
   const C rho = C(0.5) * (rl + rr);
@@ -278,21 +288,21 @@ template <class C>
 void farfield_state(const C* wi, const C* wf, const C* n, C g, C* w) {
   const C vi[3] = {wi[1], wi[2], wi[3]}, vf[3] = {wf[1], wf[2], wf[3]};
   const C vni = dot3(vi, n), vnf = dot3(vf, n);
-  const C ci = std::sqrt(g * wi[4] / wi[0]);
-  const C cf = std::sqrt(g * wf[4] / wf[0]);
+  const C ci = osqrt(g * wi[4] / wi[0]);
+  const C cf = osqrt(g * wf[4] / wf[0]);
   const C rp = vni + C(2.0) * ci / (g - C(1.0));
   const C rm = vnf - C(2.0) * cf / (g - C(1.0));
   const C vnb = C(0.5) * (rp + rm);
   const C cb = C(0.25) * (g - C(1.0)) * (rp - rm);
   C ent, vt[3];
   if (vnb < C(0.0)) {
-    ent = wf[4] / std::exp(g * std::log(wf[0]));
+    ent = wf[4] / oexp(g * olog(wf[0]));
     for (int i = 0; i < 3; ++i) vt[i] = vf[i] - vnf * n[i];
   } else {
-    ent = wi[4] / std::exp(g * std::log(wi[0]));
+    ent = wi[4] / oexp(g * olog(wi[0]));
     for (int i = 0; i < 3; ++i) vt[i] = vi[i] - vni * n[i];
   }
-  const C rb = std::exp(std::log(cb * cb / (g * ent)) / (g - C(1.0)));
+  const C rb = oexp(olog(cb * cb / (g * ent)) / (g - C(1.0)));
   if (vni >= ci) {
     for (int k = 0; k < 5; ++k) w[k] = wi[k];
   } else if (vnf <= -cf) {
@@ -310,14 +320,14 @@ void inlet_state(const C* wi, const C* data, const C* n, C g, C rgas, C* w) {
   const C p0 = data[0], t0 = data[1];
   const C gm = g - C(1.0);
   const C vi[3] = {wi[1], wi[2], wi[3]};
-  const C ci = std::sqrt(g * wi[4] / wi[0]);
+  const C ci = osqrt(g * wi[4] / wi[0]);
   const C r = dot3(vi, n) + C(2.0) * ci / gm;
   const C h0 = g * rgas * t0 / gm;
   const C disc = smax((gm + C(2.0)) * h0 - C(0.5) * gm * r * r, C(0.0));
-  const C cb = gm * (r + std::sqrt(disc)) / (gm + C(2.0));
+  const C cb = gm * (r + osqrt(disc)) / (gm + C(2.0));
   const C vnb = smin(r - C(2.0) * cb / gm, C(0.0));
   const C tb = t0 - C(0.5) * gm * vnb * vnb / (g * rgas);
-  const C pb = p0 * std::exp(g / gm * std::log(tb / t0));
+  const C pb = p0 * oexp(g / gm * olog(tb / t0));
   const C rb = pb / (rgas * tb);
   w[0] = rb;
   w[1] = vnb * n[0];
@@ -330,7 +340,7 @@ void inlet_state(const C* wi, const C* data, const C* n, C g, C rgas, C* w) {
 template <class C>
 void outlet_state(const C* wi, const C* data, const C* n, C g, C* w) {
   const C vi[3] = {wi[1], wi[2], wi[3]};
-  const C ci = std::sqrt(g * wi[4] / wi[0]);
+  const C ci = osqrt(g * wi[4] / wi[0]);
   const C pb = data[4];
   const C dp = wi[4] - pb;
   const C rb = wi[0] - dp / (ci * ci);
@@ -452,6 +462,140 @@ void numerical_flux(int flux, const C* wl, const C* wr, const C* n, C g, C minf,
     hllc(wl, wr, n, g, pref, out);
   } else {
     ausm_up(wl, wr, n, g, minf, pref, out);
+  }
+}
+
+// ------------------------------------------------------------------------------ Jacobians
+// The approximate Jacobian's blocks of the reference's implicit solvers (kernels/euler.py:434-440,
+// 854-860, 1051-1057): the same bodies evaluated on dual numbers, a value and its derivatives
+// along N directions, each operation applying its chain rule; a select takes the derivative of the
+// branch its value takes.
+template <class C, int N>
+struct Dual {
+  C v;
+  C d[N];
+  Dual() = default;
+  template <class T, std::enable_if_t<std::is_arithmetic_v<T>, int> = 0>
+  Dual(T x) : v(static_cast<C>(x)) {
+    for (int i = 0; i < N; ++i) d[i] = C(0);
+  }
+  friend Dual operator+(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v + b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
+    return r;
+  }
+  friend Dual operator-(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v - b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
+    return r;
+  }
+  friend Dual operator-(const Dual& a) {
+    Dual r;
+    r.v = -a.v;
+    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+    return r;
+  }
+  friend Dual operator*(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v * b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+  }
+  friend Dual operator/(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v / b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) / b.v;
+    return r;
+  }
+  friend bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+  friend bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+  friend bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+  friend bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+};
+
+template <class C, int N>
+Dual<C, N> osqrt(const Dual<C, N>& a) {
+  Dual<C, N> r;
+  r.v = osqrt(a.v);
+  const C h = C(0.5) / r.v;
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+  return r;
+}
+
+template <class C, int N>
+Dual<C, N> oexp(const Dual<C, N>& a) {
+  Dual<C, N> r;
+  r.v = oexp(a.v);
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * r.v;
+  return r;
+}
+
+template <class C, int N>
+Dual<C, N> olog(const Dual<C, N>& a) {
+  Dual<C, N> r;
+  r.v = olog(a.v);
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / a.v;
+  return r;
+}
+
+// dF/dW_L then dF/dW_R of one face (50 values, entry (k, i, j) at 25 k + 5 i + j), not area-weighted.
+template <class C>
+void face_jacobian(int flux, const C* wl, const C* wr, const C* n, C g, C minf, C pref, C* out) {
+  using D = Dual<C, 5>;
+  const D nd[3] = {D(n[0]), D(n[1]), D(n[2])};
+  for (int side = 0; side < 2; ++side) {
+    D a[5], b[5], o[5];
+    for (int k = 0; k < 5; ++k) {
+      a[k] = D(wl[k]);
+      b[k] = D(wr[k]);
+      (side == 0 ? a : b)[k].d[k] = C(1);
+    }
+    numerical_flux<D>(flux, a, b, nd, D(g), D(minf), D(pref), o);
+    for (int i = 0; i < 5; ++i) {
+      for (int j = 0; j < 5; ++j) out[25 * side + 5 * i + j] = o[i].d[j];
+    }
+  }
+}
+
+// dF/dW_f of one boundary face (25 values): slip and wall faces the pressure flux's, the other
+// kinds the flux against boundary_value of W_f, through it (kernels/euler.py:868-940).
+template <class C>
+void boundary_jacobian(int flux, const C* wf, C kind, const C* data, const C* n, C g, C rgas, C minf, C pref, C* out) {
+  for (int q = 0; q < 25; ++q) out[q] = C(0);
+  if ((kind > C(1.5) && kind < C(2.5)) || kind > C(4.5)) {
+    for (int k = 0; k < 3; ++k) out[5 * (1 + k) + 4] = n[k];
+    return;
+  }
+  using D = Dual<C, 5>;
+  const D nd[3] = {D(n[0]), D(n[1]), D(n[2])};
+  D a[5], dd[5], wb[5], o[5];
+  for (int k = 0; k < 5; ++k) {
+    a[k] = D(wf[k]);
+    a[k].d[k] = C(1);
+    dd[k] = D(data[k]);
+  }
+  boundary_value<D>(a, D(kind), dd, nd, D(g), D(rgas), D(pref), wb);
+  numerical_flux<D>(flux, a, wb, nd, D(g), D(minf), D(pref), o);
+  for (int i = 0; i < 5; ++i) {
+    for (int j = 0; j < 5; ++j) out[5 * i + j] = o[i].d[j];
+  }
+}
+
+// dW/dq of one cell (25 values, entry (i, j) at 5 i + j), q the stored state.
+template <class C>
+void dwdu(const C* dq, const C* ur, C g, C* out) {
+  using D = Dual<C, 5>;
+  D q[5], r[5], w[5];
+  for (int k = 0; k < 5; ++k) {
+    q[k] = D(dq[k]);
+    q[k].d[k] = C(1);
+    r[k] = D(ur[k]);
+  }
+  to_primitive<D>(q, r, D(g), w);
+  for (int i = 0; i < 5; ++i) {
+    for (int j = 0; j < 5; ++j) out[5 * i + j] = w[i].d[j];
   }
 }
 
@@ -1195,6 +1339,59 @@ void oracle_flux(int policy, int flux, const double* wl, const double* wr, const
   for (int k = 0; k < 3; ++k) m[k] = float(n[k]);
   oracle::numerical_flux<float>(flux, a, b, m, float(g), float(minf), float(pref), o);
   for (int k = 0; k < 5; ++k) out[k] = o[k];
+}
+
+// The Jacobian blocks of one face, one boundary face and one cell, in the policy's compute type.
+extern "C++" {
+template <class C>
+static void copy_in(const double* x, C* y, int n) {
+  for (int k = 0; k < n; ++k) y[k] = static_cast<C>(x[k]);
+}
+
+template <class C>
+static void jacobian_of(int what, int flux, const double* a, const double* b, double kind, const double* n, double g,
+                        double rgas, double minf, double pref, double* out) {
+  C x[5], y[5], m[3], o[50];
+  copy_in(a, x, 5);
+  if (b != nullptr) copy_in(b, y, 5);
+  if (n != nullptr) copy_in(n, m, 3);
+  int count = 25;
+  if (what == 0) {
+    oracle::face_jacobian<C>(flux, x, y, m, C(g), C(minf), C(pref), o);
+    count = 50;
+  } else if (what == 1) {
+    oracle::boundary_jacobian<C>(flux, x, C(kind), y, m, C(g), C(rgas), C(minf), C(pref), o);
+  } else {
+    oracle::dwdu<C>(x, y, C(g), o);
+  }
+  for (int q = 0; q < count; ++q) out[q] = o[q];
+}
+}  // extern "C++"
+
+void oracle_face_jacobian(int policy, int flux, const double* wl, const double* wr, const double* n, double g,
+                          double minf, double pref, double* out) {
+  if (policy == 0) {
+    jacobian_of<double>(0, flux, wl, wr, 0.0, n, g, 0.0, minf, pref, out);
+  } else {
+    jacobian_of<float>(0, flux, wl, wr, 0.0, n, g, 0.0, minf, pref, out);
+  }
+}
+
+void oracle_boundary_jacobian(int policy, int flux, const double* wf, double kind, const double* data,
+                              const double* n, double g, double rgas, double minf, double pref, double* out) {
+  if (policy == 0) {
+    jacobian_of<double>(1, flux, wf, data, kind, n, g, rgas, minf, pref, out);
+  } else {
+    jacobian_of<float>(1, flux, wf, data, kind, n, g, rgas, minf, pref, out);
+  }
+}
+
+void oracle_dwdu(int policy, const double* dq, const double* ur, double g, double* out) {
+  if (policy == 0) {
+    jacobian_of<double>(2, 0, dq, ur, 0.0, nullptr, g, 0.0, 0.0, 0.0, out);
+  } else {
+    jacobian_of<float>(2, 0, dq, ur, 0.0, nullptr, g, 0.0, 0.0, 0.0, out);
+  }
 }
 
 void oracle_physical_flux(const double* w, const double* n, double g, double pref, double* out) {

@@ -80,6 +80,9 @@ def lib() -> ctypes.CDLL:
     L.oracle_physical_flux.argtypes = [_p, _p, _d, _d, _p]
     L.oracle_boundary_value.argtypes = [_p, _d, _p, _p, _d, _d, _d, _p]
     L.oracle_to_primitive.argtypes = [_p, _p, _d, _p]
+    L.oracle_face_jacobian.argtypes = [ctypes.c_int, ctypes.c_int, _p, _p, _p, _d, _d, _d, _p]
+    L.oracle_boundary_jacobian.argtypes = [ctypes.c_int, ctypes.c_int, _p, _d, _p, _p, _d, _d, _d, _d, _p]
+    L.oracle_dwdu.argtypes = [ctypes.c_int, _p, _p, _d, _p]
     L.oracle_warp.argtypes = [_p] * 6 + [_d, _d, _p]
     L.oracle_set_threads.argtypes = [ctypes.c_int]
     L.oracle_face_force.argtypes = [_p, _p, _p, ctypes.c_int64, _p]
@@ -134,6 +137,34 @@ def to_primitive(dq, ur, gamma=1.4) -> np.ndarray:
     dq, ur = _a(dq), _a(ur)
     lib().oracle_to_primitive(dq.ctypes.data, ur.ctypes.data, gamma, out.ctypes.data)
     return out
+
+
+def face_jacobian(kind: int, wl, wr, n, gamma=1.4, minf=0.5, pref=0.0, policy=0) -> np.ndarray:
+    """(2, 5, 5): dF/dW_L and dF/dW_R of one face's flux (not area-weighted), forward-mode through
+    the flux body in the policy's compute type."""
+    out = np.empty(50)
+    wl, wr, n = _a(wl), _a(wr), _a(n)
+    lib().oracle_face_jacobian(policy, kind, wl.ctypes.data, wr.ctypes.data, n.ctypes.data, gamma, minf, pref,
+                               out.ctypes.data)
+    return out.reshape(2, 5, 5)
+
+
+def boundary_jacobian(kind: int, wf, bc_kind, data, n, gamma=1.4, rgas=287.0, minf=0.5, pref=0.0, policy=0):
+    """(5, 5): dF/dW_f of one boundary face, through boundary_value (the pressure flux's for slip
+    and wall faces)."""
+    out = np.empty(25)
+    wf, data, n = _a(wf), _a(data), _a(n)
+    lib().oracle_boundary_jacobian(policy, kind, wf.ctypes.data, float(bc_kind), data.ctypes.data, n.ctypes.data,
+                                   gamma, rgas, minf, pref, out.ctypes.data)
+    return out.reshape(5, 5)
+
+
+def dwdu(dq, ur, gamma=1.4, policy=0) -> np.ndarray:
+    """(5, 5): dW/dq of one cell's primitives, q the stored state."""
+    out = np.empty(25)
+    dq, ur = _a(dq), _a(ur)
+    lib().oracle_dwdu(policy, dq.ctypes.data, ur.ctypes.data, gamma, out.ctypes.data)
+    return out.reshape(5, 5)
 
 
 def venkat(d, dmax, dmin, eps2) -> float:
