@@ -43,17 +43,17 @@ def fd(fun, w, j, h):
 
 def check_block(got, fun, w, tol=2e-6):
     """got (5 x 5, dF/dW) against central differences of fun at w, with two steps: a column whose
-    two differences disagree straddles a switch of the flux (a kink) and is not judged."""
+    two differences disagree straddles a switch of the flux (a kink) and is not checked."""
     scale = max(np.abs(got).max(), 1e-300)
-    judged = 0
+    checked = 0
     for j in range(5):
         h = 1e-6 * max(1.0, abs(w[j]))
         a, b = fd(fun, w, j, h), fd(fun, w, j, 0.25 * h)
         if np.abs(a - b).max() > 1e-5 * scale:
             continue
-        judged += 1
+        checked += 1
         assert np.abs(got[:, j] - b).max() <= tol * scale, (j, got[:, j], b)
-    return judged
+    return checked
 
 
 CASES = [
@@ -91,15 +91,15 @@ def test_face_blocks_are_the_flux_derivatives(name, flux, second):
         w = prim[:, cell]
         return reconstruct(w, grad[:, cell], phi[:, cell], r, pref) if second else w
 
-    judged = total = 0
+    checked = total = 0
     for f in range(ni):
         o, nb = m.owner[f], m.neighbour[f]
         wl, wr = side(o, rl[:, f]), side(nb, rr[:, f])
         n = nrm[:, f]
         fl = lambda w: area[f] * orc.flux(kind, w, wr, n, sc.gamma, sc.mach_ref, pref)  # noqa: E731
         fr = lambda w: area[f] * orc.flux(kind, wl, w, n, sc.gamma, sc.mach_ref, pref)  # noqa: E731
-        judged += check_block(parts["face_jac"][0, :, :, f], fl, wl)
-        judged += check_block(parts["face_jac"][1, :, :, f], fr, wr)
+        checked += check_block(parts["face_jac"][0, :, :, f], fl, wl)
+        checked += check_block(parts["face_jac"][1, :, :, f], fr, wr)
         total += 10
     kinds = np.asarray(c.bc_kind, dtype=np.int64)
     for b in range(m.n_boundary):
@@ -118,9 +118,9 @@ def test_face_blocks_are_the_flux_derivatives(name, flux, second):
             wb = orc.boundary_value(w, k, data, n, sc.gamma, sc.gas_constant, pref)
             return a * orc.flux(kind, w, wb, n, sc.gamma, sc.mach_ref, pref)
 
-        judged += check_block(got, fb, wf)
+        checked += check_block(got, fb, wf)
         total += 5
-    assert judged >= 0.97 * total, (judged, total)
+    assert checked >= 0.97 * total, (checked, total)
 
 
 def test_dwdu_is_the_primitives_derivative():
@@ -162,7 +162,7 @@ def test_first_order_matrix_is_the_residuals_jacobian(flux):
     av = jacobian.to_components(a @ jacobian.to_cells(v, n), n)
     scale = np.abs(d4).max(axis=1, keepdims=True)
     # A cell whose two differences disagree has a face state within the step of a switch of the
-    # flux (HLLC's wave-speed min / max): a kink, not judged.
+    # flux (HLLC's wave-speed min / max): a kink, not checked.
     smooth = np.all(np.abs(d - d4) <= 1e-6 * scale, axis=0)
     assert smooth.mean() >= 0.99, smooth.mean()
     np.testing.assert_allclose((av / scale)[:, smooth], (d4 / scale)[:, smooth], rtol=0, atol=2e-6)
