@@ -173,6 +173,24 @@ def test_monitors_and_refusals():
         history.record(1, [1.0] * 5, time=0.5, provisional=True)
 
 
+def test_an_oracle_steady_runs_history():
+    """The oracle's steady run (first-order start, then second order) written as history lines:
+    one per completed iteration, its residuals' bits, the first-order start's lines provisional."""
+    import cases
+    from autan_amd import steady
+    from oracle.oracle import Oracle
+
+    case = cases.tets_case(3)
+    ora = Oracle(case.mesh, case.scheme, "fp64", case.bc_kind, case.bc, case.source)
+    run = steady.solve_with(cases.OracleTarget(ora, case.u), max_iterations=150, residual_orders=8.0)
+    records = [json.loads(x) for x in history.steady_lines(run)]
+    assert not _problems(records, steady=True)
+    assert len(records) == run.iterations and records[-1]["iteration"] == run.iterations
+    assert sum(r["provisional"] for r in records) == run.first_order_steps > 0
+    got = np.array([[r["residuals"][k] for k in history.EQUATIONS] for r in records])
+    assert np.array_equal(got.view(np.uint64), run.history.view(np.uint64))
+
+
 @pytest.mark.gpu
 def test_a_device_steady_runs_history_is_the_oracle_runs():
     """The device's steady run (first-order start, fp32+acc64 level rule) written as history.jsonl
