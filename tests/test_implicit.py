@@ -19,11 +19,21 @@ def p_ref(sc) -> float:
     return (sc.gamma - 1.0) * (r[4] - 0.5 * (r[1] * r[1] + r[2] * r[2] + r[3] * r[3]) / r[0])
 
 
+def reconstruct(w, g, lim, r, pref):
+    """kernels/euler.py:619-627: W + phi (grad W . r), the cell value where the density or pressure
+    would not stay positive."""
+    d = np.array([g[3 * k] * r[0] + g[3 * k + 1] * r[1] + g[3 * k + 2] * r[2] for k in range(5)])
+    f = w + lim * d
+    return f if (f[0] > 0.0 and f[4] + pref > 0.0) else w
+
+
 class OracleStepper:
-    """residual_and_jacobian and spectral_radius of the oracle, first order, FP64."""
+    """residual_and_jacobian and spectral_radius of the oracle in FP64: the Jacobian blocks at the
+    reconstructed face states, the gradients and limiter held fixed."""
 
     def __init__(self, case, flux):
         self.case, self.mesh, self.kind = case, case.mesh, FLUXES[flux]
+        self.second = case.scheme.second_order
         self.ora = orc.Oracle(case.mesh, case.scheme, "fp64", case.bc_kind, case.bc, case.source)
 
     def residual_and_jacobian(self, u):
@@ -31,16 +41,22 @@ class OracleStepper:
         n, ni, nb = m.n_cells, m.n_interior, m.n_boundary
         ref = np.asarray(sc.reference)
         res = self.ora.residual(np.asarray(u).reshape(-1)).reshape(5, n)
+        arr = self.ora.arrays()
+        w, grad, phi = arr["prim"].reshape(5, n), arr["grad"].reshape(15, n), arr["phi"].reshape(5, n)
         q = np.asarray(u).reshape(5, n) - ref[:, None]
-        w = np.array([orc.to_primitive(q[:, c], ref, sc.gamma) for c in range(n)]).T
         dwdu = np.stack([orc.dwdu(q[:, c], ref, sc.gamma) for c in range(n)], axis=2)
         nrm, area, pr = m.face_normal.reshape(3, -1), m.face_area, p_ref(sc)
-        fj = np.stack([area[f] * orc.face_jacobian(self.kind, w[:, m.owner[f]], w[:, m.neighbour[f]], nrm[:, f],
-                                                   sc.gamma, sc.mach_ref, pr) for f in range(ni)], axis=3)
+        rl, rr = m.r_left.reshape(3, -1), m.r_right.reshape(3, -1)
+
+        def side(c, r):
+            return reconstruct(w[:, c], grad[:, c], phi[:, c], r, pr) if self.second else w[:, c]
+
+        fj = np.stack([area[f] * orc.face_jacobian(self.kind, side(m.owner[f], rl[:, f]), side(m.neighbour[f], rr[:, f]),
+                                                   nrm[:, f], sc.gamma, sc.mach_ref, pr) for f in range(ni)], axis=3)
         bc = np.asarray(self.case.bc, dtype=np.float64).reshape(5, nb)
-        bj = np.stack([area[ni + b] * orc.boundary_jacobian(self.kind, w[:, m.owner[ni + b]], self.case.bc_kind[b],
-                                                            bc[:, b], nrm[:, ni + b], sc.gamma, sc.gas_constant,
-                                                            sc.mach_ref, pr) for b in range(nb)], axis=2)
+        bj = np.stack([area[ni + b] * orc.boundary_jacobian(self.kind, side(m.owner[ni + b], rl[:, ni + b]),
+                                                            self.case.bc_kind[b], bc[:, b], nrm[:, ni + b], sc.gamma,
+                                                            sc.gas_constant, sc.mach_ref, pr) for b in range(nb)], axis=2)
         return {"res": res, "dwdu": dwdu, "face_jac": fj, "bnd_jac": bj}
 
     def spectral_radius(self):
@@ -71,3 +87,11 @@ def test_the_implicit_solve_converges_on_the_oracle():
     c = cases.tets_case(3, flux="hllc", second_order=False)
     out = implicit.solve(OracleStepper(c, "hllc"), c.u, max_iterations=40, residual_orders=8.0, cfl0=10.0)
     assert out.converged and out.iterations <= 20, (out.iterations, out.residuals[-1] / out.residuals[0])
+
+
+def test_the_implicit_solve_converges_at_second_order_with_the_approximate_jacobian():
+    """Second order: the Jacobian holds the reconstruction's gradients and limiter fixed, so the
+    solve is no longer Newton's; it still drops 8 orders in a few dozen solves (25 here)."""
+    c = cases.tets_case(3, flux="hllc", second_order=True)
+    out = implicit.solve(OracleStepper(c, "hllc"), c.u, max_iterations=40, residual_orders=8.0, cfl0=10.0)
+    assert out.converged, (out.iterations, out.residuals[-1] / out.residuals[0])
