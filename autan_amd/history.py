@@ -25,7 +25,6 @@ EQUATIONS = ("density", "momentum-x", "momentum-y", "momentum-z", "energy")
 
 _ALPHA, _GAMMA = -60, -32  # the exponent window of the scaled numbers (the paper's alpha, gamma)
 _POW_MIN, _POW_STEP = -300, 8  # decimal exponents of the cached powers: -300, -292, ..., 324
-_M64 = (1 << 64) - 1
 
 
 @lru_cache(maxsize=None)
@@ -177,6 +176,40 @@ def _value(v) -> str:
     if isinstance(v, str):
         return json.dumps(v)
     return format_double(float(v))
+
+
+def forces(force, moment, ref_area: float | None = None, ref_length: float | None = None, density: float = 1.0,
+           speed: float = 1.0, aoa: float = 0.0, sideslip: float = 0.0) -> dict:
+    """A forces monitor's record (runtime/src/run.cpp:1590-1614; docs/contracts/results.md, Forces
+    monitor): fx fy fz mx my mz, then with a reference area cd cs cl (the force on the drag
+    direction (cos a cos b, -sin b, sin a cos b), on the side direction lift x drag and on the lift
+    direction (-sin a, 0, cos a), over q A with q = 1/2 rho V^2), and with a reference length also
+    cmx cmy cmz (the moment over q A L); aoa and sideslip in radians, as the runtime holds them."""
+    f = [float(x) for x in force]
+    m = [float(x) for x in moment]
+    r = {"fx": f[0], "fy": f[1], "fz": f[2], "mx": m[0], "my": m[1], "mz": m[2]}
+    if ref_area is None:
+        return r
+    q = 0.5 * density * speed * speed
+    a, b = aoa, sideslip
+    drag = (math.cos(a) * math.cos(b), -math.sin(b), math.sin(a) * math.cos(b))
+    lift = (-math.sin(a), 0.0, math.cos(a))
+    side = (lift[1] * drag[2] - lift[2] * drag[1], lift[2] * drag[0] - lift[0] * drag[2],
+            lift[0] * drag[1] - lift[1] * drag[0])
+
+    def dot(u, v):
+        return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]
+
+    qa = q * ref_area
+    r["cd"] = dot(f, drag) / qa
+    r["cs"] = dot(f, side) / qa
+    r["cl"] = dot(f, lift) / qa
+    if ref_length is not None:
+        qal = qa * ref_length
+        r["cmx"] = m[0] / qal
+        r["cmy"] = m[1] / qal
+        r["cmz"] = m[2] / qal
+    return r
 
 
 def record(iteration: int, residuals, time: float | None = None, monitors: dict | None = None,

@@ -161,6 +161,23 @@ def test_unsteady_lines_accumulate_time_in_fp64():
     assert "provisional" not in records[0]
 
 
+def test_the_forces_monitor():
+    """Keys in the runtime's order; at zero incidence the coefficients are the force components
+    over q A (drag x, side y, lift z); at 90 degrees of incidence lift is -x and drag z."""
+    f, m = (3.0, -1.5, 6.0), (0.5, 2.0, -4.0)
+    assert list(history.forces(f, m)) == ["fx", "fy", "fz", "mx", "my", "mz"]
+    r = history.forces(f, m, ref_area=2.0, ref_length=0.5, density=1.2, speed=10.0)
+    assert list(r) == ["fx", "fy", "fz", "mx", "my", "mz", "cd", "cs", "cl", "cmx", "cmy", "cmz"]
+    qa = 0.5 * 1.2 * 100.0 * 2.0
+    assert (r["cd"], r["cs"], r["cl"]) == (3.0 / qa, -1.5 / qa, 6.0 / qa)
+    assert (r["cmx"], r["cmy"], r["cmz"]) == (0.5 / (qa * 0.5), 2.0 / (qa * 0.5), -4.0 / (qa * 0.5))
+    r = history.forces(f, m, ref_area=2.0, density=1.2, speed=10.0, aoa=math.pi / 2)
+    assert "cmx" not in r
+    assert abs(r["cl"] + 3.0 / qa) < 1e-15 and abs(r["cd"] - 6.0 / qa) < 1e-15 and abs(r["cs"] + 1.5 / qa) < 1e-15
+    line = history.record(5, [1e-3] * 5, provisional=False, monitors={"forces": r})
+    assert '"monitors":{"forces":{"fx":3.0,"fy":-1.5,"fz":6.0,"mx":0.5,"my":2.0,"mz":-4.0,"cd":' in line
+
+
 def test_monitors_and_refusals():
     line = history.record(3, [1.0] * 5, provisional=False,
                           monitors={"forces": {"fx": 1.25, "fy": 0.0, "fz": -3e-7}})
