@@ -726,6 +726,14 @@ class Tokens {
     return static_cast<std::size_t>(v);
   }
   double f64() { return binary_ ? raw<double>() : real(); }
+  // A declared count of items that take at least `text_bytes` (ASCII) or `binary_bytes` each must
+  // fit in what is left of the file: checked before anything is allocated for them.
+  void check_fits(std::size_t n, std::size_t text_bytes, std::size_t binary_bytes) const {
+    const std::size_t each = binary_ ? binary_bytes : text_bytes;
+    if (n > (s_.size() - std::min(p_, s_.size())) / each) {
+      throw std::runtime_error("E0132: gmsh: a count larger than the file can hold");
+    }
+  }
   // After a binary block: the newline before its end marker.
   void skip_newline() {
     if (p_ < s_.size() && s_[p_] == '\n') ++p_;
@@ -836,7 +844,10 @@ Mesh read_gmsh(const std::string& path) {
       t.skip_newline();
       t.set_binary(binary);
       std::size_t count[4];
-      for (auto& c : count) c = t.count();
+      for (auto& c : count) {
+        c = t.count();
+        t.check_fits(c, 10, 36);  // a tag, coordinates and a physical count at least
+      }
       for (int dim = 0; dim < 4; ++dim) {
         for (std::size_t i = 0; i < count[dim]; ++i) {
           const long long tag = t.i32();
@@ -862,6 +873,8 @@ Mesh read_gmsh(const std::string& path) {
       const std::size_t n = t.count();
       (void)t.u64();
       (void)t.u64();
+      t.check_fits(blocks, 8, 20);
+      t.check_fits(n, 8, 32);  // a tag and three coordinates
       x.reserve(n);
       y.reserve(n);
       z.reserve(n);
@@ -872,6 +885,7 @@ Mesh read_gmsh(const std::string& path) {
         const long long parametric = t.i32();
         const std::size_t m = t.count();
         if (x.size() + m > n) throw std::runtime_error("E0132: gmsh: $Nodes blocks hold more nodes than declared");
+        t.check_fits(m, 8, 32);
         std::vector<long long> tags(m);
         for (auto& g : tags) g = t.u64();
         for (std::size_t i = 0; i < m; ++i) {
@@ -896,6 +910,8 @@ Mesh read_gmsh(const std::string& path) {
       const std::size_t n = t.count();
       (void)t.u64();
       (void)t.u64();
+      t.check_fits(blocks, 8, 20);
+      t.check_fits(n, 4, 16);  // a tag and a node at least
       std::size_t read = 0;
       std::vector<index_t> v;
       for (std::size_t b = 0; b < blocks; ++b) {

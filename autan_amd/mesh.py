@@ -78,7 +78,7 @@ class _Handle:
 def _call(fn, *args) -> None:
     err = ctypes.create_string_buffer(1024)
     if fn(*args, err, ctypes.c_size_t(len(err))) != 0:
-        raise RuntimeError(err.value.decode())
+        raise RuntimeError(err.value.decode(errors="replace"))
 
 
 def _build(create, *args, order: str | None = "rcm", kinds: np.ndarray | None = None) -> Mesh:

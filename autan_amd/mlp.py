@@ -37,7 +37,7 @@ class MLP:
                                    ctypes.c_int32(outputs), ctypes.c_uint64(seed), ctypes.byref(h), err,
                                    ctypes.c_size_t(len(err)))
         if rc != 0:
-            raise RuntimeError(err.value.decode())
+            raise RuntimeError(err.value.decode(errors="replace"))
         self.h = h
         self.norm = [np.zeros(inputs, np.float32), np.ones(inputs, np.float32), np.zeros(outputs, np.float32),
                      np.ones(outputs, np.float32)]
@@ -54,7 +54,7 @@ class MLP:
         err = ctypes.create_string_buffer(1024)
         rc = fn(self.h, *args, err, ctypes.c_size_t(len(err)))
         if rc != 0:
-            raise RuntimeError(err.value.decode())
+            raise RuntimeError(err.value.decode(errors="replace"))
 
     @property
     def param_count(self) -> int:

@@ -120,7 +120,7 @@ def solve(stepper, max_iterations: int = 10000, residual_orders: float = 8.0, cf
     rc = lib.mxeuler_steady(stepper.handle, ctypes.byref(opts), ctypes.byref(res), ctypes.c_void_p(hist.ctypes.data),
                             ctypes.c_void_p(prov.ctypes.data), err, ctypes.c_size_t(len(err)))
     if rc != 0:
-        raise RuntimeError(err.value.decode())
+        raise RuntimeError(err.value.decode(errors="replace"))
     return _finish(res, hist, prov)
 
 
@@ -174,5 +174,5 @@ def solve_with(target, max_iterations: int = 10000, residual_orders: float = 8.0
     if failure:
         raise failure[0]
     if rc != 0:
-        raise RuntimeError(err.value.decode())
+        raise RuntimeError(err.value.decode(errors="replace"))
     return _finish(res, hist, prov)

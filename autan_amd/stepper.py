@@ -133,7 +133,7 @@ def _err() -> ctypes.Array:
 
 def _check(rc: int, err) -> None:
     if rc != 0:
-        raise RuntimeError(err.value.decode())
+        raise RuntimeError(err.value.decode(errors="replace"))
 
 
 class DeviceStepper:

@@ -203,3 +203,27 @@ def test_the_device_residual_on_a_gmsh_mesh_is_the_oracles(policy):
     dev.close()
     want = Oracle(m, sc, policy, kinds.astype(np.float64), bc).residual(u)
     assert np.array_equal(np.asarray(got).reshape(-1).view(np.uint64), np.asarray(want).reshape(-1).view(np.uint64))
+
+
+@pytest.mark.parametrize("encoding", ["ascii", "binary"])
+def test_damaged_files_are_errors_not_crashes(tmp_path, encoding):
+    """Truncations and single-byte changes of a test mesh: each either reads or raises the reader's
+    error (the counts a file declares are checked against its size before anything is allocated,
+    tests/io/test_readers.cpp:386-426)."""
+    data = (GOLDEN / "gmsh" / f"hex.{encoding}.msh").read_bytes()
+    rng = np.random.default_rng(12)
+    path = tmp_path / "d.msh"
+    outcomes = {"read": 0, "refused": 0}
+    cases_ = [data[: int(k)] for k in rng.integers(0, len(data), 150)]
+    for k in rng.integers(0, len(data), 150):
+        b = bytearray(data)
+        b[int(k)] = int(rng.integers(0, 256))
+        cases_.append(bytes(b))
+    for d in cases_:
+        path.write_bytes(d)
+        try:
+            meshes.read_gmsh(path)
+            outcomes["read"] += 1
+        except RuntimeError:
+            outcomes["refused"] += 1
+    assert outcomes["refused"] > 0 and sum(outcomes.values()) == 300
