@@ -557,83 +557,139 @@ namespace {
 
 std::string slurp(const std::string& path) {
   std::ifstream in(path, std::ios::binary);
-  if (!in) throw std::runtime_error("cannot open " + path);
+  if (!in) throw std::runtime_error("E0130: cannot open " + path);
   std::ostringstream s;
   s << in.rdbuf();
   return s.str();
 }
 
-// The text after the FoamFile header and comments, from the list count on.
-std::size_t list_start(const std::string& s) {
-  std::size_t p = s.find("FoamFile");
-  p = p == std::string::npos ? 0 : s.find('}', p) + 1;
-  while (p < s.size()) {
-    if (s.compare(p, 2, "//") == 0) {
-      p = s.find('\n', p);
-    } else if (s.compare(p, 2, "/*") == 0) {
-      p = s.find("*/", p) + 2;
-    } else if (std::isdigit(static_cast<unsigned char>(s[p]))) {
-      return p;
-    } else {
-      ++p;
+// A cursor over one polyMesh file's text, every read checked: a damaged file is an error
+// (E0132), never a read past its end.
+class FoamText {
+ public:
+  FoamText(std::string text, std::string name) : s_(std::move(text)), name_(std::move(name)) {
+    // The list starts after the FoamFile header and comments, at its count.
+    std::size_t p = s_.find("FoamFile");
+    p = p == std::string::npos ? 0 : s_.find('}', p);
+    if (p == std::string::npos) fail("an unterminated FoamFile header");
+    p_ = p == 0 ? 0 : p + 1;
+    while (p_ < s_.size()) {
+      if (s_.compare(p_, 2, "//") == 0) {
+        p_ = s_.find('\n', p_);
+        if (p_ == std::string::npos) p_ = s_.size();
+      } else if (s_.compare(p_, 2, "/*") == 0) {
+        const std::size_t q = s_.find("*/", p_);
+        if (q == std::string::npos) fail("an unterminated comment");
+        p_ = q + 2;
+      } else if (std::isdigit(static_cast<unsigned char>(s_[p_]))) {
+        return;
+      } else {
+        ++p_;
+      }
     }
+    fail("no list");
   }
-  throw std::runtime_error("polyMesh: no list found");
-}
+  [[noreturn]] void fail(const std::string& why) const {
+    throw std::runtime_error("E0132: polyMesh " + name_ + ": " + why);
+  }
+  long integer() {
+    const char* b = s_.c_str() + p_;
+    char* e = nullptr;
+    errno = 0;
+    const long v = std::strtol(b, &e, 10);
+    if (e == b || errno != 0) fail("expected an integer");
+    p_ += static_cast<std::size_t>(e - b);
+    return v;
+  }
+  double real() {
+    const char* b = s_.c_str() + p_;
+    char* e = nullptr;
+    const double v = std::strtod(b, &e);
+    if (e == b) fail("expected a number");
+    p_ += static_cast<std::size_t>(e - b);
+    return v;
+  }
+  void open(char c) {
+    const std::size_t q = s_.find(c, p_);
+    if (q == std::string::npos) fail(std::string("expected '") + c + "'");
+    p_ = q + 1;
+  }
+  // A list count, no larger than the items of at least `each` characters that the rest holds.
+  std::size_t count(std::size_t each) {
+    const long n = integer();
+    if (n < 0 || static_cast<std::size_t>(n) > (s_.size() - p_) / each) fail("a count larger than the file can hold");
+    return static_cast<std::size_t>(n);
+  }
 
-std::vector<long> labels(const std::string& path) {
-  const std::string s = slurp(path);
-  std::size_t p = list_start(s);
-  char* end = nullptr;
-  const long n = std::strtol(s.c_str() + p, &end, 10);
-  const char* q = std::strchr(end, '(') + 1;
-  std::vector<long> out(static_cast<std::size_t>(n));
-  for (long i = 0; i < n; ++i) out[static_cast<std::size_t>(i)] = std::strtol(q, const_cast<char**>(&q), 10);
+ private:
+  std::string s_;
+  std::string name_;
+  std::size_t p_ = 0;
+};
+
+std::vector<long> labels(const std::string& dir, const char* file) {
+  FoamText t(slurp(dir + "/" + file), file);
+  const std::size_t n = t.count(2);
+  t.open('(');
+  std::vector<long> out(n);
+  for (auto& v : out) v = t.integer();
   return out;
 }
 
 }  // namespace
 
 Mesh read_polymesh(const std::string& dir) {
-  // points
   std::vector<double> x, y, z;
   {
-    const std::string s = slurp(dir + "/points");
-    std::size_t p = list_start(s);
-    char* end = nullptr;
-    const long n = std::strtol(s.c_str() + p, &end, 10);
-    const char* q = std::strchr(end, '(') + 1;
-    for (long i = 0; i < n; ++i) {
-      q = std::strchr(q, '(') + 1;
-      x.push_back(std::strtod(q, const_cast<char**>(&q)));
-      y.push_back(std::strtod(q, const_cast<char**>(&q)));
-      z.push_back(std::strtod(q, const_cast<char**>(&q)));
-      q = std::strchr(q, ')') + 1;
+    FoamText t(slurp(dir + "/points"), "points");
+    const std::size_t n = t.count(7);  // "(a b c)"
+    t.open('(');
+    x.reserve(n);
+    y.reserve(n);
+    z.reserve(n);
+    for (std::size_t i = 0; i < n; ++i) {
+      t.open('(');
+      x.push_back(t.real());
+      y.push_back(t.real());
+      z.push_back(t.real());
+      t.open(')');
     }
   }
+  const long np = static_cast<long>(x.size());
   RawFaces f;
   {
-    const std::string s = slurp(dir + "/faces");
-    std::size_t p = list_start(s);
-    char* end = nullptr;
-    const long n = std::strtol(s.c_str() + p, &end, 10);
-    const char* q = std::strchr(end, '(') + 1;
-    for (long i = 0; i < n; ++i) {
-      const long k = std::strtol(q, const_cast<char**>(&q), 10);
-      q = std::strchr(q, '(') + 1;
-      for (long j = 0; j < k; ++j) f.nodes.push_back(static_cast<index_t>(std::strtol(q, const_cast<char**>(&q), 10)));
-      q = std::strchr(q, ')') + 1;
+    FoamText t(slurp(dir + "/faces"), "faces");
+    const std::size_t n = t.count(8);  // "3(a b c)"
+    t.open('(');
+    for (std::size_t i = 0; i < n; ++i) {
+      const long k = t.integer();
+      if (k < 3 || k > 1024) t.fail("a face of " + std::to_string(k) + " nodes");
+      t.open('(');
+      for (long j = 0; j < k; ++j) {
+        const long v = t.integer();
+        if (v < 0 || v >= np) t.fail("a face node out of range");
+        f.nodes.push_back(static_cast<index_t>(v));
+      }
+      t.open(')');
       f.off.push_back(static_cast<offset_t>(f.nodes.size()));
     }
   }
-  const auto own = labels(dir + "/owner");
-  const auto nei = labels(dir + "/neighbour");
+  const auto own = labels(dir, "owner");
+  const auto nei = labels(dir, "neighbour");
   const std::size_t nf = f.off.size() - 1;
-  if (own.size() != nf) throw std::runtime_error("polyMesh: owner and faces differ in length");
+  if (own.size() != nf) throw std::runtime_error("E0132: polyMesh: owner and faces differ in length");
+  if (nei.size() > nf) throw std::runtime_error("E0132: polyMesh: more neighbours than faces");
   long n_cells = 0;
   for (long c : own) n_cells = std::max(n_cells, c + 1);
   for (long c : nei) n_cells = std::max(n_cells, c + 1);
-  // boundary: patches in order, each nFaces from startFace.
+  for (long c : own) {
+    if (c < 0) throw std::runtime_error("E0132: polyMesh: a negative owner");
+  }
+  for (long c : nei) {
+    if (c < 0) throw std::runtime_error("E0132: polyMesh: a negative neighbour");
+  }
+  if (n_cells > INT32_MAX) throw std::runtime_error("E0132: polyMesh: too many cells");
+  // boundary: patches in order, each nFaces from startFace, within the boundary faces.
   std::vector<index_t> zone(nf, -1);
   {
     const std::string s = slurp(dir + "/boundary");
@@ -642,7 +698,12 @@ Mesh read_polymesh(const std::string& dir) {
     while ((p = s.find("nFaces", p)) != std::string::npos) {
       const long count = std::strtol(s.c_str() + p + 6, nullptr, 10);
       const std::size_t q = s.find("startFace", p);
+      if (q == std::string::npos) throw std::runtime_error("E0132: polyMesh boundary: nFaces without startFace");
       const long start = std::strtol(s.c_str() + q + 9, nullptr, 10);
+      if (count < 0 || start < static_cast<long>(nei.size()) || start > static_cast<long>(nf) ||
+          count > static_cast<long>(nf) - start) {
+        throw std::runtime_error("E0132: polyMesh boundary: a patch outside the boundary faces");
+      }
       for (long i = start; i < start + count; ++i) zone[static_cast<std::size_t>(i)] = patch;
       ++patch;
       p = q;
@@ -651,6 +712,7 @@ Mesh read_polymesh(const std::string& dir) {
   for (std::size_t i = 0; i < nf; ++i) {
     f.c0.push_back(static_cast<index_t>(own[i]));
     f.c1.push_back(i < nei.size() ? static_cast<index_t>(nei[i]) : -1);
+    if (i >= nei.size() && zone[i] < 0) throw std::runtime_error("E0132: polyMesh boundary: a face in no patch");
     f.zone.push_back(i < nei.size() ? -1 : zone[i]);
   }
   return assemble(static_cast<index_t>(n_cells), std::move(x), std::move(y), std::move(z), std::move(f));

@@ -227,3 +227,27 @@ def test_damaged_files_are_errors_not_crashes(tmp_path, encoding):
         except RuntimeError:
             outcomes["refused"] += 1
     assert outcomes["refused"] > 0 and sum(outcomes.values()) == 300
+
+
+def test_damaged_polymesh_files_are_errors_not_crashes(tmp_path):
+    """Truncations and single-byte changes of each file of a polyMesh: the reader refuses with its
+    error or reads a mesh; every count and index is checked before it is used."""
+    import shutil
+
+    rng = np.random.default_rng(5)
+    refused = 0
+    for trial in range(150):
+        d = tmp_path / f"m{trial}"
+        shutil.copytree(GOLDEN / "hex", d)
+        name = ("points", "faces", "owner", "neighbour", "boundary")[trial % 5]
+        data = bytearray((d / name).read_bytes())
+        if trial % 2:
+            data = data[: int(rng.integers(0, len(data)))]
+        else:
+            data[int(rng.integers(0, len(data)))] = int(rng.integers(0, 256))
+        (d / name).write_bytes(bytes(data))
+        try:
+            meshes.read_polymesh(d)
+        except RuntimeError:
+            refused += 1
+    assert refused > 0
