@@ -2,7 +2,8 @@
 continuation, (V / dt_c I + dR/dU) dU = -R with the pseudo-time step of each cell dt_c = CFL V_c /
 lambda_c (lambda_c the convective spectral radius, so V / dt_c = lambda_c / CFL), the CFL grown as
 the residual falls (switched evolution relaxation: CFL_k = CFL_0 |R_0| / |R_k|, capped), and each
-linear system solved on the host, by sparse LU or by GMRES with block-Jacobi preconditioning.
+linear system solved on the host: by sparse LU, by GMRES with block-Jacobi preconditioning, or by a
+few symmetric multicolour block Gauss-Seidel sweeps, the reference's relaxation.
 
 The Jacobian is the reference's (Euler::residual_and_jacobian and Euler::assemble,
 physics/src/euler.cpp:983-1053); its implicit solvers iterate on the device (block Gauss-Seidel,
@@ -36,29 +37,27 @@ def block_jacobi(a, n: int):
     """The inverse of the matrix's 5 x 5 diagonal blocks, as a preconditioner (scipy LinearOperator)."""
     from scipy.sparse.linalg import LinearOperator
 
-    bsr = a.tobsr(blocksize=(5, 5))
-    inv = np.zeros((n, 5, 5))
-    rows = np.repeat(np.arange(n), np.diff(bsr.indptr))
-    on_diag = bsr.indices == rows
-    inv[rows[on_diag]] = bsr.data[on_diag]
-    inv = np.linalg.inv(inv)
+    inv = np.linalg.inv(jacobian.diagonal_blocks(a, n))
     return LinearOperator((5 * n, 5 * n), matvec=lambda x: np.einsum("cij,cj->ci", inv, x.reshape(n, 5)).reshape(-1))
 
 
 def solve(stepper, u0: np.ndarray, max_iterations: int = 50, residual_orders: float = 8.0, cfl0: float = 10.0,
-          cfl_max: float = 1e10, linear: str = "lu", linear_rtol: float = 1e-3) -> ImplicitResult:
+          cfl_max: float = 1e10, linear: str = "lu", linear_rtol: float = 1e-3, sweeps: int = 4) -> ImplicitResult:
     """Steps from u0 until every equation's RMS residual has dropped `residual_orders` orders from
     its first value (equations whose first value is zero are met), or `max_iterations` solves.
     `stepper` gives residual_and_jacobian(u) and spectral_radius() (after a residual) and has
     `mesh`; a DeviceStepper does. `linear`: "lu" factors each system exactly; "gmres" solves it to
-    `linear_rtol` by GMRES preconditioned with the inverse diagonal blocks (block Jacobi)."""
+    `linear_rtol` by GMRES preconditioned with the inverse diagonal blocks (block Jacobi); "gs" takes
+    `sweeps` symmetric multicolour block Gauss-Seidel sweeps from zero, the reference's relaxation
+    (physics/include/autan/physics/block_matrix.hpp, BlockMatrix::gauss_seidel)."""
     from scipy.sparse.linalg import gmres, spsolve
 
-    if linear not in ("lu", "gmres"):
-        raise ValueError("linear is 'lu' or 'gmres'")
+    if linear not in ("lu", "gmres", "gs"):
+        raise ValueError("linear is 'lu', 'gmres' or 'gs'")
 
     mesh = stepper.mesh
     n = mesh.n_cells
+    cols = jacobian.colours(mesh) if linear == "gs" else None
     u = np.asarray(u0, dtype=np.float64).reshape(-1).copy()
     history, cfls = [], []
     first = None
@@ -79,6 +78,8 @@ def solve(stepper, u0: np.ndarray, max_iterations: int = 50, residual_orders: fl
         rhs = -jacobian.to_cells(parts["res"], n)
         if linear == "lu":
             du = spsolve(a.tocsc(), rhs)
+        elif linear == "gs":
+            du = jacobian.gauss_seidel(a, rhs, np.zeros_like(rhs), sweeps, cols)
         else:
             du, _ = gmres(a.tocsr(), rhs, rtol=linear_rtol, restart=50, maxiter=20, M=block_jacobi(a, n))
         u = u + jacobian.to_components(du, n).reshape(-1)

@@ -49,3 +49,54 @@ def assemble(mesh, parts: dict, shift: np.ndarray | None = None):
     c = (5 * cols[:, None, None] + i[None, None, :]) + 0 * i[None, :, None]
     a = sp.coo_matrix((blocks.reshape(-1), (r.reshape(-1), c.reshape(-1))), shape=(5 * n, 5 * n))
     return a.tobsr(blocksize=(5, 5))
+
+
+def colours(mesh) -> list[np.ndarray]:
+    """Cells in colours that share no face (greedy, in cell order: each cell the least colour none
+    of its face neighbours has taken)."""
+    n, ni = mesh.n_cells, mesh.n_interior
+    own = np.asarray(mesh.owner[:ni], dtype=np.int64)
+    nei = np.asarray(mesh.neighbour, dtype=np.int64)
+    adj = [[] for _ in range(n)]
+    for a, b in zip(own.tolist(), nei.tolist()):
+        adj[a].append(b)
+        adj[b].append(a)
+    colour = np.full(n, -1, dtype=np.int64)
+    for c in range(n):
+        taken = {int(colour[d]) for d in adj[c] if colour[d] >= 0}
+        k = 0
+        while k in taken:
+            k += 1
+        colour[c] = k
+    return [np.flatnonzero(colour == k) for k in range(int(colour.max()) + 1)]
+
+
+def diagonal_blocks(a, n: int) -> np.ndarray:
+    """The matrix's 5 x 5 diagonal blocks, (n, 5, 5)."""
+    bsr = a.tobsr(blocksize=(5, 5))
+    d = np.zeros((n, 5, 5))
+    rows = np.repeat(np.arange(n), np.diff(bsr.indptr))
+    on_diag = bsr.indices == rows
+    d[rows[on_diag]] = bsr.data[on_diag]
+    return d
+
+
+def gauss_seidel(a, b: np.ndarray, x: np.ndarray, sweeps: int, cols: list[np.ndarray]) -> np.ndarray:
+    """`sweeps` symmetric multicolour block Gauss-Seidel sweeps on A x = b from x (cell-major
+    vectors), the colours in increasing then decreasing order, after the diagonal blocks are
+    inverted (BlockMatrix::factor and gauss_seidel, physics/include/autan/physics/block_matrix.hpp):
+    the cells of one colour share no face, so they update together, each from its row with the
+    other cells' latest values."""
+    n = a.shape[0] // 5
+    x = np.asarray(x, dtype=np.float64).copy()
+    csr = a.tocsr()
+    d = diagonal_blocks(a, n)
+    inv = np.linalg.inv(d)
+    order = list(cols) + list(reversed(cols))
+    for _ in range(sweeps):
+        for cells in order:
+            ax = (csr @ x).reshape(n, 5)
+            xc = x.reshape(n, 5)
+            r = b.reshape(n, 5)[cells] - ax[cells] + np.einsum("cij,cj->ci", d[cells], xc[cells])
+            xc[cells] = np.einsum("cij,cj->ci", inv[cells], r)
+    return x

@@ -106,3 +106,12 @@ def test_gmres_with_block_jacobi_converges_as_the_exact_solves_do():
                         linear="gmres")
     assert lu.converged and kr.converged, (kr.iterations, kr.residuals[-1] / kr.residuals[0])
     assert kr.iterations <= 2 * lu.iterations + 4, (kr.iterations, lu.iterations)
+
+
+def test_gauss_seidel_sweeps_converge_as_the_reference_relaxes():
+    """Four symmetric multicolour block Gauss-Seidel sweeps per system (the reference's relaxation,
+    BlockMatrix::gauss_seidel) in place of the exact solve: still 8 orders, in more solves."""
+    c = cases.tets_case(3, flux="hllc", second_order=False)
+    out = implicit.solve(OracleStepper(c, "hllc"), c.u, max_iterations=60, residual_orders=8.0, cfl0=10.0,
+                         linear="gs", sweeps=4)
+    assert out.converged, (out.iterations, out.residuals[-1] / out.residuals[0])

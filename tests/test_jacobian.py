@@ -73,3 +73,33 @@ def test_the_implicit_solve_reaches_the_solution_of_a_linear_residual():
     exact = jacobian.to_components(spsolve(st.a.tocsc(), jacobian.to_cells(b, n)), n)
     np.testing.assert_allclose(out.u.reshape(5, n), exact, rtol=1e-8, atol=1e-10)
     assert out.cfl == sorted(out.cfl)  # the CFL grows as the residual falls
+
+
+def test_colours_share_no_face():
+    m = meshes.kuhn_tets(3, 0.1, 0)
+    cols = jacobian.colours(m)
+    colour = np.full(m.n_cells, -1)
+    for k, cells in enumerate(cols):
+        assert np.all(colour[cells] == -1)
+        colour[cells] = k
+    assert np.all(colour >= 0)
+    assert not np.any(colour[m.owner[: m.n_interior]] == colour[m.neighbour])
+
+
+def test_gauss_seidel_reaches_the_linear_solution():
+    """Symmetric multicolour block Gauss-Seidel on an M-matrix of the face structure converges to
+    the exact solution."""
+    from scipy.sparse.linalg import spsolve
+
+    m = meshes.kuhn_tets(2, 0.1, 0)
+    n, ni, nb = m.n_cells, m.n_interior, m.n_boundary
+    rng = np.random.default_rng(9)
+    eye = np.eye(5)[:, :, None]
+    face = np.stack([0.5 * np.repeat(eye, ni, axis=2), -0.5 * np.repeat(eye, ni, axis=2)])
+    parts = {"dwdu": np.repeat(eye, n, axis=2), "face_jac": face + 0.02 * rng.normal(size=(2, 5, 5, ni)),
+             "bnd_jac": np.repeat(eye, nb, axis=2) + 0.02 * rng.normal(size=(5, 5, nb))}
+    a = jacobian.assemble(m, parts, shift=np.full(n, 0.1))
+    b = rng.normal(size=5 * n)
+    exact = spsolve(a.tocsc(), b)
+    x = jacobian.gauss_seidel(a, b, np.zeros(5 * n), 60, jacobian.colours(m))
+    np.testing.assert_allclose(x, exact, rtol=0, atol=1e-9 * np.abs(exact).max())
