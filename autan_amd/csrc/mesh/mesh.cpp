@@ -859,6 +859,122 @@ FaceKey face_key(const index_t* v, std::size_t count) {
   return k;
 }
 
+// The surface links of $Periodic as periodic faces (the mesh's model: an interior face whose
+// neighbour sits, as seen from the owner, at its own position plus `shift`): each boundary face of
+// a slave surface is paired with the master surface's face at its position minus the link's
+// translation t, node by node within 1e-9 |t|; the master face becomes the interior face (owner the
+// master side's cell, neighbour the slave side's, shift -t) and the slave face goes. Node pairs
+// that the translation does not carry onto each other, a slave face without its image, and two
+// links of one surface pair with different translations are refused.
+template <class Link>
+RawFaces periodic_faces(RawFaces in, const std::vector<Link>& links, const std::map<FaceKey, long long>& surface_of,
+                        const std::unordered_map<long long, index_t>& node_of, const std::vector<double>& x,
+                        const std::vector<double>& y, const std::vector<double>& z) {
+  const std::size_t n = in.c0.size();
+  std::vector<long long> surface(n, -1);
+  for (std::size_t i = 0; i < n; ++i) {
+    if (in.c1[i] >= 0) continue;
+    const auto it = surface_of.find(face_key(&in.nodes[static_cast<std::size_t>(in.off[i])],
+                                             static_cast<std::size_t>(in.off[i + 1] - in.off[i])));
+    if (it != surface_of.end()) surface[i] = it->second;
+  }
+  auto point = [&](index_t v) { return std::array<double, 3>{x[at(v)], y[at(v)], z[at(v)]}; };
+  std::vector<double> shift(3 * n, 0.0);
+  std::vector<char> gone(n, 0);
+  std::map<std::pair<long long, long long>, std::array<double, 3>> seen;
+  for (const Link& l : links) {
+    const double len = std::sqrt(l.t[0] * l.t[0] + l.t[1] * l.t[1] + l.t[2] * l.t[2]);
+    const double tol = 1e-9 * len;
+    const auto key = std::make_pair(std::min(l.slave, l.master), std::max(l.slave, l.master));
+    const std::array<double, 3> tt{l.t[0], l.t[1], l.t[2]};
+    const auto prior = seen.find(key);
+    if (prior != seen.end() && prior->second != tt) {
+      throw std::runtime_error("E0132: gmsh: two periodic links of one surface pair with different translations");
+    }
+    seen[key] = tt;
+    auto carried = [&](index_t s, index_t m) {
+      const auto ps = point(s), pm = point(m);
+      const double d[3] = {ps[0] - pm[0] - l.t[0], ps[1] - pm[1] - l.t[1], ps[2] - pm[2] - l.t[2]};
+      return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= tol;
+    };
+    for (const auto& [ts, tm] : l.pairs) {
+      const auto s = node_of.find(ts), m = node_of.find(tm);
+      if (s == node_of.end() || m == node_of.end()) throw std::runtime_error("E0132: gmsh: a periodic node pair of undefined nodes");
+      if (!carried(s->second, m->second)) throw std::runtime_error("E0132: gmsh: a periodic node pair the translation does not carry onto each other");
+    }
+    if (len == 0.0) throw std::runtime_error("E0132: gmsh: a periodic link without a translation");
+    // Master faces by the cell of space their centroid falls in (side 4 tol): an image lies in
+    // the same cell or a neighbouring one.
+    const double cell = 4.0 * tol;
+    auto centroid = [&](std::size_t i) {
+      std::array<double, 3> c{0.0, 0.0, 0.0};
+      for (auto k = in.off[i]; k < in.off[i + 1]; ++k) {
+        const auto p = point(in.nodes[static_cast<std::size_t>(k)]);
+        for (int q = 0; q < 3; ++q) c[q] += p[q];
+      }
+      const double m = static_cast<double>(in.off[i + 1] - in.off[i]);
+      for (int q = 0; q < 3; ++q) c[q] /= m;
+      return c;
+    };
+    auto bin = [&](const std::array<double, 3>& c) {
+      return std::array<long long, 3>{static_cast<long long>(std::floor(c[0] / cell)),
+                                      static_cast<long long>(std::floor(c[1] / cell)),
+                                      static_cast<long long>(std::floor(c[2] / cell))};
+    };
+    std::map<std::array<long long, 3>, std::vector<std::size_t>> masters;
+    for (std::size_t i = 0; i < n; ++i) {
+      if (surface[i] == l.master && !gone[i] && in.c1[i] < 0) masters[bin(centroid(i))].push_back(i);
+    }
+    for (std::size_t i = 0; i < n; ++i) {
+      if (surface[i] != l.slave || gone[i] || in.c1[i] >= 0) continue;
+      auto c = centroid(i);
+      for (int q = 0; q < 3; ++q) c[q] -= l.t[q];
+      const auto b = bin(c);
+      std::size_t found = n;
+      for (long long dx = -1; dx <= 1 && found == n; ++dx) {
+        for (long long dy = -1; dy <= 1 && found == n; ++dy) {
+          for (long long dz = -1; dz <= 1 && found == n; ++dz) {
+            const auto it = masters.find({b[0] + dx, b[1] + dy, b[2] + dz});
+            if (it == masters.end()) continue;
+            for (const std::size_t j : it->second) {
+              if (in.c1[j] >= 0 || in.off[j + 1] - in.off[j] != in.off[i + 1] - in.off[i]) continue;
+              bool all = true;  // every slave node carried onto some node of the master face
+              for (auto k = in.off[i]; k < in.off[i + 1] && all; ++k) {
+                bool any = false;
+                for (auto h = in.off[j]; h < in.off[j + 1] && !any; ++h) {
+                  any = carried(in.nodes[static_cast<std::size_t>(k)], in.nodes[static_cast<std::size_t>(h)]);
+                }
+                all = any;
+              }
+              if (all) {
+                found = j;
+                break;
+              }
+            }
+          }
+        }
+      }
+      if (found == n) throw std::runtime_error("E0132: gmsh: a periodic face without its image on the master surface");
+      in.c1[found] = in.c0[i];
+      in.zone[found] = -1;
+      for (int q = 0; q < 3; ++q) shift[3 * found + static_cast<std::size_t>(q)] = -l.t[q];
+      gone[i] = 1;
+    }
+  }
+  RawFaces out;
+  for (std::size_t i = 0; i < n; ++i) {
+    if (gone[i]) continue;
+    out.nodes.insert(out.nodes.end(), in.nodes.begin() + in.off[i], in.nodes.begin() + in.off[i + 1]);
+    out.off.push_back(static_cast<offset_t>(out.nodes.size()));
+    out.c0.push_back(in.c0[i]);
+    out.c1.push_back(in.c1[i]);
+    out.zone.push_back(in.zone[i]);
+    out.shift.insert(out.shift.end(), shift.begin() + static_cast<std::ptrdiff_t>(3 * i),
+                     shift.begin() + static_cast<std::ptrdiff_t>(3 * i + 3));
+  }
+  return out;
+}
+
 }  // namespace
 
 Mesh read_gmsh(const std::string& path) {
@@ -887,6 +1003,13 @@ Mesh read_gmsh(const std::string& path) {
   std::vector<index_t> cell_nodes;
   std::vector<const Shape*> cell_shape;
   std::vector<std::pair<FaceKey, int>> labelled;  // boundary elements and their physical tags
+  std::map<FaceKey, long long> surface_of;         // every 2-D element's surface entity
+  struct Link {
+    long long slave, master;
+    double t[3];
+    std::vector<std::pair<long long, long long>> pairs;  // node tags: slave, master
+  };
+  std::vector<Link> links;  // the surface links of $Periodic
   std::array<Shape, 8> shapes{};                  // the volume shapes met, by element type
   bool have_nodes = false, have_elements = false;
 
@@ -1007,8 +1130,9 @@ Mesh read_gmsh(const std::string& path) {
           if (dim == 3) {
             cell_nodes.insert(cell_nodes.end(), v.begin(), v.end());
             cell_shape.push_back(kept);
-          } else if (dim == 2 && ph != 0) {
-            labelled.emplace_back(face_key(v.data(), v.size()), ph);
+          } else if (dim == 2) {
+            surface_of[face_key(v.data(), v.size())] = entity;
+            if (ph != 0) labelled.emplace_back(face_key(v.data(), v.size()), ph);
           }
         }
       }
@@ -1017,7 +1141,41 @@ Mesh read_gmsh(const std::string& path) {
       t.expect("$EndElements");
       have_elements = true;
     } else if (section == "$Periodic") {
-      throw std::runtime_error("E0133: gmsh: periodic links are not read");
+      // Links of a slave entity to its master by an affine map (16 values, row-major 4 x 4) and
+      // their node pairs; the surface links become periodic faces below.
+      t.skip_newline();
+      t.set_binary(binary);
+      const std::size_t nl = t.count();
+      t.check_fits(nl, 8, 20);
+      for (std::size_t l = 0; l < nl; ++l) {
+        const long long dim = t.i32();
+        const long long slave = t.i32();
+        const long long master = t.i32();
+        if (dim < 0 || dim > 2) throw std::runtime_error("E0132: gmsh: a periodic link of dimension " + std::to_string(dim));
+        if (slave <= 0 || master <= 0) throw std::runtime_error("E0132: gmsh: a periodic link to entity tag 0 or less");
+        const std::size_t na = t.count();
+        if (na != 0 && na != 16) throw std::runtime_error("E0132: gmsh: an affine map of " + std::to_string(na) + " values");
+        double a16[16] = {};
+        for (std::size_t k = 0; k < na; ++k) a16[k] = t.f64();
+        const std::size_t np = t.count();
+        t.check_fits(np, 4, 16);
+        Link link{slave, master, {a16[3], a16[7], a16[11]}, {}};
+        link.pairs.reserve(np);
+        for (std::size_t k = 0; k < np; ++k) {
+          const long long ns = t.u64();
+          const long long nm = t.u64();
+          link.pairs.emplace_back(ns, nm);
+        }
+        if (dim != 2) continue;
+        if (na != 16) throw std::runtime_error("E0133: gmsh: a periodic surface link without its affine map");
+        const double id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        for (int k : {0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14, 15}) {
+          if (a16[k] != id[k]) throw std::runtime_error("E0133: gmsh: a periodic link that is not a translation");
+        }
+        links.push_back(std::move(link));
+      }
+      t.set_binary(false);
+      t.expect("$EndPeriodic");
     } else if (section.size() > 1 && section[0] == '$') {
       // A section this reader does not use ($Comments, $NodeData, ...): skipped to its end.
       const std::string end = "$End" + section.substr(1);
@@ -1087,6 +1245,7 @@ Mesh read_gmsh(const std::string& path) {
   };
   RawFaces faces = match_faces(cf, zone_of);
   cf = CellFaces();
+  if (!links.empty()) faces = periodic_faces(std::move(faces), links, surface_of, node_of, x, y, z);
   return assemble(n_cells, std::move(x), std::move(y), std::move(z), std::move(faces));
 }
 

@@ -154,8 +154,8 @@ Mesh bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double
 Mesh read_polymesh(const std::string& dir);
 /// Gmsh MSH 4.1 file, ASCII or little-endian binary (io/src/gmsh.cpp): linear tetrahedra,
 /// hexahedra, prisms and pyramids; boundary zones the 2-D physical groups in increasing tag order,
-/// then one zone for boundary faces in no group; cells in file order. Other versions and periodic
-/// links are refused.
+/// then one zone for boundary faces in no group; cells in file order; surface links of $Periodic
+/// by a translation become periodic faces. Other versions and links that rotate are refused.
 Mesh read_gmsh(const std::string& path);
 
 }  // namespace mx

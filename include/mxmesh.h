@@ -39,9 +39,10 @@ int mxmesh_bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, 
 int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_len);
 /* A Gmsh MSH 4.1 file, ASCII or binary (io/src/gmsh.cpp; docs/contracts/mesh.md, Readers): linear volume
  * elements; boundary zone k is the k-th 2-D physical group by increasing tag, then one zone for
- * boundary faces in no group. Errors carry the reference's reader codes (E0130 missing file, E0131
- * not a mesh file, E0132 malformed, E0133 unsupported: MSH 2.2, big-endian, higher-order elements,
- * periodic links; E0135 an entity in two physical groups). */
+ * boundary faces in no group; surface links of $Periodic by a translation become periodic faces.
+ * Errors carry the reference's reader codes (E0130 missing file, E0131 not a mesh file, E0132
+ * malformed, E0133 unsupported: MSH 2.2, big-endian, higher-order elements, periodic links that are
+ * not translations; E0135 an entity in two physical groups). */
 int mxmesh_read_gmsh(const char* path, mxmesh** out, char* err, size_t err_len);
 void mxmesh_destroy(mxmesh* m);
 

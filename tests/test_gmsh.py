@@ -167,7 +167,8 @@ def test_a_hand_written_tetrahedron(tmp_path, nodes):
     (("4.1 0 8", "4.1 2 8"), "E0132"),
     (("4.1 0 8", "2.2 0 8"), "E0133"),
     (("3 1 4 1", "3 1 11 1"), "E0133"),
-    (("$EndElements\n", "$EndElements\n$Periodic\n0\n$EndPeriodic\n"), "E0133"),
+    (("$EndElements\n", "$EndElements\n$Periodic\n1\n2 1 2\n16 0 -1 0 0 1 0 0 0 0 0 1 0 0 0 0 1\n0\n$EndPeriodic\n"),
+     "E0133"),
     (("2 0 0 0 1 1 1 1 8 0", "2 0 0 0 1 1 1 2 7 8 0"), "E0135"),
     (("$MeshFormat", "$Format"), "E0131"),
     (("6 NODES", "6 1 2 3 99"), "E0132"),
@@ -251,3 +252,67 @@ def test_damaged_polymesh_files_are_errors_not_crashes(tmp_path):
         except RuntimeError:
             refused += 1
     assert refused > 0
+
+
+
+PERIODIC = GOLDEN / "gmsh" / "periodic-box.ascii.msh"
+
+
+@pytest.mark.parametrize("encoding", ["ascii", "binary"])
+def test_periodic_surface_links_become_periodic_faces(encoding):
+    """The reference's periodic box (tetrahedra in the unit cube, xmax meshed as xmin moved by
+    (1, 0, 0), ymax as ymin by (0, 1, 0); tests/io/test_readers.cpp:428-444): those faces become
+    interior faces whose neighbour lies a period away, every cell stays closed, and only zmin and
+    zmax stay boundary zones."""
+    m = meshes.read_gmsh(GOLDEN / "gmsh" / f"periodic-box.{encoding}.msh")
+    zones = gmsh_zones(PERIODIC)  # the binary file's names are the ASCII file's
+    assert sorted(zones[z] for z in set(m.face_zone.tolist())) == ["zmax", "zmin"]
+    shift = m.arrays["shift"].reshape(3, -1)
+    moved = np.flatnonzero(np.abs(shift).sum(axis=0))
+    assert moved.size > 0
+    assert {tuple(np.abs(shift[:, f])) for f in moved} == {(1.0, 0.0, 0.0), (0.0, 1.0, 0.0)}
+    av = m.arrays["face_area_vector"].reshape(3, -1)
+    closed = np.zeros((m.n_cells, 3))
+    np.add.at(closed, m.owner, av.T)
+    np.add.at(closed, m.neighbour, -av[:, : m.n_interior].T)
+    assert np.abs(closed).max() < 1e-15
+    assert abs(m.cell_volume.sum() - 1.0) < 1e-13
+
+
+def test_a_uniform_flow_is_steady_on_the_periodic_box():
+    """Every face, periodic ones included, oriented and placed right: a uniform flow with its own
+    state on zmin and zmax has a zero residual (to round-off) in every cell."""
+    from oracle.oracle import Oracle
+
+    m = meshes.read_gmsh(PERIODIC)
+    kinds = np.zeros(m.n_boundary, dtype=np.int32)
+    meshes.set_boundary_kinds(m, kinds)
+    w = np.array([1.0, 0.3, 0.2, 0.1, 1.0 / 1.4])
+    sc = freestream_scheme(w[0], w[1:4], w[4], flux="roe", second_order=True, limiter=True)
+    bc = np.repeat(w[:, None], m.n_boundary, axis=1)
+    u = np.repeat(np.asarray(cases.conserved(*w))[:, None], m.n_cells, axis=1)
+    r = Oracle(m, sc, "fp64", kinds.astype(np.float64), bc).residual(u.reshape(-1)).reshape(5, -1)
+    assert np.abs(r).max() < 1e-14
+
+
+@pytest.mark.parametrize("edit,code", [
+    (("2 2 1\n16 1 0 0 1 0 1 0 0", "2 2 1\n16 0 -1 0 1 1 0 0 0"), "E0133"),  # a rotation
+    (("2 2 1\n16 1 0 0 1 0", "2 2 1\n16 1 0 0 1.01 0"), "E0132"),  # node pairs off the translation
+    (("58\n5 1\n6 2\n", "58\n5 2\n6 1\n"), "E0132"),  # a node pair swapped
+    (("2 2 1\n16 1 0 0 1", "4 2 1\n16 1 0 0 1"), "E0132"),  # a dimension the format lacks
+    (("\n1 3 1\n", "\n1 0 1\n"), "E0132"),  # a link to entity tag 0
+    (("2 2 1\n16 1 0 0 1 0 1 0 0 0 0 1 0 0 0 0 1\n", "2 2 1\n0\n"), "E0133"),  # no affine map
+    (("$Periodic\n15\n", "$Periodic\n16\n2 2 1\n16 1 0 0 2 0 1 0 0 0 0 1 0 0 0 0 1\n0\n"), "E0132"),
+    (("2 2 1\n16 1 0 0 1 0 1 0 0", "2 2 1\n16 1 0 0 1 0 1 0 1e-09"), None),  # off by the tolerance
+])
+def test_periodic_links_are_checked(tmp_path, edit, code):
+    """The reference's edits of the periodic box's links (tests/io/test_readers.cpp:446-481)."""
+    text = PERIODIC.read_text()
+    assert edit[0] in text
+    path = tmp_path / "p.msh"
+    path.write_text(text.replace(edit[0], edit[1], 1))
+    if code is None:
+        assert meshes.read_gmsh(path).n_cells > 0
+    else:
+        with pytest.raises(RuntimeError, match=code):
+            meshes.read_gmsh(path)
