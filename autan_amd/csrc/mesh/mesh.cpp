@@ -572,6 +572,18 @@ class FoamText {
     std::size_t p = s_.find("FoamFile");
     p = p == std::string::npos ? 0 : s_.find('}', p);
     if (p == std::string::npos) fail("an unterminated FoamFile header");
+    if (p > 0) {
+      // A binary file's lists hold raw little-endian labels (32 or 64 bits) and 64-bit scalars.
+      const std::string header = s_.substr(0, p);
+      const std::size_t f = header.find("format");
+      const std::size_t v = f == std::string::npos ? std::string::npos : header.find_first_not_of(" \t", f + 6);
+      binary_ = v != std::string::npos && header.compare(v, 6, "binary") == 0;
+      if (binary_) {
+        if (header.find("label=64") != std::string::npos) label_bytes_ = 8;
+        if (header.find("scalar=32") != std::string::npos) fail("32-bit scalars");
+        if (header.find("MSB") != std::string::npos) fail("a big-endian file");
+      }
+    }
     p_ = p == 0 ? 0 : p + 1;
     while (p_ < s_.size()) {
       if (s_.compare(p_, 2, "//") == 0) {
@@ -614,26 +626,54 @@ class FoamText {
     if (q == std::string::npos) fail(std::string("expected '") + c + "'");
     p_ = q + 1;
   }
-  // A list count, no larger than the items of at least `each` characters that the rest holds.
+  // A list count, no larger than the items of at least `each` characters (or bytes, in a binary
+  // file) that the rest holds.
   std::size_t count(std::size_t each) {
     const long n = integer();
     if (n < 0 || static_cast<std::size_t>(n) > (s_.size() - p_) / each) fail("a count larger than the file can hold");
     return static_cast<std::size_t>(n);
   }
+  [[nodiscard]] bool binary() const { return binary_; }
+  [[nodiscard]] std::size_t label_bytes() const { return label_bytes_; }
+  // A binary list's items: right after its '(', raw, then its ')'.
+  long raw_label() {
+    if (label_bytes_ == 8) return static_cast<long>(raw<std::int64_t>());
+    return static_cast<long>(raw<std::int32_t>());
+  }
+  double raw_scalar() { return raw<double>(); }
+  void close() {
+    if (p_ >= s_.size() || s_[p_] != ')') fail("expected ')' after a binary list");
+    ++p_;
+  }
 
  private:
+  template <class T>
+  T raw() {
+    if (p_ + sizeof(T) > s_.size()) fail("the file ends inside a list");
+    T v;
+    std::memcpy(&v, s_.data() + p_, sizeof(T));
+    p_ += sizeof(T);
+    return v;
+  }
+  bool binary_ = false;
+  std::size_t label_bytes_ = 4;
   std::string s_;
   std::string name_;
   std::size_t p_ = 0;
 };
 
-std::vector<long> labels(const std::string& dir, const char* file) {
-  FoamText t(slurp(dir + "/" + file), file);
-  const std::size_t n = t.count(2);
+std::vector<long> label_list(FoamText& t) {
+  const std::size_t n = t.count(t.binary() ? t.label_bytes() : 2);
   t.open('(');
   std::vector<long> out(n);
-  for (auto& v : out) v = t.integer();
+  for (auto& v : out) v = t.binary() ? t.raw_label() : t.integer();
+  if (t.binary()) t.close();
   return out;
+}
+
+std::vector<long> labels(const std::string& dir, const char* file) {
+  FoamText t(slurp(dir + "/" + file), file);
+  return label_list(t);
 }
 
 }  // namespace
@@ -642,25 +682,48 @@ Mesh read_polymesh(const std::string& dir) {
   std::vector<double> x, y, z;
   {
     FoamText t(slurp(dir + "/points"), "points");
-    const std::size_t n = t.count(7);  // "(a b c)"
+    const std::size_t n = t.count(t.binary() ? 24 : 7);  // three doubles, or "(a b c)"
     t.open('(');
     x.reserve(n);
     y.reserve(n);
     z.reserve(n);
     for (std::size_t i = 0; i < n; ++i) {
+      if (t.binary()) {
+        x.push_back(t.raw_scalar());
+        y.push_back(t.raw_scalar());
+        z.push_back(t.raw_scalar());
+        continue;
+      }
       t.open('(');
       x.push_back(t.real());
       y.push_back(t.real());
       z.push_back(t.real());
       t.open(')');
     }
+    if (t.binary()) t.close();
   }
   const long np = static_cast<long>(x.size());
   RawFaces f;
   {
     FoamText t(slurp(dir + "/faces"), "faces");
-    const std::size_t n = t.count(8);  // "3(a b c)"
-    t.open('(');
+    if (t.binary()) {
+      // faceCompactList: the faces' offsets (one more than the faces), then their nodes.
+      const std::vector<long> off = label_list(t);
+      const std::vector<long> nodes = label_list(t);
+      if (off.empty() || off[0] != 0 || off.back() != static_cast<long>(nodes.size())) t.fail("face offsets");
+      for (std::size_t i = 0; i + 1 < off.size(); ++i) {
+        const long k = off[i + 1] - off[i];
+        if (k < 3 || k > 1024) t.fail("a face of " + std::to_string(k) + " nodes");
+        for (long j = off[i]; j < off[i + 1]; ++j) {
+          const long v = nodes[static_cast<std::size_t>(j)];
+          if (v < 0 || v >= np) t.fail("a face node out of range");
+          f.nodes.push_back(static_cast<index_t>(v));
+        }
+        f.off.push_back(static_cast<offset_t>(f.nodes.size()));
+      }
+    }
+    const std::size_t n = t.binary() ? 0 : t.count(8);  // "3(a b c)"
+    if (!t.binary()) t.open('(');
     for (std::size_t i = 0; i < n; ++i) {
       const long k = t.integer();
       if (k < 3 || k > 1024) t.fail("a face of " + std::to_string(k) + " nodes");
@@ -689,6 +752,19 @@ Mesh read_polymesh(const std::string& dir) {
     if (c < 0) throw std::runtime_error("E0132: polyMesh: a negative neighbour");
   }
   if (n_cells > INT32_MAX) throw std::runtime_error("E0132: polyMesh: too many cells");
+  // A cell has 4 faces at least and a face 2 cells at most: a label past (faces + interior faces) / 4
+  // cannot be a cell of this mesh, and is refused before anything is sized by it.
+  if (n_cells > static_cast<long>((nf + nei.size()) / 4)) {
+    throw std::runtime_error("E0132: polyMesh: a cell label beyond the cells the faces can close");
+  }
+  {
+    std::vector<int> faces_of(static_cast<std::size_t>(n_cells), 0);
+    for (long c : own) ++faces_of[static_cast<std::size_t>(c)];
+    for (long c : nei) ++faces_of[static_cast<std::size_t>(c)];
+    for (int k : faces_of) {
+      if (k < 4) throw std::runtime_error("E0132: polyMesh: a cell with fewer than 4 faces");
+    }
+  }
   // boundary: patches in order, each nFaces from startFace, within the boundary faces.
   std::vector<index_t> zone(nf, -1);
   {

@@ -150,7 +150,7 @@ Mesh bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, double h,
 /// as bump_hex.
 Mesh bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
                  int prism_layers, int band_lo, int band_hi);
-/// OpenFOAM polyMesh (ASCII) directory: points, faces, owner, neighbour, boundary.
+/// OpenFOAM polyMesh directory, ASCII or binary: points, faces, owner, neighbour, boundary.
 Mesh read_polymesh(const std::string& dir);
 /// Gmsh MSH 4.1 file, ASCII or little-endian binary (io/src/gmsh.cpp): linear tetrahedra,
 /// hexahedra, prisms and pyramids; boundary zones the 2-D physical groups in increasing tag order,

@@ -35,8 +35,9 @@ int mxmesh_bump_hex(int nx, int ny, int nz, double lx, double ly, double lz, dou
  * merged 2 x 2 x 2 into polyhedra, hexahedra elsewhere. */
 int mxmesh_bump_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, double h, double w, double x0,
                        int prism_layers, int band_lo, int band_hi, mxmesh** out, char* err, size_t err_len);
-/* An ASCII OpenFOAM polyMesh directory (io/src/polymesh.cpp): points, faces, owner, neighbour,
- * boundary; every count, label and patch range checked (E0130 a missing file, E0132 malformed). */
+/* An OpenFOAM polyMesh directory, ASCII or binary (io/src/polymesh.cpp): points, faces, owner,
+ * neighbour, boundary; every count, label and patch range checked (E0130 a missing file, E0132
+ * malformed). */
 int mxmesh_read_polymesh(const char* dir, mxmesh** out, char* err, size_t err_len);
 /* A Gmsh MSH 4.1 file, ASCII or binary (io/src/gmsh.cpp; docs/contracts/mesh.md, Readers): linear volume
  * elements; boundary zone k is the k-th 2-D physical group by increasing tag, then one zone for

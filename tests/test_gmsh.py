@@ -230,7 +230,18 @@ def test_damaged_files_are_errors_not_crashes(tmp_path, encoding):
     assert outcomes["refused"] > 0 and sum(outcomes.values()) == 300
 
 
-def test_damaged_polymesh_files_are_errors_not_crashes(tmp_path):
+@pytest.mark.parametrize("name", ["hybrid", "poly"])
+def test_binary_polymesh_reads_as_the_ascii_one(name):
+    """OpenFOAM's binary polyMesh (labels and scalars raw, the faces as a compact list) of the
+    reference's test meshes: the ASCII directory's mesh, bit for bit (test_readers.cpp:42-43)."""
+    a = meshes.read_polymesh(GOLDEN / name)
+    b = meshes.read_polymesh(GOLDEN / f"{name}-binary")
+    for k in a.arrays:
+        assert np.array_equal(np.asarray(a.arrays[k]).view(np.uint8), np.asarray(b.arrays[k]).view(np.uint8)), k
+
+
+@pytest.mark.parametrize("source", ["hex", "hybrid-binary"])
+def test_damaged_polymesh_files_are_errors_not_crashes(tmp_path, source):
     """Truncations and single-byte changes of each file of a polyMesh: the reader refuses with its
     error or reads a mesh; every count and index is checked before it is used."""
     import shutil
@@ -239,7 +250,7 @@ def test_damaged_polymesh_files_are_errors_not_crashes(tmp_path):
     refused = 0
     for trial in range(150):
         d = tmp_path / f"m{trial}"
-        shutil.copytree(GOLDEN / "hex", d)
+        shutil.copytree(GOLDEN / source, d)
         name = ("points", "faces", "owner", "neighbour", "boundary")[trial % 5]
         data = bytearray((d / name).read_bytes())
         if trial % 2:
