@@ -5,6 +5,7 @@
     libmxmlp.so    the warm-start network (BF16 matrix-core kernels), hipcc for gfx950
     oracle/_build/liboracle.so   the CPU oracle (test infrastructure), g++ -O3 -ffp-contract=off
     tests/_build/adapter_check   a test program of the C++ adapter include/mxeuler.hpp, g++
+    tests/_build/exact_check     a test program of the host exact sums (csrc/device/exact.hpp), g++
     oracle/_ref/json_dump        the reference runtime's JSON library as a checker (oracle/build_ref.sh;
                                  built only where the reference tree is)
 
@@ -85,6 +86,15 @@ def build_adapter_check() -> Path:
     return out
 
 
+def build_exact_check() -> Path:
+    """The host exact sums' test program (the header the HIP library's host code includes)."""
+    out = ROOT / "tests" / "_build" / "exact_check"
+    out.parent.mkdir(parents=True, exist_ok=True)
+    _run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-Wall", "-Wextra", "-Werror", "-o", out,
+          ROOT / "tests" / "cpp" / "exact_check.cpp"])
+    return out
+
+
 def build_ref() -> None:
     _run(["sh", ROOT / "oracle" / "build_ref.sh"])
 
@@ -97,6 +107,7 @@ def build_all() -> None:
     build_device()
     build_mlp()
     build_adapter_check()
+    build_exact_check()
     build_ref()
 
 
@@ -104,4 +115,4 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"mesh": build_mesh, "device": build_device, "oracle": build_oracle, "mlp": build_mlp,
-         "adapter": build_adapter_check, "ref": build_ref, "all": build_all}[w]()
+         "adapter": build_adapter_check, "exact": build_exact_check, "ref": build_ref, "all": build_all}[w]()

@@ -1,6 +1,17 @@
-// Host side of the order-independent exact sum (runtime/src/exact_sum.cpp:22-103): 68 limbs of
-// 32 bits, limb i weighing 2^(32 i - 1088), plus counts of +inf, -inf and NaN terms; the sum is
-// rounded once, to nearest even. The device kernels add into the same word layout.
+// Host side of the order-independent exact sums. The words are the reference's exact-sum format
+// (runtime/src/exact_sum.cpp:22-103, kept so that the words mean the same): 68 signed 64-bit limbs,
+// limb i weighing 2^(32 i - 1088), then the counts of +inf, -inf and NaN terms. The device kernels
+// add into the same words.
+//
+// Rounding: the limbs are resolved once into the sign and the 68 base-2^32 digits of |sum| (one
+// carry pass in 128-bit arithmetic); the 64 bits from the leading one down are read as one integer
+// whose lowest bit is OR-ed with every bit below them (rounding to odd at 64 bits), and that integer
+// is converted to a double, which rounds it to nearest even at 53 bits. Rounding to odd at p + 2 or
+// more bits then to nearest at p bits gives the correctly rounded result (Boldo and Melquiond 2008,
+// "Emulation of FMA and correctly rounded sums: proved algorithms using rounding to odd", IEEE TC
+// 57(4)). The power-of-two scaling after it is exact: a sum of doubles is a multiple of 2^-1074, so
+// a result below 2^-1021 has fewer than 54 significant bits and was never rounded, and above it the
+// scaled value is normal.
 #pragma once
 
 #include <cmath>
@@ -12,91 +23,122 @@ namespace mxe {
 
 inline constexpr int kExactLimbs = 68;
 inline constexpr int kExactWords = kExactLimbs + 3;
+inline constexpr int kExactBias = 1088;  // limb 0's weight is 2^-kExactBias
+
+namespace exact_detail {
+
+// |sum| as base-2^32 digits plus whatever lies above the last limb (`over`), and the sign.
+struct Magnitude {
+  std::uint32_t digit[kExactLimbs];
+  unsigned __int128 over;  // multiples of 2^(32 * kExactLimbs) in units of limb 0
+  bool negative;
+};
+
+inline Magnitude resolve(const long long* w) {
+  Magnitude m{};
+  __int128 carry = 0;
+  for (int i = 0; i < kExactLimbs; ++i) {
+    const __int128 acc = carry + w[i];
+    m.digit[i] = static_cast<std::uint32_t>(static_cast<unsigned __int128>(acc) & 0xffffffffu);
+    carry = acc >> 32;  // floor division: the digits stay in [0, 2^32)
+  }
+  m.negative = carry < 0;
+  if (!m.negative) {
+    m.over = static_cast<unsigned __int128>(carry);
+    return m;
+  }
+  // Two's complement of (carry, digits): invert, add one from the lowest digit up.
+  unsigned long long up = 1;
+  for (int i = 0; i < kExactLimbs; ++i) {
+    const unsigned long long v = static_cast<unsigned long long>(~m.digit[i]) + up;
+    m.digit[i] = static_cast<std::uint32_t>(v);
+    up = v >> 32;
+  }
+  m.over = static_cast<unsigned __int128>(-(carry + 1)) + up;
+  return m;
+}
+
+// Bit `b` (0 = the lowest bit of digit 0) of the digits.
+inline unsigned bit_of(const Magnitude& m, int b) {
+  return b < 0 ? 0u : (m.digit[b >> 5] >> (b & 31)) & 1u;
+}
+
+// Whether any bit below position `b` is set.
+inline bool any_below(const Magnitude& m, int b) {
+  if (b <= 0) return false;
+  const int whole = b >> 5;
+  for (int i = 0; i < whole; ++i)
+    if (m.digit[i] != 0) return true;
+  const int part = b & 31;
+  return part != 0 && (m.digit[whole] & ((1u << part) - 1u)) != 0;
+}
+
+}  // namespace exact_detail
 
 // The value of exact words, rounded once to the nearest double (ties to even).
 inline double exact_value(const long long* in) {
+  using namespace exact_detail;
   constexpr double inf = std::numeric_limits<double>::infinity();
-  constexpr int L = kExactLimbs;
-  if (in[L + 2] > 0 || (in[L] > 0 && in[L + 1] > 0)) return std::numeric_limits<double>::quiet_NaN();
-  if (in[L] > 0) return inf;
-  if (in[L + 1] > 0) return -inf;
-  long long w[L];
-  for (int i = 0; i < L; ++i) w[i] = in[i];
-  auto normalize = [&w] {
-    for (int i = 0; i + 1 < L; ++i) {
-      const long long carry = w[i] >> 32;
-      w[i] -= carry * (1LL << 32);
-      w[i + 1] += carry;
-    }
-  };
-  normalize();
-  const bool negative = w[L - 1] < 0;
-  if (negative) {
-    for (long long& v : w) v = -v;
-    normalize();
-  }
-  int t = L - 1;
-  while (t >= 0 && w[t] == 0) --t;
-  if (t < 0) return 0.0;
-  auto limb = [&w](int k) { return k < 0 ? 0ULL : static_cast<unsigned long long>(w[k]); };
-  const unsigned long long top = limb(t);
-  if (top > 0xffffffffULL) return negative ? -inf : inf;
-  int b = 0;
-  while ((top >> b) != 0) ++b;
-  const unsigned long long below = (limb(t - 1) << 32) | limb(t - 2);
-  const unsigned long long lead = (top << (64 - b)) | (below >> b);
-  bool sticky = (below & ((1ULL << b) - 1)) != 0;
-  for (int k = t - 3; k >= 0 && !sticky; --k) sticky = limb(k) != 0;
-  unsigned long long mantissa = lead >> 11;
-  const unsigned long long rest = lead & 0x7ff;
-  if (rest > 0x400 || (rest == 0x400 && (sticky || (mantissa & 1) != 0))) ++mantissa;
-  int exponent = 32 * (t - 2) - 1088 + b + 11;
-  if (mantissa == (1ULL << 53)) {
-    mantissa >>= 1;
-    ++exponent;
-  }
-  const double magnitude = std::ldexp(static_cast<double>(mantissa), exponent);
-  return negative ? -magnitude : magnitude;
+  const long long pos_inf = in[kExactLimbs], neg_inf = in[kExactLimbs + 1], nans = in[kExactLimbs + 2];
+  if (nans > 0 || (pos_inf > 0 && neg_inf > 0)) return std::numeric_limits<double>::quiet_NaN();
+  if (pos_inf > 0) return inf;
+  if (neg_inf > 0) return -inf;
+  const Magnitude m = resolve(in);
+  // 2^(32 * 68 - 1088) = 2^1088 and above is past the largest double.
+  if (m.over != 0) return m.negative ? -inf : inf;
+  int lead = -1;  // position of the leading one
+  for (int i = kExactLimbs - 1; i >= 0 && lead < 0; --i)
+    if (m.digit[i] != 0) lead = 32 * i + 31 - __builtin_clz(m.digit[i]);
+  if (lead < 0) return 0.0;
+  // The 64 bits lead - 63 .. lead (zeros below bit 0), rounded to odd by everything under them.
+  std::uint64_t top = 0;
+  for (int b = lead; b > lead - 64; --b) top = (top << 1) | bit_of(m, b);
+  if (any_below(m, lead - 63)) top |= 1u;
+  const double rounded = static_cast<double>(top);  // to nearest even at 53 bits
+  const double magnitude = std::ldexp(rounded, lead - 63 - kExactBias);
+  return m.negative ? -magnitude : magnitude;
 }
 
-// A running exact sum on the host.
+// A running exact sum on the host, in the same words.
 class ExactSum {
  public:
   void add(double term) {
     std::uint64_t bits;
-    std::memcpy(&bits, &term, 8);
-    const int biased = static_cast<int>((bits >> 52) & 0x7ff);
-    const std::uint64_t fraction = bits & ((std::uint64_t{1} << 52) - 1);
-    const bool negative = (bits >> 63) != 0;
-    if (biased == 0x7ff) {
-      ++w_[fraction != 0 ? kExactLimbs + 2 : (negative ? kExactLimbs + 1 : kExactLimbs)];
+    std::memcpy(&bits, &term, sizeof bits);
+    const unsigned field = static_cast<unsigned>(bits >> 52) & 0x7ffu;
+    const std::uint64_t frac = bits & ((std::uint64_t{1} << 52) - 1);
+    const bool minus = (bits >> 63) != 0;
+    if (field == 0x7ffu) {  // inf or NaN: counted, never added
+      ++words_[frac != 0 ? kExactLimbs + 2 : kExactLimbs + (minus ? 1 : 0)];
       return;
     }
-    if (biased == 0 && fraction == 0) return;
-    const std::uint64_t m = biased == 0 ? fraction : (fraction | (std::uint64_t{1} << 52));
-    const int p = (biased == 0 ? -1074 : biased - 1075) + 1088;
-    const int i = p >> 5, s = p & 31;
-    const std::uint64_t lo = (m & 0xffffffffULL) << s, hi = (m >> 32) << s;
-    const long long sign = negative ? -1 : 1;
-    w_[i] += sign * static_cast<long long>(lo & 0xffffffffULL);
-    w_[i + 1] += sign * static_cast<long long>((lo >> 32) + (hi & 0xffffffffULL));
-    w_[i + 2] += sign * static_cast<long long>(hi >> 32);
-    // Each add moves a limb by less than 2^33: carry well before 2^63.
-    if (++pending_ == (1LL << 29)) normalize();
+    // term = significand * 2^(exponent); position in units of limb 0's weight
+    const std::uint64_t significand = field == 0 ? frac : frac | (std::uint64_t{1} << 52);
+    if (significand == 0) return;
+    const int position = (field == 0 ? -1074 : int(field) - 1075) + kExactBias;
+    const unsigned __int128 placed = static_cast<unsigned __int128>(significand) << (position & 31);
+    const int first = position >> 5;
+    for (int k = 0; k < 3; ++k) {
+      const long long piece = static_cast<long long>(static_cast<std::uint64_t>(placed >> (32 * k)) & 0xffffffffu);
+      words_[first + k] += minus ? -piece : piece;
+    }
+    // A limb moves by less than 2^32 per term; resolving carries every 2^30 terms keeps every limb
+    // below 2^63 in magnitude.
+    if (++since_carry_ == (1u << 30)) carry();
   }
-  [[nodiscard]] double value() const { return exact_value(w_); }
+  [[nodiscard]] double value() const { return exact_value(words_); }
 
  private:
-  void normalize() {
+  void carry() {
     for (int i = 0; i + 1 < kExactLimbs; ++i) {
-      const long long carry = w_[i] >> 32;
-      w_[i] -= carry * (1LL << 32);
-      w_[i + 1] += carry;
+      const long long high = words_[i] >> 32;  // floor: the limb keeps its low 32 bits
+      words_[i] &= 0xffffffffLL;
+      words_[i + 1] += high;
     }
-    pending_ = 0;
+    since_carry_ = 0;
   }
-  long long w_[kExactWords] = {};
-  long long pending_ = 0;
+  long long words_[kExactWords] = {};
+  unsigned since_carry_ = 0;
 };
 
 }  // namespace mxe

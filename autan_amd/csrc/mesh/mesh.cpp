@@ -580,8 +580,8 @@ class FoamText {
       binary_ = v != std::string::npos && header.compare(v, 6, "binary") == 0;
       if (binary_) {
         if (header.find("label=64") != std::string::npos) label_bytes_ = 8;
-        if (header.find("scalar=32") != std::string::npos) fail("32-bit scalars");
-        if (header.find("MSB") != std::string::npos) fail("a big-endian file");
+        if (header.find("scalar=32") != std::string::npos) scalar_bytes_ = 4;  // read, widened to double
+        if (header.find("MSB") != std::string::npos) unsupported("a big-endian file");
       }
     }
     p_ = p == 0 ? 0 : p + 1;
@@ -603,6 +603,9 @@ class FoamText {
   }
   [[noreturn]] void fail(const std::string& why) const {
     throw std::runtime_error("E0132: polyMesh " + name_ + ": " + why);
+  }
+  [[noreturn]] void unsupported(const std::string& what) const {
+    throw std::runtime_error("E0133: polyMesh " + name_ + ": unsupported " + what);
   }
   long integer() {
     const char* b = s_.c_str() + p_;
@@ -633,14 +636,23 @@ class FoamText {
     if (n < 0 || static_cast<std::size_t>(n) > (s_.size() - p_) / each) fail("a count larger than the file can hold");
     return static_cast<std::size_t>(n);
   }
+  // After a list's count: whether the list is written uniform, N{value}; consumes the '{'.
+  bool uniform() {
+    std::size_t q = p_;
+    while (q < s_.size() && std::isspace(static_cast<unsigned char>(s_[q]))) ++q;
+    if (q >= s_.size() || s_[q] != '{') return false;
+    p_ = q + 1;
+    return true;
+  }
   [[nodiscard]] bool binary() const { return binary_; }
   [[nodiscard]] std::size_t label_bytes() const { return label_bytes_; }
+  [[nodiscard]] std::size_t scalar_bytes() const { return scalar_bytes_; }
   // A binary list's items: right after its '(', raw, then its ')'.
   long raw_label() {
     if (label_bytes_ == 8) return static_cast<long>(raw<std::int64_t>());
     return static_cast<long>(raw<std::int32_t>());
   }
-  double raw_scalar() { return raw<double>(); }
+  double raw_scalar() { return scalar_bytes_ == 4 ? static_cast<double>(raw<float>()) : raw<double>(); }
   void close() {
     if (p_ >= s_.size() || s_[p_] != ')') fail("expected ')' after a binary list");
     ++p_;
@@ -657,6 +669,7 @@ class FoamText {
   }
   bool binary_ = false;
   std::size_t label_bytes_ = 4;
+  std::size_t scalar_bytes_ = 8;
   std::string s_;
   std::string name_;
   std::size_t p_ = 0;
@@ -671,9 +684,117 @@ std::vector<long> label_list(FoamText& t) {
   return out;
 }
 
-std::vector<long> labels(const std::string& dir, const char* file) {
+// A labelList that may also be written uniform, N{v} (as OpenFOAM writes a list of one repeated
+// value, in either format), with N at most `most`.
+std::vector<long> labels(const std::string& dir, const char* file, std::size_t most) {
   FoamText t(slurp(dir + "/" + file), file);
-  return label_list(t);
+  const long n = t.integer();
+  if (t.uniform()) {
+    if (n < 0 || static_cast<std::size_t>(n) > most) t.fail("a uniform list of more labels than the mesh has faces");
+    const long v = t.integer();
+    t.open('}');
+    return std::vector<long>(static_cast<std::size_t>(n), v);
+  }
+  FoamText again(slurp(dir + "/" + file), file);
+  return label_list(again);
+}
+
+// The boundary file's patch dictionaries, each read as a unit: name { key value; ... }.
+struct Patch {
+  std::string name, type;
+  long start = -1, count = -1;
+};
+
+std::vector<Patch> boundary_patches(const std::string& text) {
+  const auto fail = [](const std::string& why) -> void {
+    throw std::runtime_error("E0132: polyMesh boundary: " + why);
+  };
+  std::size_t p = text.find("FoamFile");
+  p = p == std::string::npos ? 0 : text.find('}', p);
+  if (p == std::string::npos) fail("an unterminated FoamFile header");
+  // Tokens: words, '{', '}', ';', '(' ... ')' groups kept whole; comments skipped.
+  std::vector<std::string> tok;
+  for (std::size_t i = p == 0 ? 0 : p + 1; i < text.size();) {
+    const char c = text[i];
+    if (std::isspace(static_cast<unsigned char>(c))) {
+      ++i;
+    } else if (text.compare(i, 2, "//") == 0) {
+      const std::size_t q = text.find('\n', i);
+      i = q == std::string::npos ? text.size() : q;
+    } else if (text.compare(i, 2, "/*") == 0) {
+      const std::size_t q = text.find("*/", i);
+      if (q == std::string::npos) fail("an unterminated comment");
+      i = q + 2;
+    } else if (c == '{' || c == '}' || c == ';' || c == '(' || c == ')') {
+      tok.emplace_back(1, c);
+      ++i;
+    } else if (c == '"') {
+      const std::size_t q = text.find('"', i + 1);
+      if (q == std::string::npos) fail("an unterminated string");
+      tok.push_back(text.substr(i, q + 1 - i));
+      i = q + 1;
+    } else {
+      const std::size_t b = i;
+      while (i < text.size() && !std::isspace(static_cast<unsigned char>(text[i])) &&
+             std::strchr("{};()\"", text[i]) == nullptr)
+        ++i;
+      tok.push_back(text.substr(b, i - b));
+    }
+  }
+  std::size_t k = 0;
+  const auto next = [&]() -> const std::string& {
+    if (k >= tok.size()) fail("the file ends inside the patch list");
+    return tok[k++];
+  };
+  const auto integer = [&](const std::string& w, const std::string& what) {
+    char* e = nullptr;
+    errno = 0;
+    const long v = std::strtol(w.c_str(), &e, 10);
+    if (w.empty() || *e != '\0' || errno != 0) fail(what + " is not an integer");
+    return v;
+  };
+  const long n = integer(next(), "the patch count");
+  if (n < 0 || static_cast<std::size_t>(n) > tok.size()) fail("a patch count larger than the file can hold");
+  if (next() != "(") fail("expected '(' after the patch count");
+  std::vector<Patch> out;
+  for (long i = 0; i < n; ++i) {
+    Patch patch;
+    patch.name = next();
+    if (patch.name == "(" || patch.name == ")" || patch.name == "{" || patch.name == "}" || patch.name == ";")
+      fail("expected a patch name");
+    if (next() != "{") fail("patch '" + patch.name + "' has no dictionary");
+    // key value ... ; entries until the closing brace (nested dictionaries and lists skipped whole)
+    for (;;) {
+      const std::string key = next();
+      if (key == "}") break;
+      std::vector<std::string> value;
+      int depth = 0;
+      for (;;) {
+        const std::string w = next();
+        if (w == "{" || w == "(") ++depth;
+        if (w == "}" || w == ")") {
+          if (depth == 0) fail("patch '" + patch.name + "': unbalanced '" + w + "'");
+          --depth;
+          if (depth == 0 && w == "}") break;  // a sub-dictionary ends its entry without ';'
+          continue;
+        }
+        if (depth == 0 && w == ";") break;
+        if (depth == 0) value.push_back(w);
+      }
+      if (key == "type" || key == "nFaces" || key == "startFace") {
+        if (value.size() != 1) fail("patch '" + patch.name + "': '" + key + "' takes one value");
+        if (key == "type") patch.type = value[0];
+        if (key == "nFaces") patch.count = integer(value[0], "patch '" + patch.name + "' nFaces");
+        if (key == "startFace") patch.start = integer(value[0], "patch '" + patch.name + "' startFace");
+      }
+    }
+    if (patch.type.empty()) fail("patch '" + patch.name + "' has no 'type'");
+    if (patch.count < 0) fail("patch '" + patch.name + "' has no non-negative 'nFaces'");
+    if (patch.start < 0) fail("patch '" + patch.name + "' has no non-negative 'startFace'");
+    out.push_back(std::move(patch));
+  }
+  if (next() != ")") fail("more patches than the count says");
+  return out;
 }
 
 }  // namespace
@@ -682,7 +803,7 @@ Mesh read_polymesh(const std::string& dir) {
   std::vector<double> x, y, z;
   {
     FoamText t(slurp(dir + "/points"), "points");
-    const std::size_t n = t.count(t.binary() ? 24 : 7);  // three doubles, or "(a b c)"
+    const std::size_t n = t.count(t.binary() ? 3 * t.scalar_bytes() : 7);  // three scalars, or "(a b c)"
     t.open('(');
     x.reserve(n);
     y.reserve(n);
@@ -737,9 +858,9 @@ Mesh read_polymesh(const std::string& dir) {
       f.off.push_back(static_cast<offset_t>(f.nodes.size()));
     }
   }
-  const auto own = labels(dir, "owner");
-  const auto nei = labels(dir, "neighbour");
   const std::size_t nf = f.off.size() - 1;
+  const auto own = labels(dir, "owner", nf);
+  const auto nei = labels(dir, "neighbour", nf);
   if (own.size() != nf) throw std::runtime_error("E0132: polyMesh: owner and faces differ in length");
   if (nei.size() > nf) throw std::runtime_error("E0132: polyMesh: more neighbours than faces");
   long n_cells = 0;
@@ -765,24 +886,28 @@ Mesh read_polymesh(const std::string& dir) {
       if (k < 4) throw std::runtime_error("E0132: polyMesh: a cell with fewer than 4 faces");
     }
   }
-  // boundary: patches in order, each nFaces from startFace, within the boundary faces.
+  // boundary: the patches in order, each continuing the boundary face range where the previous
+  // one ends (the first at the first boundary face), together covering every boundary face, as
+  // io/src/polymesh.cpp:486-509 requires; processor patches (a decomposed mesh) are refused.
   std::vector<index_t> zone(nf, -1);
   {
-    const std::string s = slurp(dir + "/boundary");
-    std::size_t p = 0;
+    long expected = static_cast<long>(nei.size());
     index_t patch = 0;
-    while ((p = s.find("nFaces", p)) != std::string::npos) {
-      const long count = std::strtol(s.c_str() + p + 6, nullptr, 10);
-      const std::size_t q = s.find("startFace", p);
-      if (q == std::string::npos) throw std::runtime_error("E0132: polyMesh boundary: nFaces without startFace");
-      const long start = std::strtol(s.c_str() + q + 9, nullptr, 10);
-      if (count < 0 || start < static_cast<long>(nei.size()) || start > static_cast<long>(nf) ||
-          count > static_cast<long>(nf) - start) {
-        throw std::runtime_error("E0132: polyMesh boundary: a patch outside the boundary faces");
+    for (const Patch& pt : boundary_patches(slurp(dir + "/boundary"))) {
+      if (pt.type == "processor" || pt.type == "processorCyclic") {
+        throw std::runtime_error("E0133: polyMesh boundary: unsupported decomposed mesh (processor patch '" +
+                                 pt.name + "')");
       }
-      for (long i = start; i < start + count; ++i) zone[static_cast<std::size_t>(i)] = patch;
+      if (pt.start != expected || pt.count > static_cast<long>(nf) - pt.start) {
+        throw std::runtime_error("E0132: polyMesh boundary: patch '" + pt.name +
+                                 "' does not continue the boundary face range");
+      }
+      for (long i = pt.start; i < pt.start + pt.count; ++i) zone[static_cast<std::size_t>(i)] = patch;
+      expected = pt.start + pt.count;
       ++patch;
-      p = q;
+    }
+    if (expected != static_cast<long>(nf)) {
+      throw std::runtime_error("E0132: polyMesh boundary: the patches do not cover every boundary face");
     }
   }
   for (std::size_t i = 0; i < nf; ++i) {
@@ -1262,6 +1387,16 @@ Mesh read_gmsh(const std::string& path) {
     }
   }
   if (!have_elements || cell_shape.empty()) throw std::runtime_error("E0132: gmsh: no volume elements");
+  // Periodic surfaces pair zones: both surfaces of a link must be in a physical group
+  // (io/src/gmsh.cpp:178-182).
+  for (const Link& l : links) {
+    for (const long long e : {l.master, l.slave}) {
+      const auto it = physical.find({2, e});
+      if (it == physical.end() || it->second == 0) {
+        throw std::runtime_error("E0132: gmsh: periodic surface " + std::to_string(e) + " is in no physical group");
+      }
+    }
+  }
 
   // Zones: the 2-D physical groups in increasing tag order, then one zone for boundary faces of no group.
   std::sort(surface_groups.begin(), surface_groups.end());

@@ -1,24 +1,25 @@
 """Benchmark: SSP-RK3 steps of the explicit Euler residual on one MI355X.
 
-Default workload (the manufactured-solution tetrahedra, the configuration the metric is quoted on
-that fits one GPU): the unit cube in n^3 cubes of 6 Kuhn tetrahedra (n = 128: 12,582,912 cells),
-interior nodes jittered by +-10 % of the spacing (seed 0), cells renumbered along a Morton curve; a
-smooth manufactured Euler solution (rho, u, v, w, p sinusoids of amplitude 0.1 about rho = 1,
-p = 1/1.4, Mach 0.3) with its volume source and exact `state` boundary faces; second-order HLLC with
-face2 least-squares gradients and the Venkatakrishnan limiter (K = 5), fp32+acc64, SSP-RK3 with the
-global CFL 0.5 time step. One step = one SSP-RK3 step (three residuals, the norms of R(u), the
-spectral radius and time step, validity checks).
+Default workload (the roofline target: second-order HLLC in fp32+acc64 on about 8 M polyhedral
+cells): the hybrid bump channel, a 3 x 1 x 1 channel with a Gaussian bump (height 0.1, width 0.2)
+on its lower wall, meshed as a 462 x 154 x 154 hexahedral lattice with prism rows on the wall and a
+band of merged polyhedra with non-planar faces (7,915,215 cells), cells renumbered along a Morton
+curve; inflow Mach 0.5 from the freestream start, slip walls, characteristic farfield; face2
+least-squares gradients with the Venkatakrishnan limiter, SSP-RK3 on local time steps (CFL 0.8).
+One step = one SSP-RK3 step (three residuals, the norms of R(u), the spectral radius and time
+steps, validity checks). The run also reports a steady run's residual drop per second, the
+conservation check and the fp32+acc64 residual's distance from the fp64 one on the same state.
 
-Other workloads (--workload): `vortex`, the isentropic vortex on 32^3 periodic hexahedra
-(first-order Roe, fp64, 200 steps, compared with the CPU oracle's 200 steps); `bump`, the hybrid
-bump channel (about 8 M cells with a prism layer and a band of merged polyhedra; local time steps),
-which also reports a steady run's residual drop per second, the conservation check and the
-fp32+acc64 residual's distance from the fp64 one.
+Other workloads (--workload): `tets`, the manufactured-solution Kuhn tetrahedra (128^3 x 6 =
+12,582,912 cells, jittered, HLLC, global CFL 0.5); `vortex`, the isentropic vortex on 32^3 periodic
+hexahedra (first-order Roe, fp64, 200 steps, compared with the CPU oracle's 200 steps); `sweep`,
+`warmstart`, `mms`, `mlp` (their own lines).
 
 Prints one JSON line: ns per cell per step (lower is better), the roofline of the dominant kernel
 (algorithmic bytes per launch over its event-timed duration, against the 8 TB/s HBM peak), and a
-CPU baseline: the oracle (this repo's C++ restatement of the reference's CPU loops) timed on a
-bounded sample on this host, serial and on every host thread the process may use.
+CPU baseline: the oracle (this repo's C++ restatement of the reference's CPU loops) timed on the
+same mesh and state for a bounded number of steps on this host, serial and on every host thread the
+process may use.
 """
 
 from __future__ import annotations
@@ -456,7 +457,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (tets 20, bump 20, vortex 200)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="tets",
+    ap.add_argument("--workload", default="bump",
                     choices=["tets", "vortex", "bump", "sweep", "warmstart", "mms", "mlp"])
     ap.add_argument("--levels", default="64,96,128", help="mms: tetrahedra levels n (n^3 x 6 cells)")
     ap.add_argument("--no-train", action="store_true", help="mlp: inference only (the training step is a recorded "
@@ -477,7 +478,8 @@ def main() -> int:
     ap.add_argument("--policy", default=None, choices=["fp32+acc64", "fp64"])
     ap.add_argument("--flux", default=None, choices=["roe", "hllc", "ausm+up"])
     ap.add_argument("--cpu-n", type=int, default=48, help="tets: cubes per side of the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="steps of the CPU baseline (bump 1 on the same "
+                    "mesh, tets 4 on the --cpu-n sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--steady-iterations", type=int, default=400)
@@ -546,14 +548,12 @@ def main() -> int:
     elif args.workload == "bump":
         extras = {} if args.no_extras else bump_extras(dev, w, args)
         if not args.no_cpu:
-            log("CPU baseline (oracle) on a lattice of half the size per axis")
-            nx, ny, nz = size
-            small = make_workload("bump", args, (nx // 4 * 2, ny // 2, nz // 4 * 2))
-            cpu, _ = cpu_baseline(small, args.cpu_steps, "the same channel on a lattice of half the size per axis")
+            log("CPU baseline (oracle) on the same mesh and state")
+            cpu, _ = cpu_baseline(w, args.cpu_steps or 1, "the same mesh from the same freestream state")
     elif not args.no_cpu:
         log("CPU baseline (oracle)")
         small = make_workload("tets", args, args.cpu_n)
-        cpu, _ = cpu_baseline(small, args.cpu_steps, f"the same workload at {args.cpu_n}^3 x 6 tets")
+        cpu, _ = cpu_baseline(small, args.cpu_steps or 4, f"the same workload at {args.cpu_n}^3 x 6 tets")
 
     line = {
         "metric": "ns/cell/iter",

@@ -128,3 +128,103 @@ def test_autotune_restores_the_state_and_keeps_the_bits():
         d = solo(c, "fp32+acc64")
         d.upload(c.u)
         np.testing.assert_array_equal(r[j], d.step(cfl=0.8, local=True).residuals)
+
+
+
+def sweep_cases(policy, inflow="farfield"):
+    """Cases of the sweep's own generator and builder (seed 1: the drawn heights, Mach numbers and
+    angles; two prism rows, the warped-face loops), on meshes small enough for the oracle. With
+    inflow "state", the channel's farfield faces take the freestream as `state` data, so that no
+    face calls exp/log."""
+    from autan_amd import mesh as meshes
+    from autan_amd import sweep
+
+    specs = sweep.specs(4, seed=1, cells=(2e3, 3e4))
+    cs = [sweep.build(s, policy) for s in specs]
+    if inflow == "state":
+        for c in cs:
+            kinds = np.where(c.bc_kind == meshes.FARFIELD, float(meshes.STATE), c.bc_kind)
+            c.bc_kind = kinds
+            meshes.set_boundary_kinds(c.mesh, kinds.astype(np.int32))
+    return specs, cs
+
+
+@pytest.mark.parametrize("inflow", ["state", "farfield"])
+@pytest.mark.parametrize("policy", ["fp64", "fp32+acc64"])
+def test_batch_steps_match_the_oracle(policy, inflow):
+    """The batch path against the CPU oracle, case by case: states, norms and time steps of local
+    and global SSP-RK3 steps. Bit for bit where no face calls exp/log; with farfield faces (exp/log
+    from glibc on the host, from the device library on the GPU) within the rounding those leave,
+    each step continuing from the device's state."""
+    from oracle import oracle as orc
+    from oracle.oracle import Oracle
+
+    orc.set_threads(8)
+    try:
+        _, cs = sweep_cases(policy, inflow)
+        b = batch.BatchStepper(cs, policy)
+        b.upload()
+        oracles = [Oracle(c.mesh, c.scheme, policy, c.bc_kind, c.bc) for c in cs]
+        states = [c.u.copy() for c in cs]
+        exact = inflow == "state"
+        for it in range(4):
+            local = it < 3
+            cfl = 0.8 if local else 0.5
+            r, h, inv = b.step(cfl=cfl, local=local)
+            got = b.download()
+            for j, ora in enumerate(oracles):
+                u, norms, dt, bad = ora.step(states[j], cfl=cfl, local=local)
+                assert inv[j] is None and bad is None
+                if exact:
+                    np.testing.assert_array_equal(got[j], u)
+                    np.testing.assert_array_equal(r[j], norms)
+                    if not local:
+                        assert h[j] == dt
+                else:
+                    tol = 1e-12 if policy == "fp64" else 1e-7
+                    scale = np.abs(u).reshape(5, -1).max(axis=1)[:, None]
+                    assert np.all(np.abs(got[j].reshape(5, -1) - u.reshape(5, -1)) <= tol * scale), (it, j)
+                    np.testing.assert_allclose(r[j], norms, rtol=1e-9 if policy == "fp64" else 1e-4)
+                    if not local:
+                        assert h[j] == pytest.approx(dt, rel=1e-9 if policy == "fp64" else 1e-5)
+            states = [g.copy() for g in got]
+    finally:
+        orc.set_threads(1)
+
+
+@pytest.mark.parametrize("inflow", ["state", "farfield"])
+def test_batch_steady_run_matches_the_oracle(inflow):
+    """A short steady run of every sweep case in one batch (first-order start, local steps, the
+    convergence rule with each case's round-off floors) and the same run of the oracle: the same
+    iteration counts, first-order start and status; the residual histories bit for bit where no
+    face calls exp/log."""
+    from autan_amd import sweep
+    from oracle import oracle as orc
+    from oracle.oracle import Oracle
+
+    import cases as tc
+
+    orc.set_threads(8)
+    try:
+        policy = "fp32+acc64"
+        _, cs = sweep_cases(policy, inflow)
+        b = batch.BatchStepper(cs, policy)
+        b.tune("graph", 1)
+        b.upload()
+        scales = np.array([sweep.residual_scales(c) for c in cs])
+        kw = dict(max_iterations=240, residual_orders=8.0, start_order="first")
+        got = b.steady(scales=scales, **kw)
+        states = b.download()
+        for j, c in enumerate(cs):
+            target = tc.OracleTarget(Oracle(c.mesh, c.scheme, policy, c.bc_kind, c.bc), c.u)
+            want = steady.solve_with(target, level_rule=True, scales=scales[j], **kw)
+            assert got[j].iterations == want.iterations and got[j].status == want.status, j
+            assert got[j].first_order_steps == want.first_order_steps
+            assert got[j].first_order_end == want.first_order_end
+            if inflow == "state":
+                np.testing.assert_array_equal(got[j].history, want.history)
+                np.testing.assert_array_equal(states[j], target.u)
+            else:
+                np.testing.assert_allclose(got[j].history, want.history, rtol=1e-3, atol=0)
+    finally:
+        orc.set_threads(1)
