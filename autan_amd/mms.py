@@ -17,6 +17,31 @@ AMPLITUDE = 0.1
 MACH = 0.3
 
 
+def euler_source(rho, u, v, w, p, coords, gamma):
+    """s = div F(W) of primitive fields given as sympy expressions of `coords` (x, y, z): the
+    steady Euler fluxes rho v_d, rho v v_d + p e_d, (E + p) v_d with E = p/(gamma - 1) + rho |v|^2/2,
+    each differentiated along its own coordinate and summed. Returns the five source expressions
+    (mass, three momenta, energy)."""
+    import sympy as sp
+
+    g = sp.Float(gamma) if not isinstance(gamma, sp.Basic) else gamma
+    e = p / (g - 1) + rho * (u * u + v * v + w * w) / 2
+    vel = (u, v, w)
+    fluxes = []
+    for d in range(3):
+        vd = vel[d]
+        fluxes.append(
+            [
+                rho * vd,
+                rho * u * vd + (p if d == 0 else 0),
+                rho * v * vd + (p if d == 1 else 0),
+                rho * w * vd + (p if d == 2 else 0),
+                (e + p) * vd,
+            ]
+        )
+    return [sum(sp.diff(fluxes[d][k], xd) for d, xd in enumerate(coords)) for k in range(5)]
+
+
 @functools.cache
 def _functions(gamma: float = GAMMA, amp: float = AMPLITUDE, mach: float = MACH):
     import sympy as sp
@@ -30,22 +55,7 @@ def _functions(gamma: float = GAMMA, amp: float = AMPLITUDE, mach: float = MACH)
     v = mach * amp * sp.sin(tau * x + sp.Rational(1, 4)) * sp.cos(tau * z - sp.Rational(3, 10))
     w = mach * amp * sp.cos(tau * y - sp.Rational(2, 5)) * sp.sin(tau * z + sp.Rational(1, 10))
     p = (1 + amp * sp.cos(tau * x + sp.Rational(3, 5)) * sp.cos(tau * y - sp.Rational(1, 3)) * sp.cos(tau * z)) / gamma
-    g = sp.Float(gamma)
-    e = p / (g - 1) + rho * (u * u + v * v + w * w) / 2
-    vel = (u, v, w)
-    fluxes = []
-    for d, xd in enumerate((x, y, z)):
-        vd = vel[d]
-        fluxes.append(
-            [
-                rho * vd,
-                rho * u * vd + (p if d == 0 else 0),
-                rho * v * vd + (p if d == 1 else 0),
-                rho * w * vd + (p if d == 2 else 0),
-                (e + p) * vd,
-            ]
-        )
-    source = [sum(sp.diff(fluxes[d][k], xd) for d, xd in enumerate((x, y, z))) for k in range(5)]
+    source = euler_source(rho, u, v, w, p, (x, y, z), gamma)
     prim = sp.lambdify((x, y, z), [rho, u, v, w, p], "numpy")
     src = sp.lambdify((x, y, z), source, "numpy")
     return prim, src
