@@ -181,9 +181,15 @@ def test_batch_steps_match_the_oracle(policy, inflow):
                     if not local:
                         assert h[j] == dt
                 else:
-                    tol = 1e-12 if policy == "fp64" else 1e-7
-                    scale = np.abs(u).reshape(5, -1).max(axis=1)[:, None]
-                    assert np.all(np.abs(got[j].reshape(5, -1) - u.reshape(5, -1)) <= tol * scale), (it, j)
+                    # The state's own magnitudes: density, momentum (one scale for its three
+                    # components: the spanwise one is near zero in these channels, while a
+                    # rounding difference in a boundary pressure moves every component alike),
+                    # total energy. 64 units of the compute type of that scale.
+                    tol = 1e-12 if policy == "fp64" else 64 * float(np.finfo(np.float32).eps)
+                    a = np.abs(u).reshape(5, -1).max(axis=1)
+                    scale = np.array([a[0], a[1:4].max(), a[1:4].max(), a[1:4].max(), a[4]])[:, None]
+                    diff = np.abs(got[j].reshape(5, -1) - u.reshape(5, -1))
+                    assert np.all(diff <= tol * scale), (it, j, (diff / scale).max(axis=1))
                     np.testing.assert_allclose(r[j], norms, rtol=1e-9 if policy == "fp64" else 1e-4)
                     if not local:
                         assert h[j] == pytest.approx(dt, rel=1e-9 if policy == "fp64" else 1e-5)

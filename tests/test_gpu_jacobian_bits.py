@@ -1,7 +1,4 @@
-"""Not collected by pytest until it has passed on a GPU: run it by name,
-`python -m pytest tests/wip_gpu_jacobian_bits.py -m gpu`.
-
-The device's Jacobian blocks (mxeuler_residual_and_jacobian) against the oracle's (the same flux,
+"""The device's Jacobian blocks (mxeuler_residual_and_jacobian) against the oracle's (the same flux,
 reconstruction and boundary bodies on the oracle's own dual numbers, pinned by central differences
 in test_oracle_jacobian.py), face by face and in the policy's compute type: bit for bit, except at
 farfield and inlet faces, whose boundary states call exp / log (the device's and glibc's differ in
