@@ -15,6 +15,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -91,6 +92,36 @@ int code_of(double kind) {
 
 }  // namespace
 
+// ---------------------------------------------------------------- the static face-force staging
+//
+// mxeuler_face_force_noalloc stages its face list and its forces here instead of in buffers of its
+// own: at most kStaticForceFaces faces, the list (int32) first, the forces (3 FP64 per face, SoA by
+// the call's face count) from byte offset kStaticForceOut. Each GPU holds its own copy for the life
+// of the process. In host code `scratch` names a host-side stand-in: its device address on each GPU
+// comes from hipGetSymbolAddress, taken once per device when a stepper is created there.
+constexpr long long kStaticForceFaces = MXEULER_STATIC_FORCE_FACES;
+constexpr std::size_t kStaticForceOut = std::size_t(kStaticForceFaces) * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kStaticForceOut + 3 * std::size_t(kStaticForceFaces) * sizeof(double)];
+static_assert(kStaticForceOut % 256 == 0, "the forces start on a 256-byte boundary");
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+void* scratch_address[kMaxDevices] = {};
+
+// The calling thread's current device's address of `scratch`, looked up on the first call there.
+void* static_scratch() {
+  int dev = -1;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("E1101: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (scratch_address[dev] == nullptr) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    scratch_address[dev] = ptr;
+  }
+  return scratch_address[dev];
+}
+
 class StepperBase {
  public:
   virtual ~StepperBase() = default;
@@ -111,6 +142,7 @@ class StepperBase {
   virtual void spectral_radius(double* lam) = 0;
   virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
   virtual void face_force(const int* faces, long long n, double* force) = 0;
+  virtual void face_force_noalloc(const int* faces, long long n, double* force) = 0;
   virtual void conservation(double* out) = 0;
   virtual void tune(const std::string& key, int value) = 0;
   virtual double area_scale() const = 0;
@@ -154,6 +186,8 @@ class Stepper final : public StepperBase {
       check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
       own_stream_ = true;
     }
+    // This device's address of the static face-force staging (mxeuler_face_force_noalloc).
+    scratch_ = static_scratch();
     nc_ = t.n_cells;
     nf_ = t.n_faces;
     ni_ = t.n_interior;
@@ -391,15 +425,36 @@ class Stepper final : public StepperBase {
     if (n > INT32_MAX / 3) throw std::invalid_argument("face_force: too many faces");
     Buffer<int> list(static_cast<std::size_t>(n));
     Buffer<double> out(3 * static_cast<std::size_t>(n));
-    list.upload(faces, static_cast<std::size_t>(n), stream_);
+    face_force_into(faces, n, force, list.get(), out.get());
+  }
+
+  // face_force with its staging in the device's static array: no allocation; at most
+  // kStaticForceFaces faces.
+  void face_force_noalloc(const int* faces, long long n, double* force) override {
+    if (n > kStaticForceFaces) {
+      throw std::invalid_argument("face_force_noalloc: " + std::to_string(n) + " faces, more than its static staging holds (" +
+                                  std::to_string(kStaticForceFaces) + ")");
+    }
+    for (long long i = 0; i < n; ++i) {
+      if (faces[i] < ni_ || faces[i] >= nf_) throw std::invalid_argument("face_force: not a boundary face");
+    }
+    if (n == 0) return;
+    auto* base = static_cast<unsigned char*>(scratch_);
+    face_force_into(faces, n, force, reinterpret_cast<int*>(base), reinterpret_cast<double*>(base + kStaticForceOut));
+  }
+
+  // The face list staged in `list` (n int32), the forces in `out` (3 n FP64), then copied back.
+  void face_force_into(const int* faces, long long n, double* force, int* list, double* out) {
+    check(hipMemcpyAsync(list, faces, static_cast<std::size_t>(n) * sizeof(int), hipMemcpyHostToDevice, stream_),
+          "hipMemcpyAsync H2D");
     launch_prim(u_.get());
     residual_from_prim();
     const int ni = static_cast<int>(n);
     launch("face_force", [&] {
-      k_face_force<C><<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(dev(), list.get(), ni, rb_.get(),
-                                                                          out.get());
+      k_face_force<C><<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(dev(), list, ni, rb_.get(), out);
     });
-    out.download(force, 3 * static_cast<std::size_t>(n), stream_);
+    check(hipMemcpyAsync(force, out, 3 * static_cast<std::size_t>(n) * sizeof(double), hipMemcpyDeviceToHost, stream_),
+          "hipMemcpyAsync D2H");
     check(hipStreamSynchronize(stream_), "face_force");
   }
 
@@ -1597,6 +1652,7 @@ class Stepper final : public StepperBase {
   double area_scale_ = 0.0;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
+  void* scratch_ = nullptr;  // this device's `scratch`
   int nc_ = 0, nf_ = 0, ni_ = 0, nb_ = 0;
   bool second_ = true, planar_ = true;
   int ncase_ = 1;
@@ -1894,6 +1950,14 @@ int mxeuler_face_force(mxeuler_stepper* s, const int32_t* faces, int64_t n, doub
   return guarded(err, err_len, [&] {
     if (n < 0 || (n > 0 && (faces == nullptr || force == nullptr))) throw std::invalid_argument("null argument");
     s->impl->face_force(faces, n, force);
+  });
+}
+
+int mxeuler_face_force_noalloc(mxeuler_stepper* s, const int32_t* faces, int64_t n, double* force, char* err,
+                               size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (n < 0 || (n > 0 && (faces == nullptr || force == nullptr))) throw std::invalid_argument("null argument");
+    s->impl->face_force_noalloc(faces, n, force);
   });
 }
 

@@ -136,6 +136,10 @@ def _check(rc: int, err) -> None:
         raise RuntimeError(err.value.decode(errors="replace"))
 
 
+# MXEULER_STATIC_FORCE_FACES: the most faces one mxeuler_face_force_noalloc call stages.
+STATIC_FORCE_FACES = 131072
+
+
 class DeviceStepper:
     """make_device_stepper<P> over the HIP library; arrays are FP64 SoA (5 x n_cells)."""
 
@@ -333,6 +337,15 @@ class DeviceStepper:
         f = np.ascontiguousarray(faces, dtype=np.int32)
         out = np.empty(3 * max(f.size, 1))
         self._call(self.lib.mxeuler_face_force, _p(f.ctypes.data), ctypes.c_int64(f.size), _p(out.ctypes.data))
+        return out[: 3 * f.size].reshape(3, f.size)
+
+    def face_force_noalloc(self, faces: np.ndarray) -> np.ndarray:
+        """face_force staged in the library's static device array (mxeuler_face_force_noalloc): no
+        device allocation, at most STATIC_FORCE_FACES faces; the same results."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        out = np.empty(3 * max(f.size, 1))
+        self._call(self.lib.mxeuler_face_force_noalloc, _p(f.ctypes.data), ctypes.c_int64(f.size),
+                   _p(out.ctypes.data))
         return out[: 3 * f.size].reshape(3, f.size)
 
     def loads(self, faces: np.ndarray, center=(0.0, 0.0, 0.0), p_inf: float = 0.0) -> tuple[np.ndarray, np.ndarray]:

@@ -173,6 +173,15 @@ int mxeuler_stream(int64_t n, int32_t repeats, double* gbps, double* ms, char* e
  * p_f S + phi_p M^T grad p in FP64, force 3 x n SoA. */
 int mxeuler_face_force(mxeuler_stepper* s, const int32_t* faces, int64_t n, double* force, char* err,
                        size_t err_len);
+/* mxeuler_face_force without allocating: no counterpart in the reference. For callers that cannot
+ * allocate device memory (tight monitor loops, code that forbids hipMalloc). The face list and the
+ * forces are staged in a static device array that each GPU holds until the process ends, at most
+ * MXEULER_STATIC_FORCE_FACES faces per call (more is refused, status 1). Every call on one GPU shares
+ * that array: calls that overlap on one GPU (two threads, or steppers on different streams) race.
+ * Same results as mxeuler_face_force. */
+#define MXEULER_STATIC_FORCE_FACES 131072
+int mxeuler_face_force_noalloc(mxeuler_stepper* s, const int32_t* faces, int64_t n, double* force, char* err,
+                               size_t err_len);
 /* EulerPhysics::loads (physics/src/euler_physics.cpp:527-562; wall_force :930-948): exact sums over
  * the listed boundary faces of the face force minus p_inf S_f, and of its moment about `center`.
  * area_vector and face_centroid (3 x n SoA, FP64) are the caller's geometry of the listed faces. */
