@@ -366,11 +366,11 @@ __device__ __forceinline__ F5<C> flux_increment(const W5<C>& w, const W5<C>& dw,
            drh * ve + rh * dve}};
 }
 
-// kernels/euler.py:653-677: warped-face term; m is the 3 x 3 moment, row-major.
+// kernels/euler.py:653-677: the warped-face term's three directions
+// d_j = (phi_L grad W_L + phi_R grad W_R) M[:, j] / 2; m is the 3 x 3 moment, row-major.
 template <class C>
-__device__ __forceinline__ F5<C> warp(const W5<C>& w, const C* gl, const C* pl, const C* gr, const C* pr,
-                                      const C* m, C g, C pref) {
-  W5<C> d[3];
+__device__ __forceinline__ void warp_directions(const C* gl, const C* pl, const C* gr, const C* pr, const C* m,
+                                                W5<C>* d) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const C c0 = m[j], c1 = m[3 + j], c2 = m[6 + j];
@@ -383,13 +383,30 @@ __device__ __forceinline__ F5<C> warp(const W5<C>& w, const C* gl, const C* pl, 
     }
     d[j] = {t[0], t[1], t[2], t[3], t[4]};
   }
-  const F5<C> f0 = flux_increment<0>(w, d[0], g, pref);
-  const F5<C> f1 = flux_increment<1>(w, d[1], g, pref);
-  const F5<C> f2 = flux_increment<2>(w, d[2], g, pref);
-  F5<C> out;
+}
+
+// The term itself, sum_j dF_j[W; d_j], for the directions of warp_directions. T may be a dual
+// number (the Jacobian's), the directions then held fixed.
+template <class T, class C>
+__device__ __forceinline__ F5<T> warp_of_directions(const W5<T>& w, const W5<C>* d, T g, T pref) {
+  const W5<T> d0{T(d[0].r), T(d[0].u), T(d[0].v), T(d[0].w), T(d[0].p)};
+  const W5<T> d1{T(d[1].r), T(d[1].u), T(d[1].v), T(d[1].w), T(d[1].p)};
+  const W5<T> d2{T(d[2].r), T(d[2].u), T(d[2].v), T(d[2].w), T(d[2].p)};
+  const F5<T> f0 = flux_increment<0>(w, d0, g, pref);
+  const F5<T> f1 = flux_increment<1>(w, d1, g, pref);
+  const F5<T> f2 = flux_increment<2>(w, d2, g, pref);
+  F5<T> out;
 #pragma unroll
   for (int k = 0; k < 5; ++k) out.f[k] = f0.f[k] + f1.f[k] + f2.f[k];
   return out;
+}
+
+template <class C>
+__device__ __forceinline__ F5<C> warp(const W5<C>& w, const C* gl, const C* pl, const C* gr, const C* pr,
+                                      const C* m, C g, C pref) {
+  W5<C> d[3];
+  warp_directions(gl, pl, gr, pr, m, d);
+  return warp_of_directions<C, C>(w, d, g, pref);
 }
 
 // kernels/euler.py:1091-1112 (wave_speed), before the area.

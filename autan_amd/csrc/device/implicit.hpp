@@ -1,0 +1,380 @@
+// The steady implicit step on the device: pseudo-transient continuation
+// (V / dt + dR/dU) du = -R with the approximate Jacobian, as EulerPhysics::steady_step takes it
+// with linear_solver = "gauss-seidel" (physics/src/euler_physics.cpp:1056-1250):
+//   k_impl_diag, k_impl_off   Euler::assemble (physics/src/euler.cpp:993-1053): per cell
+//                             lambda_c / CFL I + the sum over its faces, in cell_faces order, of
+//                             +-J D_c; per interior face A(owner, neighbour) = J_R D_neighbour and
+//                             A(neighbour, owner) = -J_L D_owner; each J D entry summed over k from 0
+//                             and added as sign * t;
+//   k_impl_invert             BlockMatrix::factor: each diagonal block's inverse by Gauss-Jordan
+//                             elimination with partial pivoting (physics/src/block_matrix.cpp:14-45);
+//   k_impl_relax              BlockMatrix::relax over the cells of one colour (a colour's cells share
+//                             no face, so they update together);
+//   k_impl_multiply           BlockMatrix::multiply;
+//   k_impl_dot, k_impl_maxabs the GCR's inner products as exact sums and its largest entry
+//                             (EulerPhysics::owned_dot, owned_max_abs);
+//   k_impl_admissible         EulerPhysics::admissible_step: each cell's largest fraction of du that
+//                             keeps its density and pressure above half their values, the least over
+//                             the cells (an exact minimum of the FP64 values);
+// and the GCR's vector updates (solvers/src/gcr.cpp). Everything in FP64, no contraction, so every
+// value equals the host restatement's. Matrix layout: diagonal and inverse blocks 25 x n SoA
+// (entry 5 i + j of cell c at (5 i + j) n + c), off-diagonal blocks 25 contiguous values per block,
+// the upper (row owner) blocks of the interior faces first, then the lower (row neighbour) ones.
+#pragma once
+
+#include "jacobian.hpp"
+
+namespace mxe {
+
+// out += sign * (j d) for 5 x 5 row-major blocks, each entry's product summed over k from zero.
+__device__ __forceinline__ void impl_add_product(double* out, const double* j, const double* dm, double sign) {
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) t += j[5 * r + k] * dm[5 * k + c];
+      out[5 * r + c] += sign * t;
+    }
+  }
+}
+
+__device__ __forceinline__ void impl_block(const double* __restrict__ v, long long n, long long e, int k, double* b) {
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b[q] = v[(25LL * k + q) * n + e];
+}
+
+// The diagonal blocks: lam_c / cfl on the diagonal, then the cell's faces in cell_faces order.
+// dwdu (25 x nc), fj (50 x ni), bj (25 x nb) as k_dwdu and k_face_jacobian write them.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __restrict__ dwdu,
+                                                      const double* __restrict__ fj, const double* __restrict__ bj,
+                                                      double cfl, double* __restrict__ diag) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.nc) return;
+  double dg[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) dg[q] = 0.0;
+  const double shift = static_cast<double>(d.lam[c]) / cfl;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) dg[6 * i] += shift;
+  double dc[25];
+  impl_block(dwdu, d.nc, c, 0, dc);
+  for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+    const int e = d.inc[k];
+    const int f = e & 0x7fffffff;
+    double j[25];
+    if (f >= d.ni) {
+      impl_block(bj, d.nb, f - d.ni, 0, j);
+      impl_add_product(dg, j, dc, 1.0);
+    } else if (e >= 0) {
+      impl_block(fj, d.ni, f, 0, j);
+      impl_add_product(dg, j, dc, 1.0);
+    } else {
+      impl_block(fj, d.ni, f, 1, j);
+      impl_add_product(dg, j, dc, -1.0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 25; ++q) diag[static_cast<long long>(q) * d.nc + c] = dg[q];
+}
+
+// The off-diagonal blocks of every interior face: offd[25 f ..] = J_R D_neighbour (row owner),
+// offd[25 (ni + f) ..] = -J_L D_owner (row neighbour).
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __restrict__ dwdu,
+                                                     const double* __restrict__ fj, double* __restrict__ offd) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= d.ni) return;
+  const int2 lr = d.lr[f];
+  double j[25], dm[25], out[25];
+  impl_block(fj, d.ni, f, 1, j);
+  impl_block(dwdu, d.nc, lr.y, 0, dm);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) out[q] = 0.0;
+  impl_add_product(out, j, dm, 1.0);
+  double* up = offd + 25LL * f;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) up[q] = out[q];
+  impl_block(fj, d.ni, f, 0, j);
+  impl_block(dwdu, d.nc, lr.x, 0, dm);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) out[q] = 0.0;
+  impl_add_product(out, j, dm, -1.0);
+  double* lo = offd + 25LL * (d.ni + f);
+#pragma unroll
+  for (int q = 0; q < 25; ++q) lo[q] = out[q];
+}
+
+// Each diagonal block's inverse: Gauss-Jordan on [A | I] with partial pivoting (the first row
+// holding the largest |pivot| below the diagonal, rows swapped whole, the pivot row scaled by the
+// pivot's reciprocal, every other row reduced). A zero or non-finite pivot marks the cell in
+// *singular (the least such cell).
+__global__ __launch_bounds__(kBlock) void k_impl_invert(const double* __restrict__ diag, int n,
+                                                        double* __restrict__ inverse, int* __restrict__ singular) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n) return;
+  double m[5][10];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      m[i][j] = diag[static_cast<long long>(5 * i + j) * n + c];
+      m[i][5 + j] = i == j ? 1.0 : 0.0;
+    }
+  }
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    int p = k;
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      if (fabs(m[i][k]) > fabs(m[p][k])) p = i;
+    }
+    // Registers cannot be indexed by p: swap by selects, row by row.
+#pragma unroll
+    for (int i = k + 1; i < 5; ++i) {
+      if (i == p) {
+#pragma unroll
+        for (int j = 0; j < 10; ++j) {
+          const double t = m[k][j];
+          m[k][j] = m[i][j];
+          m[i][j] = t;
+        }
+      }
+    }
+    if (m[k][k] == 0.0 || !__builtin_isfinite(m[k][k])) ok = false;
+    const double dv = 1.0 / m[k][k];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) m[k][j] *= dv;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      if (i == k) continue;
+      const double f = m[i][k];
+#pragma unroll
+      for (int j = 0; j < 10; ++j) m[i][j] -= f * m[k][j];
+    }
+  }
+  if (!ok) atomicMin(singular, c);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) inverse[static_cast<long long>(5 * i + j) * n + c] = m[i][5 + j];
+  }
+}
+
+// One row's entries: the cell's interior faces in ascending face order, each as (its block's index
+// in offd, the other cell).
+struct ImplRows {
+  const int* off;    // n + 1
+  const int2* ent;   // 2 ni
+};
+
+// BlockMatrix::relax of the cells cells[0 .. count): x_c = D_c^-1 (b_c - sum_f A_cf x_other).
+__global__ __launch_bounds__(kBlock) void k_impl_relax(ImplRows rows, const int* __restrict__ cells, int count, int n,
+                                                       const double* __restrict__ offd,
+                                                       const double* __restrict__ inverse,
+                                                       const double* __restrict__ b, double* __restrict__ x) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int c = cells[t];
+  double r[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) r[i] = b[static_cast<long long>(i) * n + c];
+  for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
+    const int2 e = rows.ent[k];
+    const double* a = offd + 25LL * e.x;
+    double xo[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) r[i] -= a[5 * i + j] * xo[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s += inverse[static_cast<long long>(5 * i + j) * n + c] * r[j];
+    x[static_cast<long long>(i) * n + c] = s;
+  }
+}
+
+// BlockMatrix::multiply: y_c = D_c x_c + sum_f A_cf x_other, the diagonal block first.
+__global__ __launch_bounds__(kBlock) void k_impl_multiply(ImplRows rows, int n, const double* __restrict__ diag,
+                                                          const double* __restrict__ offd,
+                                                          const double* __restrict__ x, double* __restrict__ y) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n) return;
+  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  double xc[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) xc[j] = x[static_cast<long long>(j) * n + c];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) r[i] += diag[static_cast<long long>(5 * i + j) * n + c] * xc[j];
+  }
+  for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
+    const int2 e = rows.ent[k];
+    const double* a = offd + 25LL * e.x;
+    double xo[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) r[i] += a[5 * i + j] * xo[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) y[static_cast<long long>(i) * n + c] = r[i];
+}
+
+// sum_i u_i v_i over m values as exact words (added into `words`, cleared beforehand): each product
+// rounded to FP64, then added exactly. Grid-stride with wave-uniform trip counts.
+__global__ __launch_bounds__(kBlock) void k_impl_dot(const double* __restrict__ u, const double* __restrict__ v,
+                                                     long long m, long long* __restrict__ words) {
+  __shared__ long long acc[kWords];
+  for (int t = threadIdx.x; t < kWords; t += kBlock) acc[t] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock + wave * kWave; base < m; base += stride) {
+    const long long i = base + lane;
+    const bool active = i < m;
+    const double t = active ? u[i] * v[i] : 0.0;
+    exact_add_wave(acc, t, active);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int l = 0; l + 1 < kLimbs; ++l) {
+      const long long carry = acc[l] >> 32;
+      acc[l] -= carry * (1LL << 32);
+      acc[l + 1] += carry;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kWords; t += kBlock) {
+    const long long w = acc[t];
+    if (w != 0) atomicAdd(reinterpret_cast<unsigned long long*>(words + t), static_cast<unsigned long long>(w));
+  }
+}
+
+// The largest |u_i| as the bits of a non-negative double (their integer order is their order);
+// a NaN entry leaves a NaN's bits, above every number's.
+__global__ __launch_bounds__(kBlock) void k_impl_maxabs(const double* __restrict__ u, long long m,
+                                                        unsigned long long* __restrict__ out) {
+  unsigned long long best = 0;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(fabs(u[i])));
+    best = b > best ? b : best;
+  }
+  if (best != 0) atomicMax(out, best);
+}
+
+// Clears n exact-sum word sets and sets *bits (the admissible fraction's minimum) to 1.0's bits,
+// *singular to INT32_MAX and the largest-entry bits to 0.
+__global__ void k_impl_clear(long long* __restrict__ words, int n, unsigned long long* __restrict__ bits,
+                             int* __restrict__ singular) {
+  for (int t = threadIdx.x; t < n * kWords; t += blockDim.x) words[t] = 0;
+  if (threadIdx.x == 0) {
+    bits[0] = static_cast<unsigned long long>(__double_as_longlong(1.0));
+    bits[1] = 0;
+    *singular = INT32_MAX;
+  }
+}
+
+// EulerPhysics::pressure(c, a): the pressure of u + a du in FP64.
+__device__ __forceinline__ double impl_pressure(const double* v0, const double* dv, double a, double gamma) {
+  double v[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) v[k] = v0[k] + a * dv[k];
+  return (gamma - 1.0) * (v[4] - 0.5 * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) / v[0]);
+}
+
+constexpr double kImplMaxDecrease = 0.5;
+constexpr double kImplMinStep = 1e-6;
+
+// Each cell's admissible fraction (its density bound, then halved until its pressure bound holds),
+// and their least as the bits of a non-negative double (atomicMin of the bits).
+__global__ __launch_bounds__(kBlock) void k_impl_admissible(const double* __restrict__ u,
+                                                            const double* __restrict__ du, int n, double gamma,
+                                                            unsigned long long* __restrict__ bits) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long key = ~0ULL;
+  if (c < n) {
+    double v0[5], dv[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      v0[k] = u[static_cast<long long>(k) * n + c];
+      dv[k] = du[static_cast<long long>(k) * n + c];
+    }
+    double a = 1.0;
+    if (dv[0] < 0.0) {
+      const double b = kImplMaxDecrease * v0[0] / -dv[0];
+      a = b < a ? b : a;  // std::min(a, b): b when b < a
+    }
+    const double p = impl_pressure(v0, dv, 0.0, gamma);
+    while (a >= kImplMinStep && !(impl_pressure(v0, dv, a, gamma) >= (1.0 - kImplMaxDecrease) * p)) a *= 0.5;
+    key = static_cast<unsigned long long>(__double_as_longlong(a));
+  }
+  key = wave_min(key);
+  if ((threadIdx.x % kWave) == 0 && key != ~0ULL) atomicMin(bits, key);
+}
+
+// ---- the GCR's vector operations (solvers/src/gcr.cpp), m values each
+
+// y = -x
+__global__ __launch_bounds__(kBlock) void k_impl_negate(const double* __restrict__ x, double* __restrict__ y,
+                                                        long long m) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < m) y[i] = -x[i];
+}
+
+// x = 0
+__global__ __launch_bounds__(kBlock) void k_impl_zero(double* __restrict__ x, long long m) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < m) x[i] = 0.0;
+}
+
+// r /= s (mode 0) or r *= s (mode 1)
+__global__ __launch_bounds__(kBlock) void k_impl_scale(double* __restrict__ r, long long m, double s, int mode) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < m) r[i] = mode == 0 ? r[i] / s : r[i] * s;
+}
+
+// z -= beta zj and w -= beta wj
+__global__ __launch_bounds__(kBlock) void k_impl_orthogonalize(double* __restrict__ z, double* __restrict__ w,
+                                                               const double* __restrict__ zj,
+                                                               const double* __restrict__ wj, long long m,
+                                                               double beta) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= m) return;
+  z[i] -= beta * zj[i];
+  w[i] -= beta * wj[i];
+}
+
+// x += sa z (sa = scale * alpha) and r -= alpha w
+__global__ __launch_bounds__(kBlock) void k_impl_advance(double* __restrict__ x, double* __restrict__ r,
+                                                         const double* __restrict__ z, const double* __restrict__ w,
+                                                         long long m, double sa, double alpha) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= m) return;
+  x[i] += sa * z[i];
+  r[i] -= alpha * w[i];
+}
+
+// u += alpha du
+__global__ __launch_bounds__(kBlock) void k_impl_update(double* __restrict__ u, const double* __restrict__ du,
+                                                        long long m, double alpha) {
+  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < m) u[i] += alpha * du[i];
+}
+
+}  // namespace mxe

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_dwdu(Dev<C> d, const double* __restr
   }
 }
 
-// The face Jacobians of the residual's face and boundary loops on planar meshes
+// The face Jacobians of the residual's face and boundary loops
 // (kernels/euler.py:854-860, 1051-1057), area-weighted as the fluxes: interior face f (owner to
 // neighbour) dF/dW_L at (5 i + j) n_interior + f and dF/dW_R at (25 + 5 i + j) n_interior + f;
 // boundary face b dF/dW_L at (5 i + j) n_boundary + b, W_L the cells' primitives: with the
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_dwdu(Dev<C> d, const double* __restr
 // in the reference's Jacobian loops (the bodies with warped-face terms, roe_boundary and its kin,
 // kernels/euler.py:868-940), slip and wall faces take the pressure flux alone, and the other
 // boundary kinds the flux against boundary_value of the reconstructed state, differentiated
-// through it.
+// through it. On warped meshes k_face_jacobian_warp then adds the face body's warp term's part.
 template <class C, int FLUX, bool SECOND>
 __global__ __launch_bounds__(kBlock) void k_face_jacobian(Dev<C> d, double* __restrict__ fj, double* __restrict__ bj) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
@@ -214,6 +214,48 @@ __global__ __launch_bounds__(kBlock) void k_face_jacobian(Dev<C> d, double* __re
       for (int i = 0; i < 5; ++i) {
 #pragma unroll
         for (int j = 0; j < 5; ++j) bj[static_cast<long long>(5 * i + j) * nb + b] = static_cast<double>(area * fx.f[i].d[j]);
+      }
+    }
+  }
+}
+
+// The warped-face term's part of the interior faces' blocks (second order, meshes with a face not
+// proven planar), added after k_face_jacobian in the compute type: the stored area dF/dW (an FP64
+// widening of a compute-type value, so its conversion back is exact) plus the warp term's
+// derivative through its state (W_L + W_R) / 2, its directions
+// dW_j = (phi_L grad W_L + phi_R grad W_R) M[:, j] / 2 held (kernels/euler.py:704-705 within
+// 858-860). A kernel of its own keeps k_face_jacobian's registers.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_face_jacobian_warp(Dev<C> d, double* __restrict__ fj) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= d.ni) return;
+  using D = Dual<C, 5>;
+  C g[8], rr[4];
+  load_n<C, 8>(d.fgeo + 8LL * f, g);
+  load_n<C, 4>(d.frr + 4LL * f, rr);
+  const int2 lr = d.lr[f];
+  const Params<C>& pa = d.ptab[cell_case(d, lr.x)];
+  C gl[20], gr[20], m9[9];
+  load_rec(d.rec, lr.x, gl);
+  load_rec(d.rec, lr.y, gr);
+  const W5<C> fl = reconstruct(load_w(d.prim, lr.x), gl, gl + 15, V3<C>{g[4], g[5], g[6]}, pa.pref);
+  const W5<C> fr = reconstruct(load_w(d.prim, lr.y), gr, gr + 15, V3<C>{rr[0], rr[1], rr[2]}, pa.pref);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m9[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+  W5<C> wd[3];
+  warp_directions(gl, gl + 15, gr, gr + 15, m9, wd);
+  for (int side = 0; side < 2; ++side) {
+    const W5<D> dl = lift<C, 5>(fl, side == 0), dr = lift<C, 5>(fr, side == 1);
+    const D half(0.5);
+    const W5<D> w{half * (dl.r + dr.r), half * (dl.u + dr.u), half * (dl.v + dr.v), half * (dl.w + dr.w),
+                  half * (dl.p + dr.p)};
+    const F5<D> wx = warp_of_directions(w, wd, D(pa.g), D(pa.pref));
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        double& out = fj[static_cast<long long>(25 * side + 5 * i + j) * d.ni + f];
+        out = static_cast<double>(static_cast<C>(out) + wx.f[i].d[j]);
       }
     }
   }

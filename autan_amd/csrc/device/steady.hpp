@@ -233,8 +233,16 @@ class SteadyCase {
   // invalid the step's invalid cell (-1: none). history (5 per iteration) and provisional may be
   // null. Returns done().
   bool record(const double* r, const double* reported, int32_t invalid, double* history, int32_t* provisional) {
+    physics(r, reported);
+    return runtime(r, reported, invalid, history, provisional);
+  }
+
+  // The physics' side of one iteration (EulerPhysics::relative_drop, then switch_order): the largest
+  // relative residual over the equations of the order being solved, and whether the first-order
+  // start ends (switched()). r: the step's norms; reported: what the history records (null: r).
+  double physics(const double* r, const double* reported) {
     switched_ = false;
-    const bool prov = !second_;
+    prov_ = !second_;
     const double* rep = reported != nullptr ? reported : r;
     // EulerPhysics::relative_drop on the residuals the runtime receives, then switch_order.
     phys_.update(rep);
@@ -257,7 +265,13 @@ class SteadyCase {
         out_.first_order_end = start_.end();
       }
     }
-    // The runtime's loop: divergence, history, convergence.
+    return drop;
+  }
+
+  // The runtime's side of the same iteration: divergence, history, convergence. Returns done().
+  bool runtime(const double* r, const double* reported, int32_t invalid, double* history, int32_t* provisional) {
+    const bool prov = prov_;
+    const double* rep = reported != nullptr ? reported : r;
     bool finite = invalid < 0;
     for (int e = 0; e < kEq; ++e) finite = finite && std::isfinite(rep[e]);
     std::copy(rep, rep + kEq, out_.residuals);
@@ -311,7 +325,7 @@ class SteadyCase {
 
   mxsteady_options o_;
   mxsteady_result out_{};
-  bool second_ = false, switched_ = false, done_ = false, has_floor_ = false, have_first_ = false;
+  bool second_ = false, switched_ = false, done_ = false, has_floor_ = false, have_first_ = false, prov_ = false;
   std::array<double, kEq> floors_{}, first_{};
   double drop_to_ = 0.0;
   Exemption phys_, runtime_;
@@ -351,6 +365,111 @@ inline void run(const mxsteady_ops& ops, const mxsteady_options& o, mxsteady_res
     int32_t invalid = -1;
     call(ops.step(ops.ctx, o.cfl, r, &invalid), "step");
     sc.record(r, prov ? reported : nullptr, invalid, history, provisional);
+    if (sc.switched()) call(ops.set_second_order(ops.ctx, 1), "set_second_order");
+  }
+  out = sc.result();
+}
+
+// ---- the steady implicit run (EulerPhysics::steady_step with its switched evolution relaxation,
+// physics/src/euler_physics.cpp:59-86, 1056-1250), over callbacks.
+
+constexpr double kCflStart = 1000.0;
+constexpr double kCflMin = 1.0;
+constexpr double kCflMax = 1e6;
+constexpr double kCflRise = 1.1;
+
+// The CFL number of each step: CFL_0 over the largest relative residual, times a penalty halved
+// after a step that was shortened or rejected or whose relative residual rose by more than
+// kCflRise, recovering by half otherwise; after a shortened or rejected step at most half the CFL
+// number used.
+class CflControl {
+ public:
+  double next(double drop) {
+    const bool rose = previous_ > 0.0 && drop > kCflRise * previous_;
+    penalty_ = (shortened_ || rose) ? 0.5 * penalty_ : std::min(1.0, 1.5 * penalty_);
+    previous_ = drop;
+    if (drop > 0.0) cfl_ = std::clamp(std::min(penalty_ * kCflStart / drop, cap_), kCflMin, kCflMax);
+    cap_ = kCflMax;
+    return cfl_;
+  }
+  // After the solve: report = {cycles, residual drop, alpha, failed, singular}. Returns whether the
+  // run has diverged (no usable step at the smallest CFL number).
+  bool after(const double* report, double drop) {
+    if (report[3] != 0.0) {
+      if (drop > 0.0) penalty_ = std::min(penalty_, cfl_ * drop / kCflStart);
+      cap_ = 0.5 * cfl_;
+      shortened_ = true;
+      return cfl_ <= kCflMin;
+    }
+    const double alpha = report[2];
+    shortened_ = alpha < 1.0;
+    if (shortened_) {
+      if (drop > 0.0) penalty_ = std::min(penalty_, cfl_ * drop / kCflStart);
+      cap_ = 0.5 * cfl_;
+    }
+    return alpha < 1e-6 && cfl_ <= kCflMin;
+  }
+  double cfl() const { return cfl_; }
+
+ private:
+  double cfl_ = kCflStart, penalty_ = 1.0, previous_ = 0.0, cap_ = kCflMax;
+  bool shortened_ = false;
+};
+
+// One steady implicit run: per iteration (the second-order residual for the history during the
+// first-order start), the residual and Jacobian at the order being solved, the physics' decisions
+// (relative drop, end of the first-order start), the CFL number, the solve and update, then the
+// runtime's (history, divergence, convergence). stats (4 per iteration: CFL, linear cycles, linear
+// residual drop, alpha) may be null.
+inline void run_implicit(const mximplicit_ops& ops, const mxsteady_options& o, mxsteady_result& out, double* history,
+                         int32_t* provisional, double* stats) {
+  if (!ops.evaluate || !ops.set_second_order || !ops.jacobian || !ops.solve) {
+    throw std::invalid_argument("implicit run: missing callback");
+  }
+  SteadyCase sc(o);
+  CflControl ctl;
+  call(ops.set_second_order(ops.ctx, sc.second() ? 1 : 0), "set_second_order");
+  {
+    double reported[kEq];
+    int32_t inv = -1;
+    call(ops.evaluate(ops.ctx, reported, &inv), "evaluate");
+    if (inv >= 0) {
+      sc.diverged_before(inv, reported);
+      out = sc.result();
+      return;
+    }
+  }
+  while (!sc.done()) {
+    double reported[kEq];
+    const bool prov = !sc.second();
+    if (prov) {
+      int32_t ignored = -1;
+      call(ops.set_second_order(ops.ctx, 1), "set_second_order");
+      call(ops.evaluate(ops.ctx, reported, &ignored), "evaluate");
+      call(ops.set_second_order(ops.ctx, 0), "set_second_order");
+    }
+    double r[kEq];
+    int32_t invalid = -1;
+    call(ops.jacobian(ops.ctx, r, &invalid), "jacobian");
+    const double drop = sc.physics(r, prov ? reported : nullptr);
+    const double cfl = ctl.next(drop);
+    double report[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (invalid < 0) {
+      call(ops.solve(ops.ctx, cfl, report), "solve");
+      if (ctl.after(report, drop)) {
+        int32_t worst = 0;
+        if (ops.worst_cell != nullptr) call(ops.worst_cell(ops.ctx, &worst), "worst_cell");
+        invalid = worst;
+      }
+    }
+    if (stats != nullptr) {
+      const int64_t it = sc.result().iterations;
+      stats[4 * it] = cfl;
+      stats[4 * it + 1] = report[0];
+      stats[4 * it + 2] = report[1];
+      stats[4 * it + 3] = report[2];
+    }
+    sc.runtime(r, prov ? reported : nullptr, invalid, history, provisional);
     if (sc.switched()) call(ops.set_second_order(ops.ctx, 1), "set_second_order");
   }
   out = sc.result();

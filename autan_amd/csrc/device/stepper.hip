@@ -22,6 +22,7 @@
 
 #include "../../../include/mxeuler.h"
 #include "exact.hpp"
+#include "implicit.hpp"
 #include "jacobian.hpp"
 #include "kernels.hpp"
 #include "steady.hpp"
@@ -143,6 +144,10 @@ class StepperBase {
   virtual void arrays(double* prim, double* grad, double* phi, double* prim_b) = 0;
   virtual void face_force(const int* faces, long long n, double* force) = 0;
   virtual void face_force_noalloc(const int* faces, long long n, double* force) = 0;
+  virtual void implicit_evaluate(double* norms, int* invalid) = 0;
+  virtual void implicit_solve(double cfl, double* report) = 0;
+  virtual void implicit_colours(int* colours) = 0;
+  virtual int implicit_worst_cell() = 0;
   virtual void conservation(double* out) = 0;
   virtual void tune(const std::string& key, int value) = 0;
   virtual double area_scale() const = 0;
@@ -272,7 +277,6 @@ class Stepper final : public StepperBase {
   // primitives, gradients and limiter the cells' dW/dq and the faces' dF/dW (jacobian.hpp).
   void residual_and_jacobian(const double* u, double* res, double* dwdu, double* face_jac,
                              double* bnd_jac) override {
-    if (!planar_) throw std::invalid_argument("residual_and_jacobian: planar meshes only (warp terms are not differentiated)");
     if (ncase() != 1) throw std::invalid_argument("residual_and_jacobian: one case");
     Buffer<double> dj(25 * std::size_t(nc_)), fj(50 * std::size_t(std::max(ni_, 1))), bj(25 * std::size_t(std::max(nb_, 1)));
     residual(u, res);
@@ -297,10 +301,286 @@ class Stepper final : public StepperBase {
     check(hipStreamSynchronize(stream_), "residual_and_jacobian");
   }
 
+  // ---- the steady implicit step (EulerPhysics::steady_step with linear_solver = "gauss-seidel",
+  // physics/src/euler_physics.cpp:1056-1250), on the stepper's state u_; implicit.hpp has the kernels.
+
+  // R(u_) at the order set, its norms and validity (as evaluate), the spectral radius, and the
+  // approximate Jacobian's parts, all kept on the device for implicit_solve.
+  void implicit_evaluate(double* norms, int* invalid) override {
+    if (ncase_ != 1) throw std::invalid_argument("implicit step: one case");
+    implicit_setup();
+    evaluate(norms, invalid);
+    const Dev<C> d = dev();
+    launch("dwdu", [&] { k_dwdu<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, u_.get(), jd_.get()); });
+    launch("face_jacobian", [&] {
+      switch (sc_.flux) {
+        case MXEULER_ROE:
+          launch_face_jacobian<0>(d, jf_.get(), jb_.get());
+          break;
+        case MXEULER_HLLC:
+          launch_face_jacobian<1>(d, jf_.get(), jb_.get());
+          break;
+        default:
+          launch_face_jacobian<2>(d, jf_.get(), jb_.get());
+          break;
+      }
+    });
+    check(hipStreamSynchronize(stream_), "implicit_evaluate");
+    have_jacobian_ = true;
+  }
+
+  // Assembles lambda / cfl I + dR/dU from the last implicit_evaluate, solves it for du by GCR (at
+  // most kImplDirections directions, to kImplTolerance) preconditioned by kImplSweeps symmetric
+  // multicolour block Gauss-Seidel sweeps from zero, and, unless the solve failed, adds the
+  // admissible fraction of du to u_. report: cycles, residual drop, alpha (0: no update), failed,
+  // singular.
+  void implicit_solve(double cfl, double* report) override {
+    if (!have_jacobian_) throw std::invalid_argument("implicit_solve: no implicit_evaluate of this state");
+    if (!(cfl > 0.0)) throw std::invalid_argument("implicit_solve: cfl must be > 0");
+    have_jacobian_ = false;
+    for (int q = 0; q < 5; ++q) report[q] = 0.0;
+    const Dev<C> d = dev();
+    const long long m = 5LL * nc_;
+    const int gm = static_cast<int>((m + kBlock - 1) / kBlock);
+    dot_next_ = 0;
+    launch("impl_clear", [&] { k_impl_clear<<<1, 1024, 0, stream_>>>(words_.get(), kImplDots, bits_.get(), flag_.get()); });
+    launch("impl_diag", [&] {
+      k_impl_diag<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl, diag_.get());
+    });
+    if (ni_ > 0) launch("impl_off", [&] { k_impl_off<C><<<blocks(ni_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), offd_.get()); });
+    launch("impl_invert", [&] { k_impl_invert<<<blocks(nc_), kBlock, 0, stream_>>>(diag_.get(), nc_, inv_.get(), flag_.get()); });
+    launch("impl_zero", [&] { k_impl_zero<<<gm, kBlock, 0, stream_>>>(du_.get(), m); });
+    launch("impl_negate", [&] { k_impl_negate<<<gm, kBlock, 0, stream_>>>(res_.get(), kr_.get(), m); });
+    int singular = INT32_MAX;
+    check(hipMemcpyAsync(&singular, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "implicit_solve");
+    if (singular != INT32_MAX) {
+      // A singular diagonal block: the solve failed (EulerPhysics::steady_step's domain_error).
+      report[1] = std::numeric_limits<double>::quiet_NaN();
+      report[3] = 1.0;
+      report[4] = 1.0;
+      return;
+    }
+    int cycles = 0;
+    const double drop = gcr(m, gm, cycles);
+    report[0] = cycles;
+    report[1] = drop;
+    if (!(drop <= kImplFailure)) {
+      report[3] = 1.0;
+      return;
+    }
+    launch("impl_admissible", [&] {
+      k_impl_admissible<<<blocks(nc_), kBlock, 0, stream_>>>(u_.get(), du_.get(), nc_, sc_.gamma, bits_.get());
+    });
+    unsigned long long abits = 0;
+    check(hipMemcpyAsync(&abits, bits_.get(), sizeof(abits), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "implicit_solve");
+    double alpha;
+    std::memcpy(&alpha, &abits, sizeof alpha);
+    report[2] = alpha;
+    if (alpha < kImplMinStep) return;
+    launch("impl_update", [&] { k_impl_update<<<gm, kBlock, 0, stream_>>>(u_.get(), du_.get(), m, alpha); });
+    prim_fresh_ = false;
+    check(hipStreamSynchronize(stream_), "implicit_solve");
+  }
+
+  // EulerPhysics::worst_cell: the cell whose density or pressure the full step u + du lowers the
+  // most, relatively (ties: the least index), from the last solve's du.
+  int implicit_worst_cell() override {
+    if (!implicit_ready_) throw std::invalid_argument("implicit_worst_cell: no implicit solve");
+    const std::size_t n = static_cast<std::size_t>(nc_);
+    std::vector<double> u(5 * n), du(5 * n);
+    u_.download(u.data(), u.size(), stream_);
+    du_.download(du.data(), du.size(), stream_);
+    check(hipStreamSynchronize(stream_), "implicit_worst_cell");
+    auto pressure = [&](std::size_t c, double a) {
+      double v[5];
+      for (std::size_t k = 0; k < 5; ++k) v[k] = u[k * n + c] + a * du[k * n + c];
+      return (sc_.gamma - 1.0) * (v[4] - 0.5 * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) / v[0]);
+    };
+    std::size_t worst = 0;
+    double lowest = std::numeric_limits<double>::infinity();
+    for (std::size_t c = 0; c < n; ++c) {
+      const double r = std::min((u[c] + du[c]) / u[c], pressure(c, 1.0) / pressure(c, 0.0));
+      if (r < lowest) {
+        lowest = r;
+        worst = c;
+      }
+    }
+    return static_cast<int>(worst);
+  }
+
+  // The colour of every cell in the implicit step's sweeps.
+  void implicit_colours(int* colours) override {
+    implicit_setup();
+    std::copy(colour_host_.begin(), colour_host_.end(), colours);
+  }
+
+ private:
+  static constexpr int kImplSweeps = 4;
+  static constexpr int kImplDirections = 10;
+  static constexpr double kImplTolerance = 0.1;
+  static constexpr double kImplFailure = 0.9;
+  // The exact inner products one solve can take: the first, then per direction k its k
+  // orthogonalisation coefficients, its norm, its step and the new residual's norm.
+  static constexpr int kImplDots = 1 + kImplDirections * (kImplDirections - 1) / 2 + 3 * kImplDirections;
+
+  // Buffers, the rows (each cell's interior faces in ascending face order: BlockMatrix's lists) and
+  // the greedy colouring in cell order, made on the first implicit call.
+  void implicit_setup() {
+    if (implicit_ready_) return;
+    const std::size_t n = static_cast<std::size_t>(nc_), ni = static_cast<std::size_t>(ni_);
+    std::vector<int> off(n + 1, 0);
+    for (std::size_t f = 0; f < ni; ++f) {
+      ++off[static_cast<std::size_t>(owner_host_[f]) + 1];
+      ++off[static_cast<std::size_t>(neighbour_host_[f]) + 1];
+    }
+    for (std::size_t c = 0; c < n; ++c) off[c + 1] += off[c];
+    std::vector<int2> ent(std::max<std::size_t>(2 * ni, 1));
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (std::size_t f = 0; f < ni; ++f) {
+      const int o = owner_host_[f], nb = neighbour_host_[f];
+      // Row owner: the upper block (index f), other cell the neighbour; row neighbour: the lower
+      // block (index ni + f), other cell the owner.
+      ent[static_cast<std::size_t>(fill[static_cast<std::size_t>(o)]++)] = make_int2(static_cast<int>(f), nb);
+      ent[static_cast<std::size_t>(fill[static_cast<std::size_t>(nb)]++)] = make_int2(static_cast<int>(ni + f), o);
+    }
+    colour_host_.assign(n, -1);
+    int colours = 0;
+    std::vector<char> seen;
+    for (std::size_t c = 0; c < n; ++c) {
+      seen.assign(static_cast<std::size_t>(colours) + 1, 0);
+      for (int k = off[c]; k < off[c + 1]; ++k) {
+        const int other = ent[static_cast<std::size_t>(k)].y;
+        if (colour_host_[static_cast<std::size_t>(other)] >= 0) seen[static_cast<std::size_t>(colour_host_[static_cast<std::size_t>(other)])] = 1;
+      }
+      int k = 0;
+      while (seen[static_cast<std::size_t>(k)] != 0) ++k;
+      colour_host_[c] = k;
+      colours = std::max(colours, k + 1);
+    }
+    std::vector<int> cells(std::max<std::size_t>(n, 1));
+    colour_off_.assign(static_cast<std::size_t>(colours) + 1, 0);
+    for (std::size_t c = 0; c < n; ++c) ++colour_off_[static_cast<std::size_t>(colour_host_[c]) + 1];
+    for (int k = 0; k < colours; ++k) colour_off_[static_cast<std::size_t>(k) + 1] += colour_off_[static_cast<std::size_t>(k)];
+    std::vector<int> at(colour_off_.begin(), colour_off_.end() - 1);
+    for (std::size_t c = 0; c < n; ++c) cells[static_cast<std::size_t>(at[static_cast<std::size_t>(colour_host_[c])]++)] = static_cast<int>(c);
+    roff_.upload(off, stream_);
+    rent_.upload(ent, stream_);
+    ccells_.upload(cells, stream_);
+    const std::size_t m = 5 * n;
+    jd_ = Buffer<double>(25 * n);
+    jf_ = Buffer<double>(50 * std::max<std::size_t>(ni, 1));
+    jb_ = Buffer<double>(25 * std::max<std::size_t>(static_cast<std::size_t>(nb_), 1));
+    diag_ = Buffer<double>(25 * n);
+    inv_ = Buffer<double>(25 * n);
+    offd_ = Buffer<double>(50 * std::max<std::size_t>(ni, 1));
+    kr_ = Buffer<double>(m);
+    du_ = Buffer<double>(m);
+    zs_ = Buffer<double>(m * kImplDirections);
+    ws_ = Buffer<double>(m * kImplDirections);
+    words_ = Buffer<long long>(std::size_t(kImplDots) * kWords);
+    bits_ = Buffer<unsigned long long>(2);
+    flag_ = Buffer<int>(1);
+    check(hipStreamSynchronize(stream_), "implicit_setup");
+    implicit_ready_ = true;
+  }
+
+  // sum u v, exact, rounded once (a fresh word set per call of one solve).
+  double exact_dot(const double* u, const double* v, long long m) {
+    if (dot_next_ >= kImplDots) throw std::logic_error("implicit_solve: more inner products than planned");
+    long long* w = words_.get() + std::size_t(dot_next_++) * kWords;
+    const int g = std::max(1, std::min(static_cast<int>((m + kBlock - 1) / kBlock), 1024));
+    launch("impl_dot", [&] { k_impl_dot<<<g, kBlock, 0, stream_>>>(u, v, m, w); });
+    long long host[kWords];
+    check(hipMemcpyAsync(host, w, sizeof host, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "implicit_solve");
+    return exact_value(host);
+  }
+
+  // kImplSweeps symmetric sweeps on A z = b from z = 0: the colours in increasing, then decreasing
+  // order (BlockMatrix::gauss_seidel).
+  void gauss_seidel(const double* b, double* z, long long m, int gm) {
+    launch("impl_zero", [&] { k_impl_zero<<<gm, kBlock, 0, stream_>>>(z, m); });
+    const ImplRows rows{roff_.get(), rent_.get()};
+    const int colours = static_cast<int>(colour_off_.size()) - 1;
+    auto relax = [&](int k) {
+      const int first = colour_off_[static_cast<std::size_t>(k)];
+      const int count = colour_off_[static_cast<std::size_t>(k) + 1] - first;
+      if (count == 0) return;
+      launch("impl_relax", [&] {
+        k_impl_relax<<<(count + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(rows, ccells_.get() + first, count, nc_,
+                                                                             offd_.get(), inv_.get(), b, z);
+      });
+    };
+    for (int s = 0; s < kImplSweeps; ++s) {
+      for (int k = 0; k < colours; ++k) relax(k);
+      for (int k = colours - 1; k >= 0; --k) relax(k);
+    }
+  }
+
+  // Gcr::solve (solvers/src/gcr.cpp) on kr_ = -R into du_ = 0: returns the residual drop.
+  double gcr(long long m, int gm, int& cycles) {
+    unsigned long long mbits = 0;
+    launch("impl_maxabs", [&] {
+      k_impl_maxabs<<<std::max(1, std::min(gm, 1024)), kBlock, 0, stream_>>>(kr_.get(), m, bits_.get() + 1);
+    });
+    check(hipMemcpyAsync(&mbits, bits_.get() + 1, sizeof mbits, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream_), "implicit_solve");
+    double scale;
+    std::memcpy(&scale, &mbits, sizeof scale);
+    if (std::isnan(scale)) scale = std::numeric_limits<double>::infinity();
+    if (!std::isfinite(scale)) return std::numeric_limits<double>::quiet_NaN();
+    if (scale == 0.0) return 0.0;
+    launch("impl_scale", [&] { k_impl_scale<<<gm, kBlock, 0, stream_>>>(kr_.get(), m, scale, 0); });
+    const double r0 = std::sqrt(exact_dot(kr_.get(), kr_.get(), m));
+    double rr = r0 * r0;
+    double ww_of[kImplDirections];
+    for (int k = 0; k < kImplDirections; ++k) {
+      double* z = zs_.get() + std::size_t(k) * std::size_t(m);
+      double* w = ws_.get() + std::size_t(k) * std::size_t(m);
+      gauss_seidel(kr_.get(), z, m, gm);
+      const ImplRows rows{roff_.get(), rent_.get()};
+      launch("impl_multiply", [&] {
+        k_impl_multiply<<<blocks(nc_), kBlock, 0, stream_>>>(rows, nc_, diag_.get(), offd_.get(), z, w);
+      });
+      for (int j = 0; j < k; ++j) {
+        const double* zj = zs_.get() + std::size_t(j) * std::size_t(m);
+        const double* wj = ws_.get() + std::size_t(j) * std::size_t(m);
+        const double beta = exact_dot(w, wj, m) / ww_of[j];
+        launch("impl_orthogonalize", [&] { k_impl_orthogonalize<<<gm, kBlock, 0, stream_>>>(z, w, zj, wj, m, beta); });
+      }
+      const double ww = exact_dot(w, w, m);
+      ww_of[k] = ww;
+      cycles = k + 1;
+      if (!(ww > 0.0)) {
+        if (!std::isfinite(ww)) rr = ww;
+        break;
+      }
+      const double alpha = exact_dot(kr_.get(), w, m) / ww;
+      const double sa = scale * alpha;
+      launch("impl_advance", [&] { k_impl_advance<<<gm, kBlock, 0, stream_>>>(du_.get(), kr_.get(), z, w, m, sa, alpha); });
+      rr = exact_dot(kr_.get(), kr_.get(), m);
+      if (std::sqrt(rr) <= kImplTolerance * r0) break;
+    }
+    return std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
+  }
+
+  bool implicit_ready_ = false, have_jacobian_ = false;
+  int dot_next_ = 0;
+  std::vector<int> colour_host_, colour_off_;
+  Buffer<int> roff_, ccells_, flag_;
+  Buffer<int2> rent_;
+  Buffer<double> jd_, jf_, jb_, diag_, inv_, offd_, kr_, du_, zs_, ws_;
+  Buffer<long long> words_;
+  Buffer<unsigned long long> bits_;
+
+ public:
   template <int FLUX>
   void launch_face_jacobian(const Dev<C>& d, double* fj, double* bj) {
     if (second_) {
       k_face_jacobian<C, FLUX, true><<<blocks(nf_), kBlock, 0, stream_>>>(d, fj, bj);
+      if (!planar_ && ni_ > 0 && d.moment != nullptr) k_face_jacobian_warp<C><<<blocks(ni_), kBlock, 0, stream_>>>(d, fj);
     } else {
       k_face_jacobian<C, FLUX, false><<<blocks(nf_), kBlock, 0, stream_>>>(d, fj, bj);
     }
@@ -977,6 +1257,7 @@ class Stepper final : public StepperBase {
     // Kept for rebuilding the face blocks at another size (the face_block tuning key).
     cell_faces_host_.assign(t.cell_faces, t.cell_faces + cfo.back());
     owner_host_.assign(t.owner, t.owner + nf_);
+    neighbour_host_.assign(t.neighbour, t.neighbour + ni_);
     build_slots(cell_faces_host_.data(), owner_host_.data(), cfo);
     cfo_host_ = std::move(cfo);
     check(hipStreamSynchronize(stream_), "topology");
@@ -1681,7 +1962,7 @@ class Stepper final : public StepperBase {
   Buffer<Params<C>> ptab_;
   Buffer<int> bcase_, fcase_, fcbeg_, cflags_;
   std::vector<int> fcbeg_host_;
-  std::vector<int> cfo_host_, cell_faces_host_, owner_host_;
+  std::vector<int> cfo_host_, cell_faces_host_, owner_host_, neighbour_host_;
   // Faces per face block: four rounds of the workgroup in FP32 (a full 256 cells of hexahedra per
   // block), two in FP64 (whose fluxes take twice the LDS); the face_block tuning key changes it.
   int block_faces_ = std::is_same_v<C, float> ? 2 * kMaxSlots : kMaxSlots;
@@ -1744,6 +2025,17 @@ int fail(const std::exception& e, char* err, size_t err_len) {
     err[err_len - 1] = '\0';
   }
   return dynamic_cast<const std::invalid_argument*>(&e) != nullptr ? 1 : 2;
+}
+
+// A callback's status for the steady drivers: 0, or 1 when the call threw.
+template <class F>
+int device_call(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception&) {
+    return 1;
+  }
 }
 
 template <class F>
@@ -2145,6 +2437,67 @@ int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady
     if (s->impl->ncase() != 1) throw std::invalid_argument("a batch runs with mxeuler_batch_steady");
     const mxsteady_ops ops{s, steady_step, steady_evaluate, steady_order};
     mxe::steady::run(ops, *options, *result, history, provisional);
+  });
+}
+
+int mxeuler_implicit_evaluate(mxeuler_stepper* s, double norms[5], int32_t* invalid_cell, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !norms || !invalid_cell) throw std::invalid_argument("null argument");
+    int inv = -1;
+    s->impl->implicit_evaluate(norms, &inv);
+    *invalid_cell = inv;
+  });
+}
+
+int mxeuler_implicit_solve(mxeuler_stepper* s, double cfl, double report[5], char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !report) throw std::invalid_argument("null argument");
+    s->impl->implicit_solve(cfl, report);
+  });
+}
+
+int mxeuler_implicit_colours(mxeuler_stepper* s, int32_t* colours, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !colours) throw std::invalid_argument("null argument");
+    s->impl->implicit_colours(colours);
+  });
+}
+
+int mximplicit_run(const mximplicit_ops* ops, const mxsteady_options* options, mxsteady_result* result,
+                   double* history, int32_t* provisional, double* stats, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!ops || !options || !result) throw std::invalid_argument("null argument");
+    mxe::steady::run_implicit(*ops, *options, *result, history, provisional, stats);
+  });
+}
+
+namespace {
+
+int implicit_jacobian(void* ctx, double residuals[5], int32_t* invalid) {
+  return device_call([&] {
+    int inv = -1;
+    static_cast<mxeuler_stepper*>(ctx)->impl->implicit_evaluate(residuals, &inv);
+    *invalid = inv;
+  });
+}
+
+int implicit_solve(void* ctx, double cfl, double report[5]) {
+  return device_call([&] { static_cast<mxeuler_stepper*>(ctx)->impl->implicit_solve(cfl, report); });
+}
+
+int implicit_worst(void* ctx, int32_t* cell) {
+  return device_call([&] { *cell = static_cast<mxeuler_stepper*>(ctx)->impl->implicit_worst_cell(); });
+}
+
+}  // namespace
+
+int mxeuler_implicit_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result,
+                            double* history, int32_t* provisional, double* stats, char* err, size_t err_len) {
+  return guarded(err, err_len, [&] {
+    if (!s || !options || !result) throw std::invalid_argument("null argument");
+    if (s->impl->ncase() != 1) throw std::invalid_argument("the implicit path runs one case per stepper");
+    const mximplicit_ops ops{s, steady_evaluate, steady_order, implicit_jacobian, implicit_solve, implicit_worst};
+    mxe::steady::run_implicit(ops, *options, *result, history, provisional, stats);
   });
 }
 

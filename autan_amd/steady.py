@@ -1,7 +1,9 @@
 """Steady runs: the first-order start, local time steps and the convergence rule of the reference's
 runtime (physics/src/euler_physics.cpp:1495-1571; runtime/src/run.cpp:2262-2389), run by the native
 library (mxsteady_run) over the device stepper or over any object with step / evaluate /
-set_second_order."""
+set_second_order; and the steady implicit run (EulerPhysics::steady_step, pseudo-transient
+continuation with switched evolution relaxation, GCR and block Gauss-Seidel; mximplicit_run) over
+the device stepper or over any object with evaluate / set_second_order / jacobian / solve."""
 
 from __future__ import annotations
 
@@ -25,6 +27,15 @@ _ORDER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int32)
 
 class _Ops(ctypes.Structure):
     _fields_ = [("ctx", ctypes.c_void_p), ("step", _STEP), ("evaluate", _EVAL), ("set_second_order", _ORDER)]
+
+
+_SOLVE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_double))
+_CELL = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32))
+
+
+class _ImplicitOps(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("evaluate", _EVAL), ("set_second_order", _ORDER), ("jacobian", _EVAL),
+                ("solve", _SOLVE), ("worst_cell", _CELL)]
 
 
 class _Options(ctypes.Structure):
@@ -176,3 +187,108 @@ def solve_with(target, max_iterations: int = 10000, residual_orders: float = 8.0
     if rc != 0:
         raise RuntimeError(err.value.decode(errors="replace"))
     return _finish(res, hist, prov)
+
+
+@dataclass
+class ImplicitResult(SteadyResult):
+    cfl: np.ndarray = None  # per iteration: the CFL number
+    cycles: np.ndarray = None  # the GCR directions taken
+    linear_drop: np.ndarray = None  # the solve's residual drop
+    alpha: np.ndarray = None  # the update fraction (0: rejected)
+
+
+def _implicit_finish(res, hist, prov, stats) -> ImplicitResult:
+    base = _finish(res, hist, prov)
+    n = base.iterations
+    st = stats.reshape(-1, 4)[:n]
+    return ImplicitResult(**base.__dict__, cfl=st[:, 0].copy(), cycles=st[:, 1].astype(int), linear_drop=st[:, 2].copy(),
+                          alpha=st[:, 3].copy())
+
+
+def solve_implicit(stepper, max_iterations: int = 500, residual_orders: float = 8.0, level_rule: bool | None = None,
+                   start_order: str = "first", scales=None, reference=None) -> ImplicitResult:
+    """The steady implicit run of the device stepper (one case) from its current state: per
+    iteration R(u) and the Jacobian on the device, the CFL number from the relative residual, one
+    solve and update (mxeuler_implicit_steady). level_rule defaults to the stepper's policy."""
+    lib = _lib.euler()
+    if level_rule is None:
+        level_rule = stepper.policy == 1
+    keep: list = []
+    opts = _options(max_iterations, residual_orders, STEADY_CFL, level_rule, start_order, scales, reference, keep)
+    res = _Result()
+    hist = np.zeros(5 * max_iterations)
+    prov = np.zeros(max_iterations, dtype=np.int32)
+    stats = np.zeros(4 * max_iterations)
+    err = ctypes.create_string_buffer(2048)
+    rc = lib.mxeuler_implicit_steady(stepper.handle, ctypes.byref(opts), ctypes.byref(res),
+                                     ctypes.c_void_p(hist.ctypes.data), ctypes.c_void_p(prov.ctypes.data),
+                                     ctypes.c_void_p(stats.ctypes.data), err, ctypes.c_size_t(len(err)))
+    if rc != 0:
+        raise RuntimeError(err.value.decode(errors="replace"))
+    return _implicit_finish(res, hist, prov, stats)
+
+
+def solve_implicit_with(target, max_iterations: int = 500, residual_orders: float = 8.0, level_rule: bool = False,
+                        start_order: str = "first", scales=None, reference=None) -> ImplicitResult:
+    """The same run (mximplicit_run) over a Python object with evaluate() -> (residuals[5], invalid
+    or None), set_second_order(on), jacobian() -> (residuals[5], invalid or None), solve(cfl) ->
+    report (5 values: GCR directions, linear drop, alpha, failed, singular) and optionally
+    worst_cell() -> int."""
+    lib = _lib.euler()
+    failure: list[BaseException] = []
+
+    def guard(fn):
+        def wrapped(*args):
+            try:
+                return fn(*args)
+            except BaseException as e:  # reported after the native call returns
+                failure.append(e)
+                return 1
+
+        return wrapped
+
+    def norms_call(method):
+        @guard
+        def call(_ctx, out, inv):
+            r, bad = method()
+            for k in range(5):
+                out[k] = float(r[k])
+            inv[0] = -1 if bad is None else int(bad)
+            return 0
+
+        return call
+
+    @guard
+    def order(_ctx, on):
+        target.set_second_order(bool(on))
+        return 0
+
+    @guard
+    def solve(_ctx, cfl, out):
+        rep = target.solve(cfl)
+        for k in range(5):
+            out[k] = float(rep[k])
+        return 0
+
+    @guard
+    def worst(_ctx, cell):
+        cell[0] = int(target.worst_cell())
+        return 0
+
+    ops = _ImplicitOps(None, _EVAL(norms_call(target.evaluate)), _ORDER(order), _EVAL(norms_call(target.jacobian)),
+                       _SOLVE(solve), _CELL(worst) if hasattr(target, "worst_cell") else _CELL())
+    keep: list = []
+    opts = _options(max_iterations, residual_orders, STEADY_CFL, level_rule, start_order, scales, reference, keep)
+    res = _Result()
+    hist = np.zeros(5 * max_iterations)
+    prov = np.zeros(max_iterations, dtype=np.int32)
+    stats = np.zeros(4 * max_iterations)
+    err = ctypes.create_string_buffer(2048)
+    rc = lib.mximplicit_run(ctypes.byref(ops), ctypes.byref(opts), ctypes.byref(res), ctypes.c_void_p(hist.ctypes.data),
+                            ctypes.c_void_p(prov.ctypes.data), ctypes.c_void_p(stats.ctypes.data), err,
+                            ctypes.c_size_t(len(err)))
+    if failure:
+        raise failure[0]
+    if rc != 0:
+        raise RuntimeError(err.value.decode(errors="replace"))
+    return _implicit_finish(res, hist, prov, stats)

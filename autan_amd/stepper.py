@@ -254,7 +254,8 @@ class DeviceStepper:
         """Euler::residual_and_jacobian: R(u) (5, n) and the approximate Jacobian's parts in FP64,
         the reconstruction's gradients and limiter held fixed: dwdu (5, 5, n) dW/dq per cell,
         face_jac (2, 5, 5, n_interior) dF/dW_L and dF/dW_R per interior face (area-weighted),
-        bnd_jac (5, 5, n_boundary) dF/dW_L per boundary face. Planar meshes."""
+        bnd_jac (5, 5, n_boundary) dF/dW_L per boundary face; on warped faces the face body's warp
+        term's derivative is included."""
         u = self._state(u)
         n, ni, nb = self.n, self.mesh.n_interior, self.mesh.n_boundary
         res = np.empty(5 * n)
@@ -265,6 +266,29 @@ class DeviceStepper:
                    _p(dwdu.ctypes.data), _p(fj.ctypes.data), _p(bj.ctypes.data))
         return {"res": res.reshape(5, n), "dwdu": dwdu.reshape(5, 5, n),
                 "face_jac": fj[: 50 * ni].reshape(2, 5, 5, ni), "bnd_jac": bj[: 25 * nb].reshape(5, 5, nb)}
+
+    def implicit_evaluate(self) -> tuple[np.ndarray, int | None]:
+        """The implicit step's evaluation of the stepper's state (mxeuler_implicit_evaluate): the
+        norms of R(u) at the order set and the first invalid cell; the residual, spectral radius and
+        Jacobian stay on the device for implicit_solve."""
+        r = np.empty(5)
+        inv = ctypes.c_int32()
+        self._call(self.lib.mxeuler_implicit_evaluate, _p(r.ctypes.data), ctypes.byref(inv))
+        return r, (None if inv.value < 0 else inv.value)
+
+    def implicit_solve(self, cfl: float) -> dict:
+        """One solve and update (mxeuler_implicit_solve) at this CFL number: GCR directions taken,
+        the solve's residual drop, the update fraction alpha (0: none), failed, singular."""
+        rep = np.zeros(5)
+        self._call(self.lib.mxeuler_implicit_solve, ctypes.c_double(cfl), _p(rep.ctypes.data))
+        return {"cycles": int(rep[0]), "residual_drop": float(rep[1]), "alpha": float(rep[2]), "failed": bool(rep[3]),
+                "singular": bool(rep[4])}
+
+    def implicit_colours(self) -> np.ndarray:
+        """The colour of every cell in the implicit step's Gauss-Seidel sweeps."""
+        col = np.empty(self.n, dtype=np.int32)
+        self._call(self.lib.mxeuler_implicit_colours, _p(col.ctypes.data))
+        return col
 
     def upload_device(self, u: int, stream: int = 0) -> None:
         """The state from device memory: `u` the address of 5 x n FP64 values on the stepper's GPU,

@@ -125,6 +125,39 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
     return out, total
 
 
+def solve_implicit(specs: list[sweep.Spec], states=None, references=None, max_iterations=500, residual_orders=8.0,
+                   policy="fp64", flux="hllc", sample=None, start_order="first", log=print, **_):
+    """As solve, by the steady implicit path (EulerPhysics::steady_step on the device), one case per
+    stepper in turn. Wall time counts the solves (mesh set-up and upload excluded)."""
+    from autan_amd import steady
+    from autan_amd.stepper import DeviceStepper
+
+    out = []
+    total = 0.0
+    for k, spec in enumerate(specs):
+        c = sweep.build(spec, policy, flux)
+        u0 = c.u if states is None else states[k]
+        dev = DeviceStepper(c.mesh, c.scheme, policy, c.bc_kind, c.bc)
+        dev.upload(u0)
+        ref = None if references is None else np.asarray(references[k])
+        t0 = time.perf_counter()
+        r = steady.solve_implicit(dev, max_iterations=max_iterations, residual_orders=residual_orders,
+                                  scales=sweep.residual_scales(c), reference=ref, start_order=start_order)
+        sec = time.perf_counter() - t0
+        total += sec
+        u = dev.download()
+        faces = np.nonzero(c.mesh.face_zone == 2)[0] + c.mesh.n_interior
+        loads = dev.loads(faces, center=(1.5, 0.0, 0.5), p_inf=1.0 / GAMMA)
+        area = float(np.sum(c.mesh.face_area[faces]))
+        dev.close()
+        if sample is not None:
+            sample(k, c.mesh, spec, u, r)
+        out.append({"result": r, "cells": c.mesh.n_cells, "loads": loads, "wall_area": area, "seconds": sec})
+        if k % 16 == 15 or k == len(specs) - 1:
+            log(f"warm start (implicit): solved cases ..{k} ({total:.1f} s so far)")
+    return out, total
+
+
 class TrainingSet:
     """A random sample of each solved case's cells (solve's `sample` callback): features and final
     primitives."""
@@ -135,12 +168,18 @@ class TrainingSet:
         self.xs: list[np.ndarray] = []
         self.ts: list[np.ndarray] = []
         self.skipped = 0
+        self.converged = 0
+        self.left_out: list[dict] = []
 
     def __call__(self, index, mesh, spec, state, result) -> None:
-        # A case that diverged teaches nothing.
+        # A case that diverged teaches nothing: it is left out and reported (`left_out`).
         if result.status == 2 or not np.all(np.isfinite(state)):
             self.skipped += 1
+            self.left_out.append({"case": int(index), "h": spec.h, "mach": spec.mach, "alpha": spec.alpha,
+                                  "cells": int(mesh.n_cells), "iterations": int(result.iterations),
+                                  "diverged_cell": result.diverged_cell})
             return
+        self.converged += int(result.status == 0)
         n = mesh.n_cells
         pick = np.sort(self.rng.choice(n, size=min(self.per_case, n), replace=False))
         self.xs.append(features(mesh, spec, pick))
@@ -194,15 +233,18 @@ def write_initial_state(path, net, spec: sweep.Spec, policy="fp32+acc64", flux="
 
 
 def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_case: int = 4096, batch: int = 4096,
-                  lr: tuple[float, float] = (1e-3, 1e-5), seed: int = 0, log=print):
-    """The network trained on the training cases' states: each case run `iterations` steady
-    iterations from the freestream (cases that diverge left out), `per_case` random cells of each
-    kept, then `epochs` of Adam with a cosine learning rate from lr[0] to lr[1]. Returns the network
-    and what it took."""
+                  lr: tuple[float, float] = (1e-3, 1e-5), seed: int = 0, log=print, method: str = "explicit",
+                  policy: str = "fp32+acc64"):
+    """The network trained on the training cases' states: each case solved from the freestream
+    (explicit: `iterations` steady iterations of the batch stepper; implicit: the steady implicit
+    path to a 1e-8 relative residual or its round-off plateau, at most `iterations`), cases that
+    diverge left out and listed, `per_case` random cells of each kept, then `epochs` of Adam with a
+    cosine learning rate from lr[0] to lr[1]. Returns the network and what it took."""
     from autan_amd.mlp import MLP
 
     collect = TrainingSet(per_case=per_case, seed=seed)
-    _, t_gen = solve(specs, max_iterations=iterations, residual_orders=8.0, sample=collect, log=log)
+    solver = solve_implicit if method == "implicit" else solve
+    _, t_gen = solver(specs, max_iterations=iterations, residual_orders=8.0, sample=collect, log=log, policy=policy)
     x, t = collect.arrays()
     log(f"training set: {x.shape[0]} cells of {len(specs) - collect.skipped} cases ({collect.skipped} diverged, "
         f"left out; {t_gen:.1f} s of solves)")
@@ -218,20 +260,23 @@ def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_cas
         t_train += sec
         if ep % 10 == 0 or ep == epochs - 1:
             log(f"epoch {ep}: loss {loss:.3e} ({sec:.2f} s)")
-    return net, {"cells": int(x.shape[0]), "diverged_left_out": collect.skipped, "solve_s": round(t_gen, 1),
+    return net, {"cells": int(x.shape[0]), "diverged_left_out": collect.skipped, "left_out": collect.left_out,
+                 "converged": collect.converged, "method": method, "policy": policy, "solve_s": round(t_gen, 1),
                  "train_s": round(t_train, 2), "first_loss": first, "loss": loss}
 
 
-def compare(net, specs: list[sweep.Spec], orders: float, max_iterations: int, log=print) -> dict:
+def compare(net, specs: list[sweep.Spec], orders: float, max_iterations: int, log=print, method: str = "explicit",
+            policy: str = "fp32+acc64") -> dict:
     """The held-out cases solved from the freestream (first-order start) and from the network's state
     (second order from the first step), the latter both against its own first residual and against
     the freestream run's (the absolute threshold a warm start from another run is held to)."""
-    free, t_free = solve(specs, max_iterations=max_iterations, residual_orders=orders, log=log)
-    warm_u, t_infer = warm_states(net, specs)
-    warm, t_warm = solve(specs, states=warm_u, max_iterations=max_iterations, residual_orders=orders,
-                         start_order="second", log=log)
+    solver = solve_implicit if method == "implicit" else solve
+    free, t_free = solver(specs, max_iterations=max_iterations, residual_orders=orders, log=log, policy=policy)
+    warm_u, t_infer = warm_states(net, specs, policy=policy)
+    warm, t_warm = solver(specs, states=warm_u, max_iterations=max_iterations, residual_orders=orders,
+                          start_order="second", log=log, policy=policy)
     refs = [f["result"].reference for f in free]
-    warm_abs, t_warm_abs = solve(specs, states=warm_u, references=refs, max_iterations=max_iterations,
-                                 residual_orders=orders, start_order="second", log=log)
+    warm_abs, t_warm_abs = solver(specs, states=warm_u, references=refs, max_iterations=max_iterations,
+                                  residual_orders=orders, start_order="second", log=log, policy=policy)
     return {"free": free, "warm": warm, "warm_abs": warm_abs, "free_s": t_free, "warm_s": t_warm,
             "warm_abs_s": t_warm_abs, "inference_s": t_infer}

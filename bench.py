@@ -264,12 +264,60 @@ def bump_extras(dev, w, args) -> dict:
     return out
 
 
+def sweep_implicit_main(args) -> int:
+    """The sweep by the steady implicit path: every case from the freestream to a 1e-8 relative
+    residual (fp64) or the round-off plateau (fp32+acc64), one stepper per case in turn; per-case
+    iterations and seconds, and the whole sweep's wall time."""
+    import numpy as np
+
+    from autan_amd import sweep
+
+    policy = args.policy or "fp32+acc64"
+    t0 = time.time()
+    rep = sweep.run_implicit(n_cases=args.cases, seed=1, max_iterations=args.max_iterations, policy=policy,
+                             flux=args.flux or "hllc", log=log)
+    cases = rep["cases"]
+    iters = [c["iterations"] for c in cases]
+    line = {
+        "metric": "wall time to 1e-8 relative residual (or the round-off plateau), whole sweep",
+        "value": round(rep["wall_s"], 2),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": int(max(iters)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64acc" if policy == "fp32+acc64" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.cases} hybrid bump channels (seed 1): cells log-uniform in [2e4, 5e5], h U(0.02, 0.12), "
+            f"Mach U(0.3, 0.8), alpha U(-4, 4) deg; second-order HLLC {policy}, steady implicit path (pseudo-transient "
+            f"continuation, GCR + symmetric multicolour block Gauss-Seidel on the device), first-order start, "
+            f"to a 1e-8 drop or the round-off plateau",
+            "cells": rep["total_cells"],
+        },
+        "cells_iter_per_s": round(rep["cells_iter_per_s"], 1),
+        "converged": sum(c["status"] == "converged" for c in cases),
+        "diverged": sum(c["status"] == "diverged" for c in cases),
+        "iterations": {"min": int(min(iters)), "median": float(np.median(iters)), "max": int(max(iters))},
+        "cases": cases,
+        "setup_s": round(time.time() - t0 - rep["wall_s"], 1),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def sweep_main(args) -> int:
     """The batched sweep: every case stepped together to its steady state; aggregate cells x
     iterations per second over the whole run (one batch, HIP graph replay, autotuned variant)."""
     import numpy as np
 
     from autan_amd import sweep
+
+    if args.method == "implicit":
+        return sweep_implicit_main(args)
 
     t0 = time.time()
     rep = sweep.run(n_cases=args.cases, seed=1, max_iterations=args.max_iterations, policy=args.policy or "fp32+acc64",
@@ -320,12 +368,16 @@ def warmstart_main(args) -> int:
     t0 = time.time()
     train_specs = sweep.specs(args.train_cases, seed=11, cells=(2e4, 1e5))
     test_specs = sweep.specs(args.test_cases, seed=12)
-    net, train = warmstart.train_network(train_specs, args.train_iterations, args.epochs, log=log)
+    implicit = args.method == "implicit"
+    policy = args.policy or ("fp64" if implicit else "fp32+acc64")
+    net, train = warmstart.train_network(train_specs, args.train_iterations if not implicit else args.max_iterations,
+                                         args.epochs, log=log, method=args.method, policy=policy)
     fwd_ms = net.time_forward(1 << 18, 5)
     tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
     log(f"network: loss {train['loss']:.3e} after {args.epochs} epochs ({train['train_s']:.1f} s); forward "
         f"{tflops:.1f} TFLOP/s")
-    cmp = warmstart.compare(net, test_specs, args.test_orders, args.max_iterations, log=log)
+    cmp = warmstart.compare(net, test_specs, args.test_orders, args.max_iterations, log=log, method=args.method,
+                            policy=policy)
     free, warm, warm_abs = cmp["free"], cmp["warm"], cmp["warm_abs"]
     t_free, t_warm, t_warm_abs, t_infer = cmp["free_s"], cmp["warm_s"], cmp["warm_abs_s"], cmp["inference_s"]
     rows = []
@@ -352,13 +404,16 @@ def warmstart_main(args) -> int:
         "unit": "x (freestream seconds / warm seconds)",
         "higher_is_better": True,
         "n_gpus": 1,
-        "dtype": "bf16 (network), f32+f64acc (solver)",
+        "dtype": f"bf16 (network), {'f32+f64acc' if policy == 'fp32+acc64' else 'f64'} (solver)",
         "data": "synthetic",
         "config": {
             "workload": f"MLP 7-256x4-5 SiLU, BF16 matrix cores, trained on {len(train_specs)} bump channels "
-            f"(seed 11, 2e4-1e5 cells) after {args.train_iterations} iterations each, tested on {len(test_specs)} "
-            f"held-out channels (seed 12, 2e4-5e5 cells) solved to {args.test_orders:g} orders below the freestream "
-            f"run's first residual",
+            f"(seed 11, 2e4-1e5 cells) "
+            + (f"solved by the steady implicit path ({policy}) to a 1e-8 drop or the round-off plateau"
+               if implicit else f"after {args.train_iterations} iterations each")
+            + f", tested on {len(test_specs)} held-out channels (seed 12, 2e4-5e5 cells) solved "
+            f"{'by the steady implicit path ' if implicit else ''}to {args.test_orders:g} orders below the "
+            f"freestream run's first residual",
         },
         "train": {**train, "forward_TFLOPs": round(tflops, 1), "forward_ms_262144": round(fwd_ms, 3)},
         "test": {"freestream_s": round(t_free, 2), "warm_s_own_reference": round(t_warm, 2),
@@ -471,6 +526,9 @@ def main() -> int:
     ap.add_argument("--test-orders", type=float, default=2.0,
                     help="warmstart: residual drop the held-out solves run to, from the freestream run's reference")
     ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--method", default="explicit", choices=["explicit", "implicit"],
+                    help="sweep / warmstart: the batch stepper's explicit SSP-RK3 steps, or the steady implicit path "
+                    "(--max-iterations is then its iteration budget per case)")
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
     ap.add_argument("--max-iterations", type=int, default=20000, help="sweep: iteration budget per case")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")

@@ -254,6 +254,48 @@ int mxsteady_run(const mxsteady_ops* ops, const mxsteady_options* options, mxste
 int mxeuler_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result, double* history,
                    int32_t* provisional, char* err, size_t err_len);
 
+/* ---- The steady implicit path (EulerPhysics::steady_step, physics/src/euler_physics.cpp:1056-1250,
+ * with linear_solver = "gauss-seidel") ------------------------------------------------------------
+ * Pseudo-transient continuation (lambda / CFL I + dR/dU) du = -R on the approximate Jacobian, the CFL
+ * number following the residual (switched evolution relaxation from CFL 1000, a penalty halved after
+ * a shortened, rejected or rising step, 1 <= CFL <= 1e6), each system solved by GCR (at most 10
+ * directions, to 0.1; a solve leaving more than 0.9 of its residual rejects the step) preconditioned
+ * by 4 symmetric multicolour block Gauss-Seidel sweeps, and the largest fraction of du (at most 1)
+ * that keeps every cell's density and pressure above half their values. */
+
+/* R(u) of the stepper's state at the order set, its norms (record_norms) and first invalid cell (as
+ * mxeuler_evaluate), the spectral radius and the Jacobian's parts (Euler::residual_and_jacobian,
+ * physics/src/euler.cpp:983-991, warped faces' terms included), kept on the device for the solve. */
+int mxeuler_implicit_evaluate(mxeuler_stepper* s, double norms[5], int32_t* invalid_cell, char* err, size_t err_len);
+/* One solve and update from the last mxeuler_implicit_evaluate: report = {GCR directions taken,
+ * residual drop of the solve, alpha (0: no update), failed (0/1), singular diagonal block (0/1)}. */
+int mxeuler_implicit_solve(mxeuler_stepper* s, double cfl, double report[5], char* err, size_t err_len);
+/* The colour of every cell in the sweeps (BlockMatrix's greedy distance-1 colouring in cell order). */
+int mxeuler_implicit_colours(mxeuler_stepper* s, int32_t* colours, char* err, size_t err_len);
+
+typedef struct mximplicit_ops {
+  void* ctx;
+  /* R(u) at the order set: norms and first invalid cell (-1: none); the history's second order. */
+  int (*evaluate)(void* ctx, double residuals[5], int32_t* invalid_cell);
+  int (*set_second_order)(void* ctx, int32_t on);
+  /* R(u) and the Jacobian at the order set: norms of R(u), first invalid cell (-1: none). */
+  int (*jacobian)(void* ctx, double residuals[5], int32_t* invalid_cell);
+  /* One solve and update at this CFL number; report as mxeuler_implicit_solve's. */
+  int (*solve)(void* ctx, double cfl, double report[5]);
+  /* The cell whose density or pressure the full step lowers the most (optional, may be NULL). */
+  int (*worst_cell)(void* ctx, int32_t* cell);
+} mximplicit_ops;
+
+/* The steady implicit run over any stepper's calls: the runtime's loop and convergence rule
+ * (runtime/src/run.cpp:2262-2389) with the first-order start, options as mxsteady_run's (cfl
+ * unused); stats receives 4 values per iteration (CFL, GCR directions, linear drop, alpha) or is
+ * NULL. */
+int mximplicit_run(const mximplicit_ops* ops, const mxsteady_options* options, mxsteady_result* result,
+                   double* history, int32_t* provisional, double* stats, char* err, size_t err_len);
+/* mximplicit_run over the device stepper's state (upload it first; one case). */
+int mxeuler_implicit_steady(mxeuler_stepper* s, const mxsteady_options* options, mxsteady_result* result,
+                            double* history, int32_t* provisional, double* stats, char* err, size_t err_len);
+
 /* ---- Batches: several meshes (cases) stepped together in one index space ------------------------
  * No counterpart in the reference (a parameter sweep on one GPU). The topology, geometry, stencil
  * and boundary arrays are the union of the cases': case k owns cells [case_first[k], case_first[k]

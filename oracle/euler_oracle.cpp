@@ -599,6 +599,39 @@ void dwdu(const C* dq, const C* ur, C g, C* out) {
   }
 }
 
+// The warped-face term's derivative along the five components of each side's reconstructed state
+// (kernels/euler.py:858-860 differentiates the whole face body, whose warp term reads
+// W = (W_L + W_R) / 2 with the gradients and limiter held): 50 values, entry (k, i, j) at
+// 25 k + 5 i + j, to be added to the area-weighted flux derivatives.
+template <class C>
+void warp_jacobian(const C* fl, const C* fr, const C* gl, const C* pl, const C* gr, const C* pr, const C* m, C g,
+                   C pref, C* out) {
+  using D = Dual<C, 5>;
+  D gld[15], grd[15], pld[5], prd[5], md[9];
+  for (int q = 0; q < 15; ++q) {
+    gld[q] = D(gl[q]);
+    grd[q] = D(gr[q]);
+  }
+  for (int q = 0; q < 5; ++q) {
+    pld[q] = D(pl[q]);
+    prd[q] = D(pr[q]);
+  }
+  for (int q = 0; q < 9; ++q) md[q] = D(m[q]);
+  for (int side = 0; side < 2; ++side) {
+    D a[5], b[5], w[5], o[5];
+    for (int k = 0; k < 5; ++k) {
+      a[k] = D(fl[k]);
+      b[k] = D(fr[k]);
+      (side == 0 ? a : b)[k].d[k] = C(1);
+      w[k] = D(0.5) * (a[k] + b[k]);
+    }
+    warp<D>(w, gld, pld, grd, prd, md, D(g), D(pref), o);
+    for (int i = 0; i < 5; ++i) {
+      for (int j = 0; j < 5; ++j) out[25 * side + 5 * i + j] = o[i].d[j];
+    }
+  }
+}
+
 // Threads of the gather loops: 1 is the reference's serial CPU path; more runs every per-cell
 // (and per-boundary-face) loop of the gather variant in parallel. Each cell's arithmetic and order
 // stay those of the serial loop, so the results do not depend on the count.
@@ -726,6 +759,7 @@ class Base {
   virtual void conservation(const double* u, double* out) = 0;
   virtual int first_invalid(const double* u, const double* res) = 0;
   virtual void set_second_order(int on) = 0;
+  virtual void residual_and_jacobian(const double* u, double* res, double* dwdu, double* fj, double* bj) = 0;
 };
 
 template <class P>
@@ -1024,6 +1058,80 @@ class Solver final : public Base {
     for (int q = 0; q < 3; ++q) r[q] = C(v[q * nf_ + f]);
   }
 
+  // Euler::residual_and_jacobian (physics/src/euler.cpp:983-991): the residual (gather loops),
+  // then the approximate Jacobian's parts in the device's layout, widened to FP64: dW/dq of every
+  // cell at (5 i + j) nc + c; per interior face f dF/dW_L at (5 i + j) ni + f and dF/dW_R at
+  // (25 + 5 i + j) ni + f (the reconstructed states of the face body, gradients and limiter held,
+  // area-weighted, plus the warped-face term's derivative on non-planar meshes); per boundary face
+  // b the derivative of its flux (the pressure flux on slip and wall faces) at (5 i + j) nb + b.
+  void residual_and_jacobian(const double* u, double* res, double* dwdu_out, double* fj, double* bj) override {
+    residual(u, res, true);
+    const C g = C(sc_.gamma), pref = C(p_ref_), minf = C(sc_.mach_ref), rgas = C(sc_.gas_constant);
+    C ur[5];
+    for (int k = 0; k < 5; ++k) ur[k] = C(sc_.reference[k]);
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (std::size_t c = 0; c < nc_; ++c) {
+      C dq[5], o[25];
+      for (int k = 0; k < 5; ++k) dq[k] = C(state_[k * nc_ + c]);
+      dwdu<C>(dq, ur, g, o);
+      for (int q = 0; q < 25; ++q) dwdu_out[q * nc_ + c] = static_cast<double>(o[q]);
+    }
+    const bool second = sc_.second_order != 0;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (std::size_t f = 0; f < ni_; ++f) {
+      const auto l = static_cast<std::size_t>(owner_[f]), r = static_cast<std::size_t>(neighbour_[f]);
+      C wl[5], gl[15], pl[5], wr[5], gr[15], pr[5], n[3], xl[3], xr[3], fl[5], fr[5], o[50];
+      load_prim(l, wl);
+      load_prim(r, wr);
+      load_grad(l, gl);
+      load_grad(r, gr);
+      load_phi(l, pl);
+      load_phi(r, pr);
+      load_normal(f, n);
+      if (second) {
+        rel(f, true, xl);
+        rel(f, false, xr);
+        reconstruct(wl, gl, pl, xl, pref, fl);
+        reconstruct(wr, gr, pr, xr, pref, fr);
+      } else {
+        std::copy(wl, wl + 5, fl);
+        std::copy(wr, wr + 5, fr);
+      }
+      face_jacobian<C>(sc_.flux, fl, fr, n, g, minf, pref, o);
+      const C area = C(area_[f]);
+      C val[50];
+      for (int q = 0; q < 50; ++q) val[q] = area * o[q];
+      if (!sc_.planar) {
+        C m[9], wj[50];
+        load_moment(f, m);
+        warp_jacobian<C>(fl, fr, gl, pl, gr, pr, m, g, pref, wj);
+        for (int q = 0; q < 50; ++q) val[q] = val[q] + wj[q];
+      }
+      for (int q = 0; q < 50; ++q) fj[q * ni_ + f] = static_cast<double>(val[q]);
+    }
+    const std::size_t nb = nf_ - ni_;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (std::size_t b = 0; b < nb; ++b) {
+      const std::size_t f = ni_ + b;
+      const auto l = static_cast<std::size_t>(owner_[f]);
+      C wl[5], gl[15], pl[5], n[3], xl[3], wf[5], data[5], o[25];
+      load_prim(l, wl);
+      load_normal(f, n);
+      if (second) {
+        load_grad(l, gl);
+        load_phi(l, pl);
+        rel(f, true, xl);
+        reconstruct(wl, gl, pl, xl, pref, wf);
+      } else {
+        std::copy(wl, wl + 5, wf);
+      }
+      for (int k = 0; k < 5; ++k) data[k] = C(bc_[k * nb + b]);
+      boundary_jacobian<C>(sc_.flux, wf, C(bc_kind_[b]), data, n, g, rgas, minf, pref, o);
+      const C area = C(area_[f]);
+      for (int q = 0; q < 25; ++q) bj[q * nb + b] = static_cast<double>(area * o[q]);
+    }
+  }
+
   // Impl::cell_loops (physics/src/euler.cpp:113-136), on one rank.
   void cell_loops() {
     const C g = C(sc_.gamma), rgas = C(sc_.gas_constant), pref = C(p_ref_);
@@ -1254,6 +1362,309 @@ class Solver final : public Base {
   std::vector<A> res_;
 };
 
+// ------------------------------------------------------------------------------ implicit step
+// The steady implicit step's linear algebra on one rank, FP64 throughout:
+//   Euler::assemble (physics/src/euler.cpp:993-1053): per cell shift_c I + sum over its faces
+//     (cell_faces order) of +-J D_c, per interior face A(owner, neighbour) = J_R D_neighbour,
+//     A(neighbour, owner) = -J_L D_owner; each product J D summed over k from 0, added as sign * t;
+//   BlockMatrix (physics/src/block_matrix.cpp): the 5 x 5 diagonal blocks inverted by Gauss-Jordan
+//     elimination with partial pivoting, the greedy distance-1 colouring in cell order, relax,
+//     multiply, the symmetric multicolour sweeps;
+//   Gcr::solve (solvers/src/gcr.cpp) on the residual scaled by its largest entry, its inner products
+//     exact sums over the cells (EulerPhysics::owned_dot, owned_max_abs);
+//   EulerPhysics::steady_step's solve and update (physics/src/euler_physics.cpp:1104-1250):
+//     shift = lambda / CFL, GCR to 0.1 in at most 10 directions preconditioned by 4 symmetric sweeps
+//     from zero, a solve leaving more than 0.9 of its residual rejected, then the admissible fraction
+//     of the update (density and pressure kept above half their values, halving down to 1e-6).
+namespace implicit {
+
+constexpr int kSweeps = 4;
+constexpr int kDirections = 10;
+constexpr double kTolerance = 0.1;
+constexpr double kFailure = 0.9;
+constexpr double kMaxDecrease = 0.5;
+constexpr double kMinStep = 1e-6;
+
+struct Topo {
+  std::size_t n, ni, nb;
+  const index_t* owner;
+  const index_t* neighbour;
+  const offset_t* cfo;
+  const index_t* cf;
+};
+
+// out += sign * (j d), each entry's product summed over k from zero.
+inline void add_product(double* out, const double* j, const double* d, double sign) {
+  for (int r = 0; r < 5; ++r) {
+    for (int c = 0; c < 5; ++c) {
+      double t = 0.0;
+      for (int k = 0; k < 5; ++k) t += j[5 * r + k] * d[5 * k + c];
+      out[5 * r + c] += sign * t;
+    }
+  }
+}
+
+// Block k of 25 SoA components over n entities.
+inline void block(const double* v, std::size_t n, std::size_t e, int k, double* b) {
+  for (int q = 0; q < 25; ++q) b[q] = v[(25 * static_cast<std::size_t>(k) + q) * n + e];
+}
+
+inline void assemble(const Topo& t, const double* dwdu, const double* fj, const double* bj, const double* shift,
+                     double* diag, double* upper, double* lower) {
+  std::fill(diag, diag + 25 * t.n, 0.0);
+  std::fill(upper, upper + 25 * t.ni, 0.0);
+  std::fill(lower, lower + 25 * t.ni, 0.0);
+  for (std::size_t c = 0; c < t.n; ++c) {
+    double* dg = diag + 25 * c;
+    for (int i = 0; i < 5; ++i) dg[6 * i] += shift[c];
+    double dc[25], j[25];
+    block(dwdu, t.n, c, 0, dc);
+    for (auto k = t.cfo[c]; k < t.cfo[c + 1]; ++k) {
+      const auto f = static_cast<std::size_t>(t.cf[k]);
+      if (f >= t.ni) {
+        block(bj, t.nb, f - t.ni, 0, j);
+        add_product(dg, j, dc, 1.0);
+      } else if (static_cast<std::size_t>(t.owner[f]) == c) {
+        block(fj, t.ni, f, 0, j);
+        add_product(dg, j, dc, 1.0);
+      } else {
+        block(fj, t.ni, f, 1, j);
+        add_product(dg, j, dc, -1.0);
+      }
+    }
+  }
+  for (std::size_t f = 0; f < t.ni; ++f) {
+    double j[25], dm[25];
+    block(fj, t.ni, f, 1, j);
+    block(dwdu, t.n, static_cast<std::size_t>(t.neighbour[f]), 0, dm);
+    add_product(upper + 25 * f, j, dm, 1.0);
+    block(fj, t.ni, f, 0, j);
+    block(dwdu, t.n, static_cast<std::size_t>(t.owner[f]), 0, dm);
+    add_product(lower + 25 * f, j, dm, -1.0);
+  }
+}
+
+// The inverse of a 5 x 5 row-major block; false when a pivot is zero or not finite.
+inline bool invert(const double* a, double* inv) {
+  double m[5][10];
+  for (int i = 0; i < 5; ++i) {
+    for (int j = 0; j < 5; ++j) {
+      m[i][j] = a[5 * i + j];
+      m[i][5 + j] = i == j ? 1.0 : 0.0;
+    }
+  }
+  for (int k = 0; k < 5; ++k) {
+    int p = k;
+    for (int i = k + 1; i < 5; ++i) {
+      if (std::fabs(m[i][k]) > std::fabs(m[p][k])) p = i;
+    }
+    if (m[p][k] == 0.0 || !std::isfinite(m[p][k])) return false;
+    if (p != k) {
+      for (int j = 0; j < 10; ++j) std::swap(m[k][j], m[p][j]);
+    }
+    const double d = 1.0 / m[k][k];
+    for (int j = 0; j < 10; ++j) m[k][j] *= d;
+    for (int i = 0; i < 5; ++i) {
+      if (i == k) continue;
+      const double f = m[i][k];
+      for (int j = 0; j < 10; ++j) m[i][j] -= f * m[k][j];
+    }
+  }
+  for (int i = 0; i < 5; ++i) {
+    for (int j = 0; j < 5; ++j) inv[5 * i + j] = m[i][5 + j];
+  }
+  return true;
+}
+
+// Each cell's interior faces in ascending face order (the lists BlockMatrix builds), and the
+// greedy colour of every cell in cell order (the least colour none of its face neighbours took).
+struct Rows {
+  std::vector<offset_t> off;
+  std::vector<index_t> face;
+  std::vector<int> colour;
+  int colours = 0;
+};
+
+inline Rows rows_of(const Topo& t) {
+  Rows r;
+  r.off.assign(t.n + 1, 0);
+  for (std::size_t f = 0; f < t.ni; ++f) {
+    ++r.off[static_cast<std::size_t>(t.owner[f]) + 1];
+    ++r.off[static_cast<std::size_t>(t.neighbour[f]) + 1];
+  }
+  for (std::size_t c = 0; c < t.n; ++c) r.off[c + 1] += r.off[c];
+  r.face.resize(static_cast<std::size_t>(r.off[t.n]));
+  std::vector<offset_t> fill(r.off.begin(), r.off.end() - 1);
+  for (std::size_t f = 0; f < t.ni; ++f) {
+    r.face[static_cast<std::size_t>(fill[static_cast<std::size_t>(t.owner[f])]++)] = static_cast<index_t>(f);
+    r.face[static_cast<std::size_t>(fill[static_cast<std::size_t>(t.neighbour[f])]++)] = static_cast<index_t>(f);
+  }
+  r.colour.assign(t.n, -1);
+  std::vector<char> seen;
+  for (std::size_t c = 0; c < t.n; ++c) {
+    seen.assign(static_cast<std::size_t>(r.colours) + 1, 0);
+    for (auto k = r.off[c]; k < r.off[c + 1]; ++k) {
+      const auto f = static_cast<std::size_t>(r.face[static_cast<std::size_t>(k)]);
+      const auto o = static_cast<std::size_t>(static_cast<std::size_t>(t.owner[f]) == c ? t.neighbour[f] : t.owner[f]);
+      if (r.colour[o] >= 0) seen[static_cast<std::size_t>(r.colour[o])] = 1;
+    }
+    int k = 0;
+    while (seen[static_cast<std::size_t>(k)] != 0) ++k;
+    r.colour[c] = k;
+    r.colours = std::max(r.colours, k + 1);
+  }
+  return r;
+}
+
+struct Matrix {
+  const Topo* t;
+  const Rows* rows;
+  const double *diag, *upper, *lower, *inverse;
+
+  // The row of c: the other cell of face f, and its block.
+  const double* off(std::size_t c, std::size_t f, std::size_t& other) const {
+    const bool owned = static_cast<std::size_t>(t->owner[f]) == c;
+    other = static_cast<std::size_t>(owned ? t->neighbour[f] : t->owner[f]);
+    return owned ? upper + 25 * f : lower + 25 * f;
+  }
+  void relax(std::size_t c, const double* b, double* x) const {
+    const std::size_t n = t->n;
+    double r[5];
+    for (int i = 0; i < 5; ++i) r[i] = b[i * n + c];
+    for (auto k = rows->off[c]; k < rows->off[c + 1]; ++k) {
+      std::size_t o;
+      const double* a = off(c, static_cast<std::size_t>(rows->face[static_cast<std::size_t>(k)]), o);
+      for (int i = 0; i < 5; ++i) {
+        for (int j = 0; j < 5; ++j) r[i] -= a[5 * i + j] * x[j * n + o];
+      }
+    }
+    const double* inv = inverse + 25 * c;
+    for (int i = 0; i < 5; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < 5; ++j) s += inv[5 * i + j] * r[j];
+      x[i * n + c] = s;
+    }
+  }
+  void multiply(const double* x, double* y) const {
+    const std::size_t n = t->n;
+    for (std::size_t c = 0; c < n; ++c) {
+      double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      auto add = [&](const double* a, std::size_t col) {
+        for (int i = 0; i < 5; ++i) {
+          for (int j = 0; j < 5; ++j) r[i] += a[5 * i + j] * x[j * n + col];
+        }
+      };
+      add(diag + 25 * c, c);
+      for (auto k = rows->off[c]; k < rows->off[c + 1]; ++k) {
+        std::size_t o;
+        const double* a = off(c, static_cast<std::size_t>(rows->face[static_cast<std::size_t>(k)]), o);
+        add(a, o);
+      }
+      for (int i = 0; i < 5; ++i) y[i * n + c] = r[i];
+    }
+  }
+  // Symmetric sweeps: the colours in increasing, then decreasing order; within a colour no two
+  // cells share a face, so the order of its cells does not matter.
+  void gauss_seidel(const double* b, double* x, int sweeps) const {
+    for (int s = 0; s < sweeps; ++s) {
+      for (int k = 0; k < rows->colours; ++k) {
+        for (std::size_t c = 0; c < t->n; ++c)
+          if (rows->colour[c] == k) relax(c, b, x);
+      }
+      for (int k = rows->colours - 1; k >= 0; --k) {
+        for (std::size_t c = 0; c < t->n; ++c)
+          if (rows->colour[c] == k) relax(c, b, x);
+      }
+    }
+  }
+};
+
+inline double exact_dot(const double* u, const double* v, std::size_t m) {
+  ExactSum s;
+  for (std::size_t i = 0; i < m; ++i) s.add(u[i] * v[i]);
+  return s.value();
+}
+
+inline double max_abs(const double* u, std::size_t m) {
+  double r = 0.0;
+  for (std::size_t i = 0; i < m; ++i) {
+    const double a = std::fabs(u[i]);
+    r = std::isnan(a) ? std::numeric_limits<double>::infinity() : std::max(r, a);
+  }
+  return r;
+}
+
+struct Report {
+  int cycles = 0;
+  double residual_drop = std::numeric_limits<double>::quiet_NaN();
+};
+
+// Gcr::solve with apply = the matrix's product and precondition = the sweeps from zero.
+inline Report gcr(const Matrix& a, double* r, double* x, std::size_t m) {
+  Report rep;
+  const double scale = max_abs(r, m);
+  if (!std::isfinite(scale)) return rep;
+  if (scale == 0.0) {
+    rep.residual_drop = 0.0;
+    return rep;
+  }
+  for (std::size_t i = 0; i < m; ++i) r[i] /= scale;
+  const double r0 = std::sqrt(exact_dot(r, r, m));
+  std::vector<std::vector<double>> z(kDirections), w(kDirections);
+  double rr = r0 * r0;
+  for (int k = 0; k < kDirections; ++k) {
+    z[k].assign(m, 0.0);
+    w[k].assign(m, 0.0);
+    a.gauss_seidel(r, z[k].data(), kSweeps);
+    a.multiply(z[k].data(), w[k].data());
+    for (int j = 0; j < k; ++j) {
+      const double beta = exact_dot(w[k].data(), w[j].data(), m) / exact_dot(w[j].data(), w[j].data(), m);
+      for (std::size_t i = 0; i < m; ++i) {
+        z[k][i] -= beta * z[j][i];
+        w[k][i] -= beta * w[j][i];
+      }
+    }
+    const double ww = exact_dot(w[k].data(), w[k].data(), m);
+    rep.cycles = k + 1;
+    if (!(ww > 0.0)) {
+      if (!std::isfinite(ww)) rr = ww;
+      break;
+    }
+    const double alpha = exact_dot(r, w[k].data(), m) / ww;
+    for (std::size_t i = 0; i < m; ++i) {
+      x[i] += scale * alpha * z[k][i];
+      r[i] -= alpha * w[k][i];
+    }
+    rr = exact_dot(r, r, m);
+    if (std::sqrt(rr) <= kTolerance * r0) break;
+  }
+  for (std::size_t i = 0; i < m; ++i) r[i] *= scale;
+  rep.residual_drop = std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
+  return rep;
+}
+
+// EulerPhysics::pressure(c, a) of u + a du.
+inline double pressure(const double* u, const double* du, std::size_t n, std::size_t c, double a, double gamma) {
+  double v[5];
+  for (int k = 0; k < 5; ++k) v[k] = u[k * n + c] + a * du[k * n + c];
+  return (gamma - 1.0) * (v[4] - 0.5 * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) / v[0]);
+}
+
+inline double admissible_step(const double* u, const double* du, std::size_t n, double gamma) {
+  double alpha = 1.0;
+  for (std::size_t c = 0; c < n; ++c) {
+    double a = 1.0;
+    const double rho = u[c], drho = du[c];
+    if (drho < 0.0) a = std::min(a, kMaxDecrease * rho / -drho);
+    const double p = pressure(u, du, n, c, 0.0, gamma);
+    while (a >= kMinStep && !(pressure(u, du, n, c, a, gamma) >= (1.0 - kMaxDecrease) * p)) a *= 0.5;
+    alpha = std::min(alpha, a);
+  }
+  return alpha;
+}
+
+}  // namespace implicit
+
 }  // namespace oracle
 
 using oracle::Inputs;
@@ -1405,6 +1816,59 @@ void oracle_boundary_value(const double* wg, double kind, const double* data, co
 
 void oracle_to_primitive(const double* dq, const double* ur, double g, double* w) {
   oracle::to_primitive<double>(dq, ur, g, w);
+}
+
+void oracle_residual_and_jacobian(void* h, const double* u, double* res, double* dwdu, double* fj, double* bj) {
+  static_cast<oracle::Base*>(h)->residual_and_jacobian(u, res, dwdu, fj, bj);
+}
+
+// One implicit solve and update from the Jacobian's parts (the device's layout), the spectral
+// radius lam (per cell, widened) and the residual res at state u: assembles lam / cfl I + dR/dU,
+// solves it by GCR and the symmetric sweeps, and unless the solve failed (or a diagonal block is
+// singular: out[4] = 1) adds the admissible fraction of du to u. out: cycles, residual drop, alpha
+// (0 when no update), failed, singular. du receives the solve's result; colours the cells' colours.
+void oracle_implicit_solve(std::int64_t n, std::int64_t ni, const std::int32_t* owner, const std::int32_t* neighbour,
+                           const std::int64_t* cfo, const std::int32_t* cf, const double* dwdu, const double* fj,
+                           const double* bj, const double* lam, double cfl, const double* res, double gamma, double* u,
+                           double* du, double* out, std::int32_t* colours) {
+  namespace im = oracle::implicit;
+  const std::size_t nc = static_cast<std::size_t>(n), nif = static_cast<std::size_t>(ni);
+  const std::size_t nb = static_cast<std::size_t>(cfo[n]) - 2 * nif;
+  const im::Topo t{nc, nif, nb, owner, neighbour, cfo, cf};
+  const im::Rows rows = im::rows_of(t);
+  for (std::size_t c = 0; c < nc; ++c) colours[c] = rows.colour[c];
+  std::vector<double> shift(nc), diag(25 * nc), upper(25 * nif), lower(25 * nif), inverse(25 * nc);
+  for (std::size_t c = 0; c < nc; ++c) shift[c] = lam[c] / cfl;
+  im::assemble(t, dwdu, fj, bj, shift.data(), diag.data(), upper.data(), lower.data());
+  for (int q = 0; q < 5; ++q) out[q] = 0.0;
+  bool singular = false;
+  for (std::size_t c = 0; c < nc && !singular; ++c) singular = !im::invert(&diag[25 * c], &inverse[25 * c]);
+  std::fill(du, du + 5 * nc, 0.0);
+  if (singular) {
+    out[1] = std::numeric_limits<double>::quiet_NaN();
+    out[3] = 1.0;
+    out[4] = 1.0;
+    return;
+  }
+  const im::Matrix a{&t, &rows, diag.data(), upper.data(), lower.data(), inverse.data()};
+  std::vector<double> r(5 * nc);
+  for (std::size_t i = 0; i < 5 * nc; ++i) r[i] = -res[i];
+  const im::Report rep = im::gcr(a, r.data(), du, 5 * nc);
+  out[0] = rep.cycles;
+  out[1] = rep.residual_drop;
+  if (!(rep.residual_drop <= im::kFailure)) {
+    out[3] = 1.0;
+    return;
+  }
+  const double alpha = im::admissible_step(u, du, nc, gamma);
+  out[2] = alpha;
+  if (alpha < im::kMinStep) return;
+  for (std::size_t i = 0; i < 5 * nc; ++i) u[i] += alpha * du[i];
+}
+
+// EulerPhysics::admissible_step of u + alpha du over n cells (5 x n SoA).
+double oracle_admissible_step(std::int64_t n, const double* u, const double* du, double gamma) {
+  return oracle::implicit::admissible_step(u, du, static_cast<std::size_t>(n), gamma);
 }
 
 double oracle_venkat(double d, double dmax, double dmin, double eps2) {

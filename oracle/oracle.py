@@ -88,6 +88,10 @@ def lib() -> ctypes.CDLL:
     L.oracle_face_force.argtypes = [_p, _p, _p, ctypes.c_int64, _p]
     L.oracle_conservation.argtypes = [_p, _p, _p]
     L.oracle_threads.restype = ctypes.c_int
+    L.oracle_residual_and_jacobian.argtypes = [_p] * 6
+    L.oracle_admissible_step.restype = _d
+    L.oracle_admissible_step.argtypes = [ctypes.c_int64, _p, _p, _d]
+    L.oracle_implicit_solve.argtypes = [ctypes.c_int64, ctypes.c_int64] + [_p] * 8 + [_d, _p, _d, _p, _p, _p, _p]
     return L
 
 
@@ -277,6 +281,20 @@ class Oracle:
         c = lib().oracle_first_invalid(self.h, u.ctypes.data, None if r is None else r.ctypes.data)
         return None if c < 0 else c
 
+    def residual_and_jacobian(self, u) -> dict[str, np.ndarray]:
+        """Euler::residual_and_jacobian: the residual (gather loops) and the approximate Jacobian's
+        parts in the device's layout (dwdu 25 x n, face_jac 50 x n_interior, bnd_jac 25 x
+        n_boundary, entry (i, j) of a block at 5 i + j), FP64."""
+        u = _a(u).reshape(-1)
+        ni = self.nf - self.nb
+        out = {"res": np.empty(5 * self.n), "dwdu": np.empty(25 * self.n), "face_jac": np.empty(50 * max(ni, 1)),
+               "bnd_jac": np.empty(25 * max(self.nb, 1))}
+        ptrs = [out[k].ctypes.data for k in ("res", "dwdu", "face_jac", "bnd_jac")]
+        lib().oracle_residual_and_jacobian(self.h, u.ctypes.data, *ptrs)
+        out["face_jac"] = out["face_jac"][: 50 * ni]
+        out["bnd_jac"] = out["bnd_jac"][: 25 * self.nb]
+        return out
+
     def step(self, u, cfl=0.0, dt=None, local=False, gather=True):
         """One SSP-RK3 step of u (copied): (new u, norms, time step, invalid cell)."""
         u = _a(u).reshape(-1).copy()
@@ -286,3 +304,39 @@ class Oracle:
         lib().oracle_step(self.h, u.ctypes.data, cfl, int(dt is not None), 0.0 if dt is None else dt, int(local),
                           int(gather), r.ctypes.data, ctypes.byref(h), ctypes.byref(inv))
         return u, r, h.value, (None if inv.value < 0 else inv.value)
+
+
+def implicit_solve(mesh, parts: dict, lam, cfl: float, gamma: float, u):
+    """One implicit solve and update on the Jacobian's parts (the device's layout), the spectral
+    radius lam and the residual parts["res"] at state u: (lam / cfl I + dR/dU) du = -R by GCR (at
+    most 10 directions, to 0.1) preconditioned by 4 symmetric multicolour block Gauss-Seidel sweeps,
+    then u + alpha du with the admissible alpha. Returns (new u, du, report, colours); report has
+    cycles, residual_drop, alpha (0: no update), failed, singular."""
+    n, ni = mesh.n_cells, mesh.n_interior
+    u = _a(u).reshape(-1).copy()
+    du = np.empty(5 * n)
+    out = np.zeros(5)
+    colours = np.empty(n, dtype=np.int32)
+    keep = [
+        np.ascontiguousarray(mesh.owner, dtype=np.int32),
+        np.ascontiguousarray(mesh.neighbour, dtype=np.int32) if ni else np.zeros(1, np.int32),
+        np.ascontiguousarray(mesh.cell_face_offsets, dtype=np.int64),
+        np.ascontiguousarray(mesh.cell_faces, dtype=np.int32),
+        _a(parts["dwdu"]),
+        _a(parts["face_jac"]) if ni else np.zeros(1),
+        _a(parts["bnd_jac"]) if mesh.n_boundary else np.zeros(1),
+        _a(lam),
+    ]
+    res = _a(parts["res"]).reshape(-1)
+    lib().oracle_implicit_solve(n, ni, *[k.ctypes.data for k in keep], float(cfl), res.ctypes.data, float(gamma),
+                                u.ctypes.data, du.ctypes.data, out.ctypes.data, colours.ctypes.data)
+    report = {"cycles": int(out[0]), "residual_drop": float(out[1]), "alpha": float(out[2]),
+              "failed": bool(out[3]), "singular": bool(out[4])}
+    return u, du, report, colours
+
+
+def admissible_step(u, du, gamma: float) -> float:
+    """EulerPhysics::admissible_step: the largest fraction (at most 1, halved down to 1e-6) of du
+    that keeps every cell's density and pressure above half their values."""
+    u, du = _a(u).reshape(-1), _a(du).reshape(-1)
+    return lib().oracle_admissible_step(u.size // 5, u.ctypes.data, du.ctypes.data, float(gamma))

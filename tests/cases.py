@@ -200,3 +200,25 @@ class OracleTarget:
 
     def set_second_order(self, on: bool) -> None:
         self.ora.set_second_order(on)
+
+
+class OracleImplicitTarget(OracleTarget):
+    """The oracle as the stepper of a steady implicit run (evaluate / set_second_order / jacobian /
+    solve): R(u) and the Jacobian's parts from oracle_residual_and_jacobian, the spectral radius,
+    then oracle_implicit_solve's assembly, GCR, sweeps and admissible update of u."""
+
+    def __init__(self, ora, u, mesh, gamma) -> None:
+        super().__init__(ora, u)
+        self.mesh, self.gamma = mesh, gamma
+        self.parts = None
+
+    def jacobian(self):
+        self.parts = self.ora.residual_and_jacobian(self.u)
+        return self.norms(self.parts["res"]), self.ora.first_invalid(self.u, self.parts["res"])
+
+    def solve(self, cfl):
+        from oracle import oracle as orc
+
+        lam = self.ora.spectral_radius()
+        self.u, self.du, rep, _ = orc.implicit_solve(self.mesh, self.parts, lam, cfl, self.gamma, self.u)
+        return [rep["cycles"], rep["residual_drop"], rep["alpha"], float(rep["failed"]), float(rep["singular"])]
