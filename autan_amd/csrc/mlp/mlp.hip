@@ -519,6 +519,32 @@ class Net {
     return loss;
   }
 
+  // One batch's forward and backward, then one of its tensors to the host as FP32, row-major
+  // n x width: what 0 the input activations of layer `layer` (BF16: the normalised inputs for
+  // layer 0), 1 its pre-activations Z (FP32), 2 the gradient dZ of its pre-activations (BF16).
+  void trace(const float* x, const float* t, int n, int what, int layer, float* out) {
+    const int nl = layers();
+    if (layer < 0 || layer >= nl || what < 0 || what > 2) throw std::invalid_argument("trace: no such tensor");
+    Buf<float> xd(std::size_t(n) * d_in_), td(std::size_t(n) * d_out_);
+    check(hipMemcpyAsync(xd.get(), x, std::size_t(n) * d_in_ * sizeof(float), hipMemcpyHostToDevice, s_), "hipMemcpy");
+    check(hipMemcpyAsync(td.get(), t, std::size_t(n) * d_out_ * sizeof(float), hipMemcpyHostToDevice, s_), "hipMemcpy");
+    ensure(n);
+    check(hipMemsetAsync(loss_sum_.get(), 0, sizeof(double), s_), "hipMemsetAsync");
+    backward(xd.get(), td.get(), n);
+    const std::size_t l = static_cast<std::size_t>(layer);
+    const std::size_t count = std::size_t(n) * std::size_t(what == 0 ? w_[l] : w_[l + 1]);
+    if (what == 1) {
+      check(hipMemcpyAsync(out, z_[l].get(), count * sizeof(float), hipMemcpyDeviceToHost, s_), "hipMemcpy");
+      check(hipStreamSynchronize(s_), "trace");
+      return;
+    }
+    std::vector<bf16> h(count);
+    const bf16* src = what == 0 ? acts_[l].get() : dz_[l].get();
+    check(hipMemcpyAsync(h.data(), src, count * sizeof(bf16), hipMemcpyDeviceToHost, s_), "hipMemcpy");
+    check(hipStreamSynchronize(s_), "trace");
+    for (std::size_t i = 0; i < count; ++i) out[i] = static_cast<float>(h[i]);
+  }
+
   // Training data resident on the device; epochs of shuffled batches with Adam.
   void set_data(const float* x, const float* t, long long n) {
     data_x_ = Buf<float>(std::size_t(n) * d_in_);
@@ -824,6 +850,14 @@ int mxmlp_gradients(mxmlp* h, const float* x, const float* t, int32_t n, float* 
   return mxm::guarded(err, err_len, [&] {
     if (n < 1) throw std::invalid_argument("n >= 1");
     *loss = h->net->gradients(x, t, n, grads);
+  });
+}
+
+int mxmlp_trace(mxmlp* h, const float* x, const float* t, int32_t n, int32_t what, int32_t layer, float* out,
+                char* err, size_t err_len) {
+  return mxm::guarded(err, err_len, [&] {
+    if (n < 1 || !x || !t || !out) throw std::invalid_argument("trace: n >= 1 and arrays");
+    h->net->trace(x, t, n, what, layer, out);
   });
 }
 

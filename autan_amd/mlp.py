@@ -100,6 +100,18 @@ class MLP:
                    _p(g.ctypes.data), ctypes.byref(loss))
         return loss.value, g
 
+    def trace(self, x: np.ndarray, t: np.ndarray, what: str, layer: int) -> np.ndarray:
+        """One batch's forward and backward, then a tensor of layer `layer` (n x width, FP32):
+        "input" its input activations (BF16 values), "z" its pre-activations, "dz" the loss's
+        gradient by them (BF16 values)."""
+        x, t = _f32(x), _f32(t)
+        w = self.widths()
+        width = w[layer] if what == "input" else w[layer + 1]
+        out = np.empty((x.shape[0], width), dtype=np.float32)
+        self._call(self.lib.mxmlp_trace, _p(x.ctypes.data), _p(t.ctypes.data), ctypes.c_int32(x.shape[0]),
+                   ctypes.c_int32({"input": 0, "z": 1, "dz": 2}[what]), ctypes.c_int32(layer), _p(out.ctypes.data))
+        return out
+
     def adam_step(self, lr: float) -> None:
         self._call(self.lib.mxmlp_adam_step, ctypes.c_float(lr))
 

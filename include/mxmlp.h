@@ -33,6 +33,12 @@ int mxmlp_predict(mxmlp* h, const float* x, int64_t n, float* y, char* err, size
  * NULL), in parameter order. */
 int mxmlp_gradients(mxmlp* h, const float* x, const float* t, int32_t n, float* grads, float* loss, char* err,
                     size_t err_len);
+/* One batch's forward and backward (as mxmlp_gradients), then one of its tensors, row-major n x
+ * width in FP32: what = 0 the input activations of `layer` (BF16 values; layer 0: the normalised,
+ * padded inputs), 1 its pre-activations Z, 2 the gradient of the loss by Z (BF16 values). For tests
+ * that check each product and epilogue on its own inputs. */
+int mxmlp_trace(mxmlp* h, const float* x, const float* t, int32_t n, int32_t what, int32_t layer, float* out,
+                char* err, size_t err_len);
 /* One Adam step with the gradients of the last mxmlp_gradients call. */
 int mxmlp_adam_step(mxmlp* h, float lr, char* err, size_t err_len);
 /* Training data kept on the device, then epochs of shuffled batches, one Adam step per batch. */
