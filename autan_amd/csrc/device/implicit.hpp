@@ -328,6 +328,106 @@ __global__ __launch_bounds__(kBlock) void k_impl_admissible(const double* __rest
   if ((threadIdx.x % kWave) == 0 && key != ~0ULL) atomicMin(bits, key);
 }
 
+// The Krylov operator's boundary term (EulerPhysics::add_boundary_gradient_term with
+// Euler::gradient_tangent, physics/src/euler_physics.cpp:1300-1320, physics/src/euler.cpp:1081-1119):
+// y += B v, the boundary faces' fluxes' derivative through the reconstruction's gradients, which
+// the assembled blocks drop. One thread per cell owning a boundary face (`cells`, ascending): the
+// primitive tangents dW = dW/dq v of the cell and its stencil cells (the direction rounded to the
+// compute type, the products summed in it), the gradient's tangent sum_k (dW_k - dW_c) w_k (FP64
+// sums, boundary values held), then over its boundary faces in cell_faces order, where the
+// reconstruction did not fall back to the cell value, bnd_jac applied to phi (grad_tan . r), and on
+// warped slip and wall faces the wall-warp term's tangent phi_p M^T grad_tan p.
+template <class C>
+__device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, const double* __restrict__ v, long long n,
+                                             long long c, C* dw) {
+  C vc[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) vc[j] = static_cast<C>(v[j * n + c]);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    C t = C(0);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) t = t + static_cast<C>(dwdu[(5 * i + j) * n + c]) * vc[j];
+    dw[i] = t;
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_impl_bgrad(Dev<C> d, const int* __restrict__ cells, int count,
+                                                       const double* __restrict__ dwdu, const double* __restrict__ bj,
+                                                       const double* __restrict__ v, double* __restrict__ y) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int c = cells[t];
+  const long long n = d.nc;
+  const Params<C>& pa = d.ptab[cell_case(d, c)];
+  C dwc[5];
+  impl_tangent<C>(dwdu, v, n, c, dwc);
+  double acc[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) acc[q] = 0.0;
+  const long long base = d.soff[c / kWave] + (c % kWave);
+  const int len = d.slen[c];
+  for (int j = 0; j < len; ++j) {
+    const long long k = base + static_cast<long long>(j) * kWave;
+    const int e = d.sidx[k];
+    const C w0 = d.sw0[k], w1 = d.sw1[k], w2 = d.sw2[k];
+    C dwn[5] = {C(0), C(0), C(0), C(0), C(0)};
+    if (e >= 0) impl_tangent<C>(dwdu, v, n, e, dwn);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const C dd = dwn[i] - dwc[i];
+      acc[3 * i] += static_cast<double>(dd * w0);
+      acc[3 * i + 1] += static_cast<double>(dd * w1);
+      acc[3 * i + 2] += static_cast<double>(dd * w2);
+    }
+  }
+  C gt[15];
+#pragma unroll
+  for (int q = 0; q < 15; ++q) gt[q] = static_cast<C>(acc[q]);
+  const W5<C> w = load_w(d.prim, c);
+  const C wc[5] = {w.r, w.u, w.v, w.w, w.p};
+  C rec[20];
+  load_rec(d.rec, c, rec);
+  const C* gc = rec;
+  const C* pc = rec + 15;
+  double dres[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
+    const int f = d.inc[k] & 0x7fffffff;
+    if (f < d.ni) continue;
+    const int b = f - d.ni;
+    C g8[8];
+    load_n<C, 8>(d.fgeo + 8LL * f, g8);
+    const C r[3] = {g8[4], g8[5], g8[6]};
+    const C d00 = gc[0] * r[0] + gc[1] * r[1] + gc[2] * r[2];
+    const C d04 = gc[12] * r[0] + gc[13] * r[1] + gc[14] * r[2];
+    const bool fell_back = !((wc[0] + pc[0] * d00) > C(0.0) && (wc[4] + pc[4] * d04) + pa.pref > C(0.0));
+    if (fell_back) continue;
+    C dwf[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) dwf[i] = pc[i] * (gt[3 * i] * r[0] + gt[3 * i + 1] * r[1] + gt[3 * i + 2] * r[2]);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) s += bj[static_cast<long long>(5 * i + j) * d.nb + b] * static_cast<double>(dwf[j]);
+      dres[i] += s;
+    }
+    const int kind = d.bkind[b];
+    if (d.moment != nullptr && (kind == 2 || kind >= 5)) {
+      C m9[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m9[q] = d.moment[static_cast<long long>(q) * d.nf + f];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        dres[1 + i] += static_cast<double>(pc[4] * (m9[i] * gt[12] + m9[3 + i] * gt[13] + m9[6 + i] * gt[14]));
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) y[i * n + c] += dres[i];
+}
+
 // ---- the GCR's vector operations (solvers/src/gcr.cpp), m values each
 
 // y = -x

@@ -327,6 +327,7 @@ class Stepper final : public StepperBase {
     });
     check(hipStreamSynchronize(stream_), "implicit_evaluate");
     have_jacobian_ = true;
+    jac_second_ = second_;
   }
 
   // Assembles lambda / cfl I + dR/dU from the last implicit_evaluate, solves it for du by GCR (at
@@ -465,6 +466,17 @@ class Stepper final : public StepperBase {
     for (int k = 0; k < colours; ++k) colour_off_[static_cast<std::size_t>(k) + 1] += colour_off_[static_cast<std::size_t>(k)];
     std::vector<int> at(colour_off_.begin(), colour_off_.end() - 1);
     for (std::size_t c = 0; c < n; ++c) cells[static_cast<std::size_t>(at[static_cast<std::size_t>(colour_host_[c])]++)] = static_cast<int>(c);
+    std::vector<int> bo;
+    for (std::size_t c = 0; c < n; ++c) {
+      for (int k = cfo_host_[c]; k < cfo_host_[c + 1]; ++k) {
+        if (cell_faces_host_[static_cast<std::size_t>(k)] >= ni_) {
+          bo.push_back(static_cast<int>(c));
+          break;
+        }
+      }
+    }
+    nbo_ = static_cast<int>(bo.size());
+    if (nbo_ > 0) bocells_.upload(bo, stream_);
     roff_.upload(off, stream_);
     rent_.upload(ent, stream_);
     ccells_.upload(cells, stream_);
@@ -544,6 +556,14 @@ class Stepper final : public StepperBase {
       launch("impl_multiply", [&] {
         k_impl_multiply<<<blocks(nc_), kBlock, 0, stream_>>>(rows, nc_, diag_.get(), offd_.get(), z, w);
       });
+      // At second order the operator adds the boundary faces' gradient term.
+      if (jac_second_ && nbo_ > 0) {
+        const Dev<C> d = dev();
+        launch("impl_bgrad", [&] {
+          k_impl_bgrad<C><<<(nbo_ + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(d, bocells_.get(), nbo_, jd_.get(),
+                                                                              jb_.get(), z, w);
+        });
+      }
       for (int j = 0; j < k; ++j) {
         const double* zj = zs_.get() + std::size_t(j) * std::size_t(m);
         const double* wj = ws_.get() + std::size_t(j) * std::size_t(m);
@@ -566,7 +586,9 @@ class Stepper final : public StepperBase {
     return std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
   }
 
-  bool implicit_ready_ = false, have_jacobian_ = false;
+  bool implicit_ready_ = false, have_jacobian_ = false, jac_second_ = false;
+  int nbo_ = 0;  // cells owning a boundary face
+  Buffer<int> bocells_;
   int dot_next_ = 0;
   std::vector<int> colour_host_, colour_off_;
   Buffer<int> roff_, ccells_, flag_;

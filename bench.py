@@ -370,6 +370,10 @@ def warmstart_main(args) -> int:
     test_specs = sweep.specs(args.test_cases, seed=12)
     implicit = args.method == "implicit"
     policy = args.policy or ("fp64" if implicit else "fp32+acc64")
+    # The implicit path runs to the configuration's 1e-8 within a few hundred iterations; the
+    # explicit one stops at 2 orders (its 20000-iteration budget reaches no further).
+    if args.test_orders is None:
+        args.test_orders = 8.0 if implicit else 2.0
     net, train = warmstart.train_network(train_specs, args.train_iterations if not implicit else args.max_iterations,
                                          args.epochs, log=log, method=args.method, policy=policy)
     fwd_ms = net.time_forward(1 << 18, 5)
@@ -523,14 +527,16 @@ def main() -> int:
     ap.add_argument("--train-cases", type=int, default=512, help="warmstart: solved cases the network learns from")
     ap.add_argument("--test-cases", type=int, default=32, help="warmstart: held-out cases")
     ap.add_argument("--train-iterations", type=int, default=8000, help="warmstart: iterations of the training solves")
-    ap.add_argument("--test-orders", type=float, default=2.0,
-                    help="warmstart: residual drop the held-out solves run to, from the freestream run's reference")
+    ap.add_argument("--test-orders", type=float, default=None,
+                    help="warmstart: residual drop the held-out solves run to, from the freestream run's reference "
+                    "(default 8 with --method implicit, 2 with the explicit method)")
     ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--method", default="explicit", choices=["explicit", "implicit"],
                     help="sweep / warmstart: the batch stepper's explicit SSP-RK3 steps, or the steady implicit path "
                     "(--max-iterations is then its iteration budget per case)")
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
-    ap.add_argument("--max-iterations", type=int, default=20000, help="sweep: iteration budget per case")
+    ap.add_argument("--max-iterations", type=int, default=None,
+                    help="sweep / warmstart / mms: iteration budget per case (default 20000, 600 with --method implicit)")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
     ap.add_argument("--bump-size", default="462,154,154", help="bump: lattice nx,ny,nz")
     ap.add_argument("--policy", default=None, choices=["fp32+acc64", "fp64"])
@@ -547,6 +553,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="keep the default kernel variants")
     ap.add_argument("--no-extras", action="store_true", help="bump: skip the steady run, conservation and fp64 check")
     args = ap.parse_args()
+    if args.max_iterations is None:
+        args.max_iterations = 600 if args.method == "implicit" else 20000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the stepper holds a whole mesh on one device and has no "
               "multi-GPU path. Run with --gpus 1.", file=sys.stderr)

@@ -760,6 +760,7 @@ class Base {
   virtual int first_invalid(const double* u, const double* res) = 0;
   virtual void set_second_order(int on) = 0;
   virtual void residual_and_jacobian(const double* u, double* res, double* dwdu, double* fj, double* bj) = 0;
+  virtual void boundary_gradient_term(const double* v, const double* dwdu, const double* bj, double* y) const = 0;
 };
 
 template <class P>
@@ -1129,6 +1130,85 @@ class Solver final : public Base {
       boundary_jacobian<C>(sc_.flux, wf, C(bc_kind_[b]), data, n, g, rgas, minf, pref, o);
       const C area = C(area_[f]);
       for (int q = 0; q < 25; ++q) bj[q * nb + b] = static_cast<double>(area * o[q]);
+    }
+  }
+
+  // The steady implicit step's Krylov operator adds the boundary faces' derivative through the
+  // reconstruction's gradients, which the assembled blocks drop (EulerPhysics::
+  // add_boundary_gradient_term with Euler::gradient_tangent, physics/src/euler_physics.cpp:1300-1320,
+  // physics/src/euler.cpp:1081-1119), for the state of the last residual_and_jacobian: y += B v.
+  // For each cell owning a boundary face: its stencil cells' primitive tangents dW = dW/dq v (the
+  // direction rounded to the state type, the products summed in the compute type), the gradient's
+  // tangent sum_k (dW_k - dW_c) w_k (FP64 sums, boundary values held), then over its boundary faces
+  // in cell_faces order, where the reconstruction did not fall back to the cell value, the face
+  // state's tangent phi (grad_tan . r) through the boundary block dF/dW_f, and on warped slip and
+  // wall faces the wall-warp term's tangent phi_p M^T grad_tan p.
+  void boundary_gradient_term(const double* v, const double* dwdu, const double* bj, double* y) const override {
+    if (!sc_.second_order) return;
+    const C pref = C(p_ref_);
+    const std::size_t nb = nf_ - ni_;
+    auto tangent = [&](std::size_t c, C* dw) {
+      for (int i = 0; i < 5; ++i) {
+        C t = C(0);
+        for (int j = 0; j < 5; ++j) t = t + C(dwdu[static_cast<std::size_t>(5 * i + j) * nc_ + c]) * C(S(v[static_cast<std::size_t>(j) * nc_ + c]));
+        dw[i] = t;
+      }
+    };
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (std::size_t c = 0; c < nc_; ++c) {
+      bool owns = false;
+      for (auto k = cfo_[c]; k < cfo_[c + 1] && !owns; ++k) owns = static_cast<std::size_t>(cf_[static_cast<std::size_t>(k)]) >= ni_;
+      if (!owns) continue;
+      C dwc[5];
+      tangent(c, dwc);
+      double acc[15] = {};
+      for (auto k = so_[c]; k < so_[c + 1]; ++k) {
+        const index_t e = st_[static_cast<std::size_t>(k)];
+        C dwn[5] = {C(0), C(0), C(0), C(0), C(0)};
+        if (e >= 0) tangent(static_cast<std::size_t>(e), dwn);
+        C w[3];
+        for (int j = 0; j < 3; ++j) w[j] = C(weight_[static_cast<std::size_t>(j) * ns_ + static_cast<std::size_t>(k)]);
+        for (int i = 0; i < 5; ++i) {
+          const C d = dwn[i] - dwc[i];
+          for (int j = 0; j < 3; ++j) acc[3 * i + j] += static_cast<double>(d * w[j]);
+        }
+      }
+      C gt[15];
+      for (int q = 0; q < 15; ++q) gt[q] = C(acc[q]);
+      C wc[5], gc[15], pc[5];
+      load_prim(c, wc);
+      load_grad(c, gc);
+      load_phi(c, pc);
+      double dres[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (auto k = cfo_[c]; k < cfo_[c + 1]; ++k) {
+        const auto f = static_cast<std::size_t>(cf_[static_cast<std::size_t>(k)]);
+        if (f < ni_) continue;
+        const std::size_t b = f - ni_;
+        C r[3];
+        rel(f, true, r);
+        // reconstruct's test: the cell value where the reconstructed density or pressure is not positive.
+        const C d0[5] = {gc[0] * r[0] + gc[1] * r[1] + gc[2] * r[2], gc[3] * r[0] + gc[4] * r[1] + gc[5] * r[2],
+                         gc[6] * r[0] + gc[7] * r[1] + gc[8] * r[2], gc[9] * r[0] + gc[10] * r[1] + gc[11] * r[2],
+                         gc[12] * r[0] + gc[13] * r[1] + gc[14] * r[2]};
+        const bool fell_back = !((wc[0] + pc[0] * d0[0]) > C(0.0) && (wc[4] + pc[4] * d0[4]) + pref > C(0.0));
+        if (fell_back) continue;
+        C dwf[5];
+        for (int i = 0; i < 5; ++i) dwf[i] = pc[i] * (gt[3 * i] * r[0] + gt[3 * i + 1] * r[1] + gt[3 * i + 2] * r[2]);
+        for (int i = 0; i < 5; ++i) {
+          double t = 0.0;
+          for (int j = 0; j < 5; ++j) t += bj[static_cast<std::size_t>(5 * i + j) * nb + b] * static_cast<double>(dwf[j]);
+          dres[i] += t;
+        }
+        const C kind = C(bc_kind_[b]);
+        if (!sc_.planar && ((kind > C(1.5) && kind < C(2.5)) || kind > C(4.5))) {
+          C m[9];
+          load_moment(f, m);
+          for (int i = 0; i < 3; ++i) {
+            dres[1 + i] += static_cast<double>(pc[4] * (m[i] * gt[12] + m[3 + i] * gt[13] + m[6 + i] * gt[14]));
+          }
+        }
+      }
+      for (int i = 0; i < 5; ++i) y[static_cast<std::size_t>(i) * nc_ + c] += dres[i];
     }
   }
 
@@ -1599,8 +1679,10 @@ struct Report {
   double residual_drop = std::numeric_limits<double>::quiet_NaN();
 };
 
-// Gcr::solve with apply = the matrix's product and precondition = the sweeps from zero.
-inline Report gcr(const Matrix& a, double* r, double* x, std::size_t m) {
+// Gcr::solve with apply = the matrix's product (plus `extra`, when given) and precondition = the
+// sweeps from zero.
+template <class Extra>
+inline Report gcr(const Matrix& a, double* r, double* x, std::size_t m, const Extra& extra) {
   Report rep;
   const double scale = max_abs(r, m);
   if (!std::isfinite(scale)) return rep;
@@ -1617,6 +1699,7 @@ inline Report gcr(const Matrix& a, double* r, double* x, std::size_t m) {
     w[k].assign(m, 0.0);
     a.gauss_seidel(r, z[k].data(), kSweeps);
     a.multiply(z[k].data(), w[k].data());
+    extra(z[k].data(), w[k].data());
     for (int j = 0; j < k; ++j) {
       const double beta = exact_dot(w[k].data(), w[j].data(), m) / exact_dot(w[j].data(), w[j].data(), m);
       for (std::size_t i = 0; i < m; ++i) {
@@ -1823,14 +1906,16 @@ void oracle_residual_and_jacobian(void* h, const double* u, double* res, double*
 }
 
 // One implicit solve and update from the Jacobian's parts (the device's layout), the spectral
-// radius lam (per cell, widened) and the residual res at state u: assembles lam / cfl I + dR/dU,
+// radius lam (per cell, widened) and the residual res at state u (h the oracle that evaluated them,
+// second the order of that evaluation: at second order the Krylov operator adds the boundary
+// faces' gradient term): assembles lam / cfl I + dR/dU,
 // solves it by GCR and the symmetric sweeps, and unless the solve failed (or a diagonal block is
 // singular: out[4] = 1) adds the admissible fraction of du to u. out: cycles, residual drop, alpha
 // (0 when no update), failed, singular. du receives the solve's result; colours the cells' colours.
-void oracle_implicit_solve(std::int64_t n, std::int64_t ni, const std::int32_t* owner, const std::int32_t* neighbour,
-                           const std::int64_t* cfo, const std::int32_t* cf, const double* dwdu, const double* fj,
-                           const double* bj, const double* lam, double cfl, const double* res, double gamma, double* u,
-                           double* du, double* out, std::int32_t* colours) {
+void oracle_implicit_solve(void* h, int second, std::int64_t n, std::int64_t ni, const std::int32_t* owner,
+                           const std::int32_t* neighbour, const std::int64_t* cfo, const std::int32_t* cf,
+                           const double* dwdu, const double* fj, const double* bj, const double* lam, double cfl,
+                           const double* res, double gamma, double* u, double* du, double* out, std::int32_t* colours) {
   namespace im = oracle::implicit;
   const std::size_t nc = static_cast<std::size_t>(n), nif = static_cast<std::size_t>(ni);
   const std::size_t nb = static_cast<std::size_t>(cfo[n]) - 2 * nif;
@@ -1853,7 +1938,12 @@ void oracle_implicit_solve(std::int64_t n, std::int64_t ni, const std::int32_t* 
   const im::Matrix a{&t, &rows, diag.data(), upper.data(), lower.data(), inverse.data()};
   std::vector<double> r(5 * nc);
   for (std::size_t i = 0; i < 5 * nc; ++i) r[i] = -res[i];
-  const im::Report rep = im::gcr(a, r.data(), du, 5 * nc);
+  // The boundary faces' gradient term, when the Jacobian was assembled at second order.
+  const auto* solver = static_cast<const oracle::Base*>(h);
+  auto extra = [&](const double* v, double* y) {
+    if (second != 0 && solver != nullptr) solver->boundary_gradient_term(v, dwdu, bj, y);
+  };
+  const im::Report rep = im::gcr(a, r.data(), du, 5 * nc, extra);
   out[0] = rep.cycles;
   out[1] = rep.residual_drop;
   if (!(rep.residual_drop <= im::kFailure)) {
@@ -1867,6 +1957,12 @@ void oracle_implicit_solve(std::int64_t n, std::int64_t ni, const std::int32_t* 
 }
 
 // EulerPhysics::admissible_step of u + alpha du over n cells (5 x n SoA).
+// y += the implicit step's boundary gradient term of direction v for the state of the last
+// oracle_residual_and_jacobian (its dwdu and bnd_jac).
+void oracle_boundary_term(void* h, const double* v, const double* dwdu, const double* bj, double* y) {
+  static_cast<oracle::Base*>(h)->boundary_gradient_term(v, dwdu, bj, y);
+}
+
 double oracle_admissible_step(std::int64_t n, const double* u, const double* du, double gamma) {
   return oracle::implicit::admissible_step(u, du, static_cast<std::size_t>(n), gamma);
 }

@@ -91,7 +91,7 @@ def lib() -> ctypes.CDLL:
     L.oracle_residual_and_jacobian.argtypes = [_p] * 6
     L.oracle_admissible_step.restype = _d
     L.oracle_admissible_step.argtypes = [ctypes.c_int64, _p, _p, _d]
-    L.oracle_implicit_solve.argtypes = [ctypes.c_int64, ctypes.c_int64] + [_p] * 8 + [_d, _p, _d, _p, _p, _p, _p]
+    L.oracle_implicit_solve.argtypes = [_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int64] + [_p] * 8 + [_d, _p, _d, _p, _p, _p, _p]
     return L
 
 
@@ -306,12 +306,14 @@ class Oracle:
         return u, r, h.value, (None if inv.value < 0 else inv.value)
 
 
-def implicit_solve(mesh, parts: dict, lam, cfl: float, gamma: float, u):
+def implicit_solve(mesh, parts: dict, lam, cfl: float, gamma: float, u, oracle=None, second: bool = False):
     """One implicit solve and update on the Jacobian's parts (the device's layout), the spectral
     radius lam and the residual parts["res"] at state u: (lam / cfl I + dR/dU) du = -R by GCR (at
     most 10 directions, to 0.1) preconditioned by 4 symmetric multicolour block Gauss-Seidel sweeps,
-    then u + alpha du with the admissible alpha. Returns (new u, du, report, colours); report has
-    cycles, residual_drop, alpha (0: no update), failed, singular."""
+    then u + alpha du with the admissible alpha. With `oracle` (the Oracle whose
+    residual_and_jacobian made the parts) and `second` (they were made at second order), the Krylov
+    operator adds the boundary faces' gradient term. Returns (new u, du, report, colours); report
+    has cycles, residual_drop, alpha (0: no update), failed, singular."""
     n, ni = mesh.n_cells, mesh.n_interior
     u = _a(u).reshape(-1).copy()
     du = np.empty(5 * n)
@@ -328,7 +330,9 @@ def implicit_solve(mesh, parts: dict, lam, cfl: float, gamma: float, u):
         _a(lam),
     ]
     res = _a(parts["res"]).reshape(-1)
-    lib().oracle_implicit_solve(n, ni, *[k.ctypes.data for k in keep], float(cfl), res.ctypes.data, float(gamma),
+    h = None if oracle is None else oracle.h
+    lib().oracle_implicit_solve(h, int(bool(second)), n, ni, *[k.ctypes.data for k in keep], float(cfl), res.ctypes.data,
+                                float(gamma),
                                 u.ctypes.data, du.ctypes.data, out.ctypes.data, colours.ctypes.data)
     report = {"cycles": int(out[0]), "residual_drop": float(out[1]), "alpha": float(out[2]),
               "failed": bool(out[3]), "singular": bool(out[4])}

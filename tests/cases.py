@@ -211,14 +211,22 @@ class OracleImplicitTarget(OracleTarget):
         super().__init__(ora, u)
         self.mesh, self.gamma = mesh, gamma
         self.parts = None
+        self.second = True
+        self.second_of_parts = True
+
+    def set_second_order(self, on: bool) -> None:
+        super().set_second_order(on)
+        self.second = bool(on)
 
     def jacobian(self):
         self.parts = self.ora.residual_and_jacobian(self.u)
+        self.second_of_parts = self.second
         return self.norms(self.parts["res"]), self.ora.first_invalid(self.u, self.parts["res"])
 
     def solve(self, cfl):
         from oracle import oracle as orc
 
         lam = self.ora.spectral_radius()
-        self.u, self.du, rep, _ = orc.implicit_solve(self.mesh, self.parts, lam, cfl, self.gamma, self.u)
+        self.u, self.du, rep, _ = orc.implicit_solve(self.mesh, self.parts, lam, cfl, self.gamma, self.u, oracle=self.ora,
+                                                     second=self.second_of_parts)
         return [rep["cycles"], rep["residual_drop"], rep["alpha"], float(rep["failed"]), float(rep["singular"])]

@@ -226,3 +226,79 @@ def test_an_oracle_implicit_run_converges_on_a_sweep_channel():
     assert r.status == steady.CONVERGED and r.iterations < 100
     assert r.first_order_end == "drop"
     assert r.cfl[0] == 1000.0 and r.cfl.max() == 1e6
+
+
+@pytest.mark.parametrize("flux", ["hllc", "ausm+up"])
+def test_the_boundary_gradient_term_is_the_boundary_fluxes_derivative_through_the_gradients(flux):
+    """B v (the Krylov operator's boundary term) against central differences of the boundary faces'
+    fluxes in which only the gradients move: each boundary-owning cell's gradient refitted from
+    W + e dW (dW = dW/dq v, boundary values held), its primitives and limiter held."""
+    zones = [meshes.FARFIELD, meshes.OUTLET, meshes.SLIP, meshes.WALL, meshes.INLET, meshes.STATE]
+    c = cases.fixture_case("poly", flux=flux, kinds_by_zone=zones)
+    m, sc = c.mesh, c.scheme
+    n, ni, nb = m.n_cells, m.n_interior, m.n_boundary
+    ora = orc.Oracle(m, sc, "fp64", c.bc_kind, c.bc, c.source)
+    parts = ora.residual_and_jacobian(c.u)
+    arr = ora.arrays()
+    w, g, phi = arr["prim"].reshape(5, n), arr["grad"].reshape(15, n), arr["phi"].reshape(5, n)
+    v = np.random.default_rng(5).normal(size=(5, n)) * np.array([1.0, 0.3, 0.3, 0.3, 2.0])[:, None]
+    # B v: the term alone, through a solve-free call (y starts at zero).
+    got = np.zeros(5 * n)
+    lib = orc.lib()
+    import ctypes
+    lib.oracle_boundary_term.argtypes = [ctypes.c_void_p] * 5
+    vv = np.ascontiguousarray(v.reshape(-1))
+    lib.oracle_boundary_term(ora.h, vv.ctypes.data, parts["dwdu"].ctypes.data, parts["bnd_jac"].ctypes.data,
+                             got.ctypes.data)
+    got = got.reshape(5, n)
+    dwdu = parts["dwdu"].reshape(5, 5, n)
+    dw = np.einsum("ijc,jc->ic", dwdu, v)
+    so, st = np.asarray(m.stencil_offsets), np.asarray(m.stencil)
+    sw = np.asarray(m.stencil_weights).reshape(3, -1)
+    nrm, area = np.asarray(m.face_normal).reshape(3, -1), np.asarray(m.face_area)
+    rl, mom = np.asarray(m.r_left).reshape(3, -1), np.asarray(m.face_moment).reshape(9, -1)
+    pr, kind = p_ref(sc), FLUXES[flux]
+    bc = np.asarray(c.bc).reshape(5, nb)
+
+    def grad_of(cell, eps):
+        wc = w[:, cell] + eps * dw[:, cell]
+        out = np.zeros(15)
+        for k in range(so[cell], so[cell + 1]):
+            e = st[k]
+            # a boundary entry -(f + 1) holds face f's boundary state
+            wn = (w[:, e] + eps * dw[:, e]) if e >= 0 else arr["prim_b"].reshape(5, nb)[:, -e - 1 - ni]
+            for i in range(5):
+                out[3 * i : 3 * i + 3] += (wn[i] - wc[i]) * sw[:, k]
+        return out
+
+    def boundary_fluxes(cell, gr):
+        out = np.zeros(5)
+        for k in range(m.cell_face_offsets[cell], m.cell_face_offsets[cell + 1]):
+            f = m.cell_faces[k]
+            if f < ni:
+                continue
+            b = f - ni
+            wf = reconstruct(w[:, cell], gr, phi[:, cell], rl[:, f], pr)
+            kd = c.bc_kind[b]
+            if kd in (meshes.SLIP, meshes.WALL):
+                fx = area[f] * np.array([0.0, *(wf[4] * nrm[:, f]), 0.0])
+                M = mom[:, f].reshape(3, 3)
+                fx[1:4] += phi[4, cell] * (M.T @ gr[12:15])
+            else:
+                wb = orc.boundary_value(wf, kd, bc[:, b], nrm[:, f], sc.gamma, sc.gas_constant, pr)
+                fx = area[f] * orc.flux(kind, wf, wb, nrm[:, f], sc.gamma, sc.mach_ref, pr)
+            out += fx
+        return out
+
+    owners = sorted(set(np.asarray(m.owner[ni:]).tolist()))
+    checked = 0
+    for cell in owners[::3]:
+        h = 1e-6
+        fd = (boundary_fluxes(cell, grad_of(cell, h)) - boundary_fluxes(cell, grad_of(cell, -h))) / (2 * h)
+        scale = np.abs(fd).max() + 1e-12
+        assert np.abs(got[:, cell] - fd).max() <= 1e-6 * scale, cell
+        checked += int(np.abs(fd).max() > 1e-8)
+    assert checked > 5
+    # Cells that own no boundary face get nothing.
+    inner = np.setdiff1d(np.arange(n), owners)
+    assert np.all(got[:, inner] == 0.0)
