@@ -12,12 +12,22 @@
 // 57(4)). The power-of-two scaling after it is exact: a sum of doubles is a multiple of 2^-1074, so
 // a result below 2^-1021 has fewer than 54 significant bits and was never rounded, and above it the
 // scaled value is normal.
+//
+// exact_value and its helpers are host and device functions when compiled as HIP: the device
+// rounds words it summed itself (the implicit solve's inner products) with the same code, so both
+// sides give the same double.
 #pragma once
 
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <limits>
+
+#if defined(__HIP__)
+#define MXE_HOST_DEVICE __host__ __device__
+#else
+#define MXE_HOST_DEVICE
+#endif
 
 namespace mxe {
 
@@ -34,7 +44,7 @@ struct Magnitude {
   bool negative;
 };
 
-inline Magnitude resolve(const long long* w) {
+MXE_HOST_DEVICE inline Magnitude resolve(const long long* w) {
   Magnitude m{};
   __int128 carry = 0;
   for (int i = 0; i < kExactLimbs; ++i) {
@@ -59,12 +69,12 @@ inline Magnitude resolve(const long long* w) {
 }
 
 // Bit `b` (0 = the lowest bit of digit 0) of the digits.
-inline unsigned bit_of(const Magnitude& m, int b) {
+MXE_HOST_DEVICE inline unsigned bit_of(const Magnitude& m, int b) {
   return b < 0 ? 0u : (m.digit[b >> 5] >> (b & 31)) & 1u;
 }
 
 // Whether any bit below position `b` is set.
-inline bool any_below(const Magnitude& m, int b) {
+MXE_HOST_DEVICE inline bool any_below(const Magnitude& m, int b) {
   if (b <= 0) return false;
   const int whole = b >> 5;
   for (int i = 0; i < whole; ++i)
@@ -76,7 +86,7 @@ inline bool any_below(const Magnitude& m, int b) {
 }  // namespace exact_detail
 
 // The value of exact words, rounded once to the nearest double (ties to even).
-inline double exact_value(const long long* in) {
+MXE_HOST_DEVICE inline double exact_value(const long long* in) {
   using namespace exact_detail;
   constexpr double inf = std::numeric_limits<double>::infinity();
   const long long pos_inf = in[kExactLimbs], neg_inf = in[kExactLimbs + 1], nans = in[kExactLimbs + 2];

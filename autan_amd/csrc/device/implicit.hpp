@@ -22,9 +22,12 @@
 // the upper (row owner) blocks of the interior faces first, then the lower (row neighbour) ones.
 #pragma once
 
+#include "exact.hpp"
 #include "jacobian.hpp"
 
 namespace mxe {
+
+static_assert(kWords == kExactWords, "the device's exact-sum words are exact_value's");
 
 // out += sign * (j d) for 5 x 5 row-major blocks, each entry's product summed over k from zero.
 __device__ __forceinline__ void impl_add_product(double* out, const double* j, const double* dm, double sign) {
@@ -449,23 +452,39 @@ __global__ __launch_bounds__(kBlock) void k_impl_scale(double* __restrict__ r, l
   if (i < m) r[i] = mode == 0 ? r[i] / s : r[i] * s;
 }
 
-// z -= beta zj and w -= beta wj
+// *out = the exact sum's value (exact_value, the host's rounding), divided by *den when given: the
+// GCR's scalars stay on the device, so its vector updates need no host round trip.
+__global__ void k_impl_scalar(const long long* __restrict__ words, const double* __restrict__ den,
+                              double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double v = exact_value(words);
+  *out = den != nullptr ? v / *den : v;
+}
+
+// z -= beta zj and w -= beta wj (beta on the device)
 __global__ __launch_bounds__(kBlock) void k_impl_orthogonalize(double* __restrict__ z, double* __restrict__ w,
                                                                const double* __restrict__ zj,
                                                                const double* __restrict__ wj, long long m,
-                                                               double beta) {
+                                                               const double* __restrict__ beta_at) {
   const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= m) return;
+  const double beta = *beta_at;
   z[i] -= beta * zj[i];
   w[i] -= beta * wj[i];
 }
 
-// x += sa z (sa = scale * alpha) and r -= alpha w
+// x += sa z (sa = scale * alpha) and r -= alpha w, with scale the residual's largest magnitude (its
+// bits) and alpha on the device; nothing when the direction's (w, w) is not positive, where the
+// GCR stops before its update.
 __global__ __launch_bounds__(kBlock) void k_impl_advance(double* __restrict__ x, double* __restrict__ r,
                                                          const double* __restrict__ z, const double* __restrict__ w,
-                                                         long long m, double sa, double alpha) {
+                                                         long long m, const unsigned long long* __restrict__ scale_bits,
+                                                         const double* __restrict__ alpha_at,
+                                                         const double* __restrict__ ww_at) {
   const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= m) return;
+  if (i >= m || !(*ww_at > 0.0)) return;
+  const double alpha = *alpha_at;
+  const double sa = __longlong_as_double(static_cast<long long>(*scale_bits)) * alpha;
   x[i] += sa * z[i];
   r[i] -= alpha * w[i];
 }

@@ -248,6 +248,7 @@ class Stepper final : public StepperBase {
     (void)hipStreamSynchronize(stream_);
     drop_graphs();
     if (host_small_ != nullptr) (void)hipHostFree(host_small_);
+    if (host_scalars_ != nullptr) (void)hipHostFree(host_scalars_);
     if (ev_in_ != nullptr) (void)hipEventDestroy(ev_in_);
     if (ev_out_ != nullptr) (void)hipEventDestroy(ev_out_);
     if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -494,20 +495,11 @@ class Stepper final : public StepperBase {
     words_ = Buffer<long long>(std::size_t(kImplDots) * kWords);
     bits_ = Buffer<unsigned long long>(2);
     flag_ = Buffer<int>(1);
+    scalars_ = Buffer<double>(kImplScalars);
+    check(hipHostMalloc(reinterpret_cast<void**>(&host_scalars_), kImplScalars * sizeof(double), hipHostMallocDefault),
+          "hipHostMalloc");
     check(hipStreamSynchronize(stream_), "implicit_setup");
     implicit_ready_ = true;
-  }
-
-  // sum u v, exact, rounded once (a fresh word set per call of one solve).
-  double exact_dot(const double* u, const double* v, long long m) {
-    if (dot_next_ >= kImplDots) throw std::logic_error("implicit_solve: more inner products than planned");
-    long long* w = words_.get() + std::size_t(dot_next_++) * kWords;
-    const int g = std::max(1, std::min(static_cast<int>((m + kBlock - 1) / kBlock), 1024));
-    launch("impl_dot", [&] { k_impl_dot<<<g, kBlock, 0, stream_>>>(u, v, m, w); });
-    long long host[kWords];
-    check(hipMemcpyAsync(host, w, sizeof host, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
-    check(hipStreamSynchronize(stream_), "implicit_solve");
-    return exact_value(host);
   }
 
   // kImplSweeps symmetric sweeps on A z = b from z = 0: the colours in increasing, then decreasing
@@ -531,8 +523,13 @@ class Stepper final : public StepperBase {
     }
   }
 
-  // Gcr::solve (solvers/src/gcr.cpp) on kr_ = -R into du_ = 0: returns the residual drop.
+  // Gcr::solve (solvers/src/gcr.cpp) on kr_ = -R into du_ = 0: returns the residual drop. The
+  // scalars (inner products rounded by exact_value, the Gram-Schmidt and step coefficients) are
+  // computed and used on the device; the host reads (w, w) and (r, r) once per direction for the
+  // stopping rules, and takes every decision the host loop takes, on the same values.
   double gcr(long long m, int gm, int& cycles) {
+    double* sc = scalars_.get();  // [0, D) (w_k, w_k), then beta, alpha, (r, r), (r_0, r_0)
+    constexpr int kBeta = kImplDirections, kAlpha = kBeta + 1, kRr = kAlpha + 1, kR0 = kRr + 1;
     unsigned long long mbits = 0;
     launch("impl_maxabs", [&] {
       k_impl_maxabs<<<std::max(1, std::min(gm, 1024)), kBlock, 0, stream_>>>(kr_.get(), m, bits_.get() + 1);
@@ -545,9 +542,8 @@ class Stepper final : public StepperBase {
     if (!std::isfinite(scale)) return std::numeric_limits<double>::quiet_NaN();
     if (scale == 0.0) return 0.0;
     launch("impl_scale", [&] { k_impl_scale<<<gm, kBlock, 0, stream_>>>(kr_.get(), m, scale, 0); });
-    const double r0 = std::sqrt(exact_dot(kr_.get(), kr_.get(), m));
-    double rr = r0 * r0;
-    double ww_of[kImplDirections];
+    device_dot(kr_.get(), kr_.get(), m, nullptr, sc + kR0);
+    double r0 = 0.0, rr = 0.0;
     for (int k = 0; k < kImplDirections; ++k) {
       double* z = zs_.get() + std::size_t(k) * std::size_t(m);
       double* w = ws_.get() + std::size_t(k) * std::size_t(m);
@@ -567,23 +563,44 @@ class Stepper final : public StepperBase {
       for (int j = 0; j < k; ++j) {
         const double* zj = zs_.get() + std::size_t(j) * std::size_t(m);
         const double* wj = ws_.get() + std::size_t(j) * std::size_t(m);
-        const double beta = exact_dot(w, wj, m) / ww_of[j];
-        launch("impl_orthogonalize", [&] { k_impl_orthogonalize<<<gm, kBlock, 0, stream_>>>(z, w, zj, wj, m, beta); });
+        device_dot(w, wj, m, sc + j, sc + kBeta);  // beta = (w, w_j) / (w_j, w_j)
+        launch("impl_orthogonalize", [&] {
+          k_impl_orthogonalize<<<gm, kBlock, 0, stream_>>>(z, w, zj, wj, m, sc + kBeta);
+        });
       }
-      const double ww = exact_dot(w, w, m);
-      ww_of[k] = ww;
+      device_dot(w, w, m, nullptr, sc + k);
+      device_dot(kr_.get(), w, m, sc + k, sc + kAlpha);  // alpha = (r, w) / (w, w)
+      launch("impl_advance", [&] {
+        k_impl_advance<<<gm, kBlock, 0, stream_>>>(du_.get(), kr_.get(), z, w, m, bits_.get() + 1, sc + kAlpha, sc + k);
+      });
+      device_dot(kr_.get(), kr_.get(), m, nullptr, sc + kRr);
+      check(hipMemcpyAsync(host_scalars_, sc, (kR0 + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_),
+            "hipMemcpyAsync");
+      check(hipStreamSynchronize(stream_), "implicit_solve");
+      if (k == 0) {
+        r0 = std::sqrt(host_scalars_[kR0]);
+        rr = r0 * r0;
+      }
+      const double ww = host_scalars_[k];
       cycles = k + 1;
       if (!(ww > 0.0)) {
         if (!std::isfinite(ww)) rr = ww;
         break;
       }
-      const double alpha = exact_dot(kr_.get(), w, m) / ww;
-      const double sa = scale * alpha;
-      launch("impl_advance", [&] { k_impl_advance<<<gm, kBlock, 0, stream_>>>(du_.get(), kr_.get(), z, w, m, sa, alpha); });
-      rr = exact_dot(kr_.get(), kr_.get(), m);
+      rr = host_scalars_[kRr];
       if (std::sqrt(rr) <= kImplTolerance * r0) break;
     }
     return std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
+  }
+
+  // sum u v as an exact sum in a fresh word set of the solve, rounded on the device into *out
+  // (divided by *den when given).
+  void device_dot(const double* u, const double* v, long long m, const double* den, double* out) {
+    if (dot_next_ >= kImplDots) throw std::logic_error("implicit_solve: more inner products than planned");
+    long long* w = words_.get() + std::size_t(dot_next_++) * kWords;
+    const int g = std::max(1, std::min(static_cast<int>((m + kBlock - 1) / kBlock), 1024));
+    launch("impl_dot", [&] { k_impl_dot<<<g, kBlock, 0, stream_>>>(u, v, m, w); });
+    launch("impl_scalar", [&] { k_impl_scalar<<<1, kWave, 0, stream_>>>(w, den, out); });
   }
 
   bool implicit_ready_ = false, have_jacobian_ = false, jac_second_ = false;
@@ -596,6 +613,10 @@ class Stepper final : public StepperBase {
   Buffer<double> jd_, jf_, jb_, diag_, inv_, offd_, kr_, du_, zs_, ws_;
   Buffer<long long> words_;
   Buffer<unsigned long long> bits_;
+  // The GCR's scalars on the device and the pinned host copy of the ones the host reads.
+  static constexpr int kImplScalars = kImplDirections + 4;
+  Buffer<double> scalars_;
+  double* host_scalars_ = nullptr;
 
  public:
   template <int FLUX>

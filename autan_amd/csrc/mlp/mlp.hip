@@ -278,23 +278,26 @@ struct LayerTable {
   int nl;
 };
 
-// The whole forward of 64 samples per workgroup (inference): the normalised inputs, then every
-// layer's activations, stay in one 64 x (hidden + 8) BF16 LDS tile; the weights stream from L2.
-// Each of the four waves computes 16 CT output columns of the 64 rows as 4 x CT matrix-core tiles,
-// K in steps of 32 in the order of k_gemm, and the epilogues are k_gemm's (bias + SiLU, then bias
-// and the output scaling of k_out): the outputs are the layer-by-layer path's bits, without its
-// HBM round trip of every layer's activations.
-template <int CT>
+// The whole forward of 16 RT samples per workgroup (inference): the normalised inputs, then every
+// layer's activations, stay in one 16 RT x (hidden + 8) BF16 LDS tile (dynamic shared memory); the
+// weights stream from L2, each weight column read once per workgroup and used for all 16 RT rows,
+// so the L2 bytes per product fall as RT grows. Each of the four waves computes 16 CT output
+// columns of the tile's rows as RT x CT matrix-core tiles, K in steps of 32 in the order of
+// k_gemm, and the epilogues are k_gemm's (bias + SiLU, then bias and the output scaling of
+// k_out): the outputs are the layer-by-layer path's bits whatever RT, without its HBM round trip
+// of every layer's activations.
+template <int CT, int RT>
 __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x, int n, int d_in, int d_out,
                                                     const float* __restrict__ norm, const bf16* __restrict__ wb,
                                                     const float* __restrict__ params, LayerTable L,
                                                     float* __restrict__ y) {
-  constexpr int H = 64 * CT, ld = H + kPad, kIn = 32, kOut = 16;
-  __shared__ __align__(16) bf16 act[64 * ld];
-  const int m0 = blockIdx.x * 64;
+  constexpr int H = 64 * CT, ld = H + kPad, kIn = 32, M = 16 * RT;
+  extern __shared__ __align__(16) unsigned char infer_smem[];
+  bf16* act = reinterpret_cast<bf16*>(infer_smem);
+  const long long m0 = static_cast<long long>(blockIdx.x) * M;
   const int t = threadIdx.x, lane = t % 64, wave = t / 64;
   const int row = lane % 16, kq = 8 * (lane / 16);
-  for (int i = t; i < 64 * kIn; i += kThreads) {
+  for (int i = t; i < M * kIn; i += kThreads) {
     const int r = i / kIn, c = i % kIn;
     const long long m = m0 + r;
     act[r * ld + c] = static_cast<bf16>(m < n && c < d_in ? (x[m * d_in + c] - norm[c]) / norm[32 + c] : 0.0f);
@@ -305,9 +308,9 @@ __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x,
     const float* bias = params + L.b[l];
     __syncthreads();  // the layer's input is complete
     if (l < L.nl - 1) {
-      f32x4 acc[4][CT];
+      f32x4 acc[RT][CT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < RT; ++i) {
 #pragma unroll
         for (int j = 0; j < CT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x,
 #pragma unroll
       for (int j = 0; j < CT; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wrow[j]);
       for (int k0 = 0; k0 < K; k0 += kTileK) {
-        bf16x8 a[4], b[CT];
+        bf16x8 b[CT];
 #pragma unroll
         for (int j = 0; j < CT; ++j) b[j] = b_next[j];
         if (k0 + kTileK < K) {
@@ -328,16 +331,15 @@ __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x,
           for (int j = 0; j < CT; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + k0 + kTileK);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&act[(16 * i + row) * ld + k0 + kq]);
+        for (int i = 0; i < RT; ++i) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[(16 * i + row) * ld + k0 + kq]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-#pragma unroll
-          for (int j = 0; j < CT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < CT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
         }
       }
       __syncthreads();  // every wave has read the layer's input: the tile takes its output
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < RT; ++i) {
 #pragma unroll
         for (int j = 0; j < CT; ++j) {
           const int nn = nb + 16 * j + lane % 16;
@@ -350,24 +352,51 @@ __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x,
         }
       }
     } else {
-      // The padded output layer (16 columns): wave w takes rows 16 w .. 16 w + 15.
-      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-      for (int k0 = 0; k0 < K; k0 += kTileK) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[(16 * wave + row) * ld + k0 + kq]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + static_cast<long long>(row) * K + k0 + kq);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-      }
-      const int nn = lane % 16;
+      // The padded output layer (16 columns): wave w takes the row tiles w, w + 4, ...
+      for (int i = wave; i < RT; i += kThreads / 64) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < K; k0 += kTileK) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[(16 * i + row) * ld + k0 + kq]);
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + static_cast<long long>(row) * K + k0 + kq);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int nn = lane % 16;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long long m = m0 + 16 * wave + 4 * (lane / 16) + r;
-        if (m < n && nn < d_out) {
-          const float z = acc[r] + bias[nn];
-          y[m * d_out + nn] = z * norm[96 + nn] + norm[64 + nn];
+        for (int r = 0; r < 4; ++r) {
+          const long long m = m0 + 16 * i + 4 * (lane / 16) + r;
+          if (m < n && nn < d_out) {
+            const float z = acc[r] + bias[nn];
+            y[m * d_out + nn] = z * norm[96 + nn] + norm[64 + nn];
+          }
         }
       }
-      (void)kOut;
     }
+  }
+}
+
+// LDS bytes of k_infer<CT, RT>'s tile.
+constexpr std::size_t infer_lds(int ct, int rt) { return std::size_t(16 * rt) * std::size_t(64 * ct + kPad) * 2; }
+
+// Launches k_infer<CT, RT> for n samples, raising its dynamic LDS limit once per process.
+template <int CT, int RT>
+void launch_infer(const float* x, int n, int d_in, int d_out, const float* norm, const bf16* wb, const float* params,
+                  const LayerTable& L, float* y, hipStream_t s) {
+  static const hipError_t raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<CT, RT>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       static_cast<int>(infer_lds(CT, RT)));
+  check(raised, "hipFuncSetAttribute(k_infer)");
+  const int g = static_cast<int>((static_cast<long long>(n) + 16 * RT - 1) / (16 * RT));
+  k_infer<CT, RT><<<g, kThreads, infer_lds(CT, RT), s>>>(x, n, d_in, d_out, norm, wb, params, L, y);
+}
+
+template <int CT>
+void launch_infer_rows(int rows, const float* x, int n, int d_in, int d_out, const float* norm, const bf16* wb,
+                       const float* params, const LayerTable& L, float* y, hipStream_t s) {
+  switch (rows) {
+    case 64: launch_infer<CT, 4>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    case 96: launch_infer<CT, 6>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    case 128: launch_infer<CT, 8>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    default: throw std::invalid_argument("inference tile: 64, 96 or 128 samples per workgroup");
   }
 }
 
@@ -638,12 +667,13 @@ class Net {
         L.w[l] = woff_[std::size_t(l)];
         L.b[l] = boff_[std::size_t(l)];
       }
-      const int g = cdiv(n, 64);
+      const float* nm = norm_.get();
+      const float* pr = params_.get();
       switch (h / 64) {
-        case 1: k_infer<1><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
-        case 2: k_infer<2><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
-        case 3: k_infer<3><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
-        default: k_infer<4><<<g, kThreads, 0, s_>>>(x, n, d_in_, d_out_, norm_.get(), wb_.get(), params_.get(), L, y); break;
+        case 1: launch_infer_rows<1>(infer_rows_, x, n, d_in_, d_out_, nm, wb_.get(), pr, L, y, s_); break;
+        case 2: launch_infer_rows<2>(infer_rows_, x, n, d_in_, d_out_, nm, wb_.get(), pr, L, y, s_); break;
+        case 3: launch_infer_rows<3>(infer_rows_, x, n, d_in_, d_out_, nm, wb_.get(), pr, L, y, s_); break;
+        default: launch_infer_rows<4>(infer_rows_, x, n, d_in_, d_out_, nm, wb_.get(), pr, L, y, s_); break;
       }
       check(hipGetLastError(), "k_infer");
       return;
@@ -655,6 +685,11 @@ class Net {
   }
 
   void set_fused(bool on) { fused_ = on; }
+  void set_infer_rows(int rows) {
+    if (rows != 64 && rows != 96 && rows != 128)
+      throw std::invalid_argument("inference tile: 64, 96 or 128 samples per workgroup");
+    infer_rows_ = rows;
+  }
 
   // Milliseconds of one inference of n samples (device-resident inputs and outputs, HIP events).
   double time_forward(int n, int repeats) {
@@ -781,6 +816,7 @@ class Net {
   std::vector<Buf<float>> z_;
   int cap_ = 0;
   bool fused_ = true;
+  int infer_rows_ = 96;  // the fastest tile on an MI355X (2 waves per SIMD)
   Buf<float> data_x_, data_t_;
   long long n_data_ = 0;
 };
@@ -883,6 +919,10 @@ int mxmlp_time_forward(mxmlp* h, int32_t n, int32_t repeats, double* ms, char* e
 
 int mxmlp_set_fused(mxmlp* h, int32_t on, char* err, size_t err_len) {
   return mxm::guarded(err, err_len, [&] { h->net->set_fused(on != 0); });
+}
+
+int mxmlp_set_infer_rows(mxmlp* h, int32_t rows, char* err, size_t err_len) {
+  return mxm::guarded(err, err_len, [&] { h->net->set_infer_rows(rows); });
 }
 
 }  // extern "C"

@@ -131,6 +131,10 @@ class MLP:
         """Inference by the fused all-layer kernel (default) or the layer-by-layer products; same bits."""
         self._call(self.lib.mxmlp_set_fused, ctypes.c_int32(1 if on else 0))
 
+    def set_infer_rows(self, rows: int) -> None:
+        """Samples per workgroup of the fused inference (64, 96 or 128); same bits whatever the tile."""
+        self._call(self.lib.mxmlp_set_infer_rows, ctypes.c_int32(rows))
+
     def time_forward(self, n: int, repeats: int = 10) -> float:
         """Milliseconds of one inference of n device-resident samples."""
         ms = ctypes.c_double()

@@ -126,42 +126,93 @@ def run(n_cases: int = 64, seed: int = 1, max_iterations: int = 20000, residual_
     }
 
 
-def run_implicit(n_cases: int = 64, seed: int = 1, max_iterations: int = 500, residual_orders: float = 8.0,
-                 policy: str = "fp32+acc64", flux: str = "hllc", cells=(2e4, 5e5), log=print) -> dict:
-    """The sweep by the steady implicit path (EulerPhysics::steady_step: pseudo-transient
-    continuation, GCR and block Gauss-Seidel on the device), one case per stepper in turn, each to
-    a 1e-8 relative residual or the fp32+acc64 round-off plateau. Wall time counts the solves; the
-    meshes' set-up and their upload are excluded."""
+def solve_implicit_cases(specs: list[Spec], policy: str = "fp64", flux: str = "hllc", states=None, references=None,
+                         max_iterations: int = 500, residual_orders: float = 8.0, start_order: str = "first",
+                         workers: int = 8, chunk: int = 64, finish=None, log=print) -> tuple[list[dict], float]:
+    """Each case by the steady implicit path (EulerPhysics::steady_step: pseudo-transient continuation,
+    GCR and block Gauss-Seidel on the device) on a stepper and stream of its own, `workers` cases
+    at a time (host threads: every run is one library call, mxeuler_implicit_steady, which waits
+    only on its own stream). Per chunk of `chunk` cases the meshes are built and uploaded first,
+    then solved together; the wall time counts the solve phases only (mesh set-up and upload
+    excluded, as for one stepper). `finish(k, case, stepper, result)` sees each case in order, in
+    the calling thread, before its stepper closes. Returns per case its result, cell count and own
+    solve seconds, and the solve phases' wall seconds."""
+    from concurrent.futures import ThreadPoolExecutor, wait
+
     from autan_amd import steady
     from autan_amd.stepper import DeviceStepper
 
-    sp = specs(n_cases, seed, cells)
-    status = ["converged", "budget", "diverged"]
-    rows = []
-    wall = 0.0
-    cells_iter = 0
-    for k, s in enumerate(sp):
-        c = build(s, policy, flux)
+    def setup(k):
+        c = build(specs[k], policy, flux)
         dev = DeviceStepper(c.mesh, c.scheme, policy, c.bc_kind, c.bc)
-        dev.upload(c.u)
+        dev.upload(c.u if states is None else states[k])
+        return c, dev
+
+    def solve(k, c, dev):
+        ref = None if references is None else np.asarray(references[k])
         t0 = time.perf_counter()
         r = steady.solve_implicit(dev, max_iterations=max_iterations, residual_orders=residual_orders,
-                                  scales=residual_scales(c))
+                                  scales=residual_scales(c), reference=ref, start_order=start_order)
         sec = time.perf_counter() - t0
-        dev.close()
-        wall += sec
-        cells_iter += c.mesh.n_cells * r.iterations
+        log(f"implicit: case {k} ({c.mesh.n_cells} cells) {('converged', 'budget', 'diverged')[r.status]} in "
+            f"{r.iterations} iterations, {sec:.2f} s")
+        return r, sec
+
+    out: list[dict] = []
+    wall = 0.0
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        for b0 in range(0, len(specs), chunk):
+            ks = range(b0, min(b0 + chunk, len(specs)))
+            made = [pool.submit(setup, k) for k in ks]
+            built = []
+            try:
+                for f in made:
+                    built.append(f.result())
+                t0 = time.perf_counter()
+                runs = [pool.submit(solve, k, c, dev) for k, (c, dev) in zip(ks, built)]
+                done = [f.result() for f in runs]
+                sec = time.perf_counter() - t0
+                wall += sec
+                for k, (c, dev), (r, own) in zip(ks, built, done):
+                    if finish is not None:
+                        finish(k, c, dev, r)
+                    out.append({"result": r, "cells": c.mesh.n_cells, "seconds": own})
+            finally:
+                wait(made)
+                for f in made:
+                    if f.done() and f.exception() is None:
+                        f.result()[1].close()
+            log(f"implicit: cases {b0}..{ks[-1]} solved in {sec:.1f} s ({workers} at a time; {wall:.1f} s so far)")
+    return out, wall
+
+
+def run_implicit(n_cases: int = 64, seed: int = 1, max_iterations: int = 500, residual_orders: float = 8.0,
+                 policy: str = "fp32+acc64", flux: str = "hllc", cells=(2e4, 5e5), workers: int = 8,
+                 log=print) -> dict:
+    """The sweep by the steady implicit path, every case from the freestream to a 1e-8 relative
+    residual or the fp32+acc64 round-off plateau (solve_implicit_cases, `workers` cases at a time).
+    Wall time counts the solves; the meshes' set-up and their upload are excluded."""
+    sp = specs(n_cases, seed, cells)
+    status = ["converged", "budget", "diverged"]
+    solved, wall = solve_implicit_cases(sp, policy, flux, max_iterations=max_iterations,
+                                        residual_orders=residual_orders, workers=workers, chunk=len(sp), log=log)
+    rows = []
+    cells_iter = 0
+    for k, (s, o) in enumerate(zip(sp, solved)):
+        r = o["result"]
+        cells_iter += o["cells"] * r.iterations
         drop = r.reference / r.residuals
         rows.append({
-            "cells": c.mesh.n_cells, "h": s.h, "mach": s.mach, "alpha": s.alpha, "iterations": r.iterations,
+            "cells": o["cells"], "h": s.h, "mach": s.mach, "alpha": s.alpha, "iterations": r.iterations,
             "status": status[r.status], "first_order_steps": r.first_order_steps,
             "first_order_end": r.first_order_end, "plateau_mask": r.plateau_mask,
-            "diverged_cell": r.diverged_cell, "seconds": round(sec, 3),
+            "diverged_cell": r.diverged_cell, "seconds": round(o["seconds"], 3),
             "orders": [round(math.log10(v), 2) if v > 0 and math.isfinite(v) else None for v in drop],
             "max_cfl": float(r.cfl.max()) if r.iterations else None,
             "rejected_steps": int(np.sum(r.alpha == 0.0)),
         })
-        log(f"sweep (implicit): case {k} {c.mesh.n_cells} cells M {s.mach:.3f} h {s.h:.4f} alpha {s.alpha:+.2f}: "
-            f"{status[r.status]} in {r.iterations} iterations ({r.first_order_steps} first order), {sec:.2f} s")
+        log(f"sweep (implicit): case {k} {o['cells']} cells M {s.mach:.3f} h {s.h:.4f} alpha {s.alpha:+.2f}: "
+            f"{status[r.status]} in {r.iterations} iterations ({r.first_order_steps} first order), "
+            f"{o['seconds']:.2f} s")
     return {"cases": rows, "wall_s": wall, "cells_iter_per_s": cells_iter / wall if wall > 0 else None,
-            "total_cells": sum(r["cells"] for r in rows)}
+            "total_cells": sum(r["cells"] for r in rows), "workers": workers}

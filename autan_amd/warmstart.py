@@ -126,35 +126,24 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
 
 
 def solve_implicit(specs: list[sweep.Spec], states=None, references=None, max_iterations=500, residual_orders=8.0,
-                   policy="fp64", flux="hllc", sample=None, start_order="first", log=print, **_):
-    """As solve, by the steady implicit path (EulerPhysics::steady_step on the device), one case per
-    stepper in turn. Wall time counts the solves (mesh set-up and upload excluded)."""
-    from autan_amd import steady
-    from autan_amd.stepper import DeviceStepper
+                   policy="fp64", flux="hllc", sample=None, start_order="first", workers=8, log=print, **_):
+    """As solve, by the steady implicit path (EulerPhysics::steady_step on the device), one stepper
+    per case, `workers` cases at a time (sweep.solve_implicit_cases). Wall time counts the solves
+    (mesh set-up and upload excluded)."""
+    extra: dict[int, tuple] = {}
 
-    out = []
-    total = 0.0
-    for k, spec in enumerate(specs):
-        c = sweep.build(spec, policy, flux)
-        u0 = c.u if states is None else states[k]
-        dev = DeviceStepper(c.mesh, c.scheme, policy, c.bc_kind, c.bc)
-        dev.upload(u0)
-        ref = None if references is None else np.asarray(references[k])
-        t0 = time.perf_counter()
-        r = steady.solve_implicit(dev, max_iterations=max_iterations, residual_orders=residual_orders,
-                                  scales=sweep.residual_scales(c), reference=ref, start_order=start_order)
-        sec = time.perf_counter() - t0
-        total += sec
+    def finish(k, c, dev, r):
         u = dev.download()
         faces = np.nonzero(c.mesh.face_zone == 2)[0] + c.mesh.n_interior
         loads = dev.loads(faces, center=(1.5, 0.0, 0.5), p_inf=1.0 / GAMMA)
-        area = float(np.sum(c.mesh.face_area[faces]))
-        dev.close()
+        extra[k] = (loads, float(np.sum(c.mesh.face_area[faces])))
         if sample is not None:
-            sample(k, c.mesh, spec, u, r)
-        out.append({"result": r, "cells": c.mesh.n_cells, "loads": loads, "wall_area": area, "seconds": sec})
-        if k % 16 == 15 or k == len(specs) - 1:
-            log(f"warm start (implicit): solved cases ..{k} ({total:.1f} s so far)")
+            sample(k, c.mesh, specs[k], u, r)
+
+    solved, total = sweep.solve_implicit_cases(specs, policy, flux, states=states, references=references,
+                                               max_iterations=max_iterations, residual_orders=residual_orders,
+                                               start_order=start_order, workers=workers, finish=finish, log=log)
+    out = [{**o, "loads": extra[k][0], "wall_area": extra[k][1]} for k, o in enumerate(solved)]
     return out, total
 
 
@@ -234,7 +223,7 @@ def write_initial_state(path, net, spec: sweep.Spec, policy="fp32+acc64", flux="
 
 def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_case: int = 4096, batch: int = 4096,
                   lr: tuple[float, float] = (1e-3, 1e-5), seed: int = 0, log=print, method: str = "explicit",
-                  policy: str = "fp32+acc64"):
+                  policy: str = "fp32+acc64", workers: int = 8):
     """The network trained on the training cases' states: each case solved from the freestream
     (explicit: `iterations` steady iterations of the batch stepper; implicit: the steady implicit
     path to a 1e-8 relative residual or its round-off plateau, at most `iterations`), cases that
@@ -244,7 +233,8 @@ def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_cas
 
     collect = TrainingSet(per_case=per_case, seed=seed)
     solver = solve_implicit if method == "implicit" else solve
-    _, t_gen = solver(specs, max_iterations=iterations, residual_orders=8.0, sample=collect, log=log, policy=policy)
+    kw = {"workers": workers} if method == "implicit" else {}
+    _, t_gen = solver(specs, max_iterations=iterations, residual_orders=8.0, sample=collect, log=log, policy=policy, **kw)
     x, t = collect.arrays()
     log(f"training set: {x.shape[0]} cells of {len(specs) - collect.skipped} cases ({collect.skipped} diverged, "
         f"left out; {t_gen:.1f} s of solves)")
@@ -266,17 +256,18 @@ def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_cas
 
 
 def compare(net, specs: list[sweep.Spec], orders: float, max_iterations: int, log=print, method: str = "explicit",
-            policy: str = "fp32+acc64") -> dict:
+            policy: str = "fp32+acc64", workers: int = 8) -> dict:
     """The held-out cases solved from the freestream (first-order start) and from the network's state
     (second order from the first step), the latter both against its own first residual and against
     the freestream run's (the absolute threshold a warm start from another run is held to)."""
     solver = solve_implicit if method == "implicit" else solve
-    free, t_free = solver(specs, max_iterations=max_iterations, residual_orders=orders, log=log, policy=policy)
+    kw = {"workers": workers} if method == "implicit" else {}
+    free, t_free = solver(specs, max_iterations=max_iterations, residual_orders=orders, log=log, policy=policy, **kw)
     warm_u, t_infer = warm_states(net, specs, policy=policy)
     warm, t_warm = solver(specs, states=warm_u, max_iterations=max_iterations, residual_orders=orders,
-                          start_order="second", log=log, policy=policy)
+                          start_order="second", log=log, policy=policy, **kw)
     refs = [f["result"].reference for f in free]
     warm_abs, t_warm_abs = solver(specs, states=warm_u, references=refs, max_iterations=max_iterations,
-                                  residual_orders=orders, start_order="second", log=log, policy=policy)
+                                  residual_orders=orders, start_order="second", log=log, policy=policy, **kw)
     return {"free": free, "warm": warm, "warm_abs": warm_abs, "free_s": t_free, "warm_s": t_warm,
             "warm_abs_s": t_warm_abs, "inference_s": t_infer}

@@ -241,6 +241,27 @@ def bump_extras(dev, w, args) -> dict:
         "mass_orders_per_second": round(math.log10(drop[0]) / t, 3) if drop[0] > 0 else None,
         "status": ["converged", "budget", "diverged"][run.status],
     }
+    u_steady = dev.download()
+    if args.implicit_iterations > 0:
+        # The same channel from the same freestream state by the steady implicit path
+        # (pseudo-transient continuation, GCR + multicolour block Gauss-Seidel on the device).
+        dev.upload(w.u)
+        t0 = time.perf_counter()
+        imp = steady.solve_implicit(dev, max_iterations=args.implicit_iterations, residual_orders=8.0,
+                                    scales=scales)
+        t = time.perf_counter() - t0
+        drop = imp.reference / imp.residuals
+        out["steady_implicit"] = {
+            "iterations": imp.iterations,
+            "seconds": round(t, 3),
+            "first_order_steps": imp.first_order_steps,
+            "orders_dropped": [round(math.log10(d), 3) if d > 0 and math.isfinite(d) else None for d in drop],
+            "mass_orders_per_second": round(math.log10(drop[0]) / t, 3) if drop[0] > 0 else None,
+            "gcr_directions": int(np.sum(imp.cycles)),
+            "max_cfl": float(imp.cfl.max()) if imp.iterations else None,
+            "status": ["converged", "budget", "diverged"][imp.status],
+        }
+        log(f"implicit steady run: {imp.iterations} iterations in {t:.1f} s, orders {out['steady_implicit']['orders_dropped']}")
     c = dev.conservation()
     out["conservation"] = {
         "mass_E": c["E"][0],
@@ -249,7 +270,7 @@ def bump_extras(dev, w, args) -> dict:
         "D": [float(v) for v in c["D"]],
     }
     if w.policy == "fp32+acc64" and not args.no_fp64_check:
-        u = dev.download()
+        u = u_steady
         r32 = dev.residual(u)
         dev.close()
         dev64 = DeviceStepper(w.mesh, w.scheme, "fp64", w.bc_kind, w.bc, w.source)
@@ -266,7 +287,7 @@ def bump_extras(dev, w, args) -> dict:
 
 def sweep_implicit_main(args) -> int:
     """The sweep by the steady implicit path: every case from the freestream to a 1e-8 relative
-    residual (fp64) or the round-off plateau (fp32+acc64), one stepper per case in turn; per-case
+    residual (fp64) or the round-off plateau (fp32+acc64), one stepper per case, --workers at a time; per-case
     iterations and seconds, and the whole sweep's wall time."""
     import numpy as np
 
@@ -275,7 +296,7 @@ def sweep_implicit_main(args) -> int:
     policy = args.policy or "fp32+acc64"
     t0 = time.time()
     rep = sweep.run_implicit(n_cases=args.cases, seed=1, max_iterations=args.max_iterations, policy=policy,
-                             flux=args.flux or "hllc", log=log)
+                             flux=args.flux or "hllc", workers=args.workers, log=log)
     cases = rep["cases"]
     iters = [c["iterations"] for c in cases]
     line = {
@@ -299,6 +320,8 @@ def sweep_implicit_main(args) -> int:
             "cells": rep["total_cells"],
         },
         "cells_iter_per_s": round(rep["cells_iter_per_s"], 1),
+        "workers": rep["workers"],
+        "case_seconds_sum": round(sum(c["seconds"] for c in cases), 2),
         "converged": sum(c["status"] == "converged" for c in cases),
         "diverged": sum(c["status"] == "diverged" for c in cases),
         "iterations": {"min": int(min(iters)), "median": float(np.median(iters)), "max": int(max(iters))},
@@ -375,13 +398,14 @@ def warmstart_main(args) -> int:
     if args.test_orders is None:
         args.test_orders = 8.0 if implicit else 2.0
     net, train = warmstart.train_network(train_specs, args.train_iterations if not implicit else args.max_iterations,
-                                         args.epochs, log=log, method=args.method, policy=policy)
+                                         args.epochs, log=log, method=args.method, policy=policy,
+                                         workers=args.workers)
     fwd_ms = net.time_forward(1 << 18, 5)
     tflops = net.flops_per_sample() * (1 << 18) / (fwd_ms * 1e-3) / 1e12
     log(f"network: loss {train['loss']:.3e} after {args.epochs} epochs ({train['train_s']:.1f} s); forward "
         f"{tflops:.1f} TFLOP/s")
     cmp = warmstart.compare(net, test_specs, args.test_orders, args.max_iterations, log=log, method=args.method,
-                            policy=policy)
+                            policy=policy, workers=args.workers)
     free, warm, warm_abs = cmp["free"], cmp["warm"], cmp["warm_abs"]
     t_free, t_warm, t_warm_abs, t_infer = cmp["free_s"], cmp["warm_s"], cmp["warm_abs_s"], cmp["inference_s"]
     rows = []
@@ -447,7 +471,16 @@ def mlp_main(args) -> int:
     t = np.stack([np.sin(2 * x[:, 0]) + x[:, 1], np.cos(x[:, 2] * x[:, 3]), x[:, 4] ** 2, x[:, 5] - x[:, 6],
                   np.tanh(x[:, 0] + x[:, 6])], axis=1).astype(np.float32)
     net.fit_normalization(x, t)
-    infer = {}
+    # The inference tile (samples per workgroup) with the shortest time on 2^20 samples.
+    tiles = {}
+    for rows in (64, 96, 128):
+        net.set_infer_rows(rows)
+        ms = net.time_forward(1 << 20, 20)
+        tiles[str(rows)] = round(net.flops_per_sample() * (1 << 20) / (ms * 1e-3) / 1e12, 1)
+        log(f"inference tile {rows} samples per workgroup: {ms:.3f} ms for 2^20 samples, {tiles[str(rows)]} TFLOP/s")
+    best = int(max(tiles, key=lambda k: tiles[k]))
+    net.set_infer_rows(best)
+    infer = {"tiles_TFLOPs": tiles, "rows": best}
     for n in (1 << 18, 1 << 20):
         ms = net.time_forward(n, 20)
         infer[str(n)] = {"ms": round(ms, 4), "TFLOP/s": round(net.flops_per_sample() * n / (ms * 1e-3) / 1e12, 1)}
@@ -535,6 +568,8 @@ def main() -> int:
                     help="sweep / warmstart: the batch stepper's explicit SSP-RK3 steps, or the steady implicit path "
                     "(--max-iterations is then its iteration budget per case)")
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
+    ap.add_argument("--workers", type=int, default=8,
+                    help="sweep / warmstart with --method implicit: cases solved at a time (one stepper and stream each)")
     ap.add_argument("--max-iterations", type=int, default=None,
                     help="sweep / warmstart / mms: iteration budget per case (default 20000, 600 with --method implicit)")
     ap.add_argument("--n", type=int, default=128, help="tets: cubes per side (6 n^3 tetrahedra)")
@@ -547,6 +582,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--steady-iterations", type=int, default=400)
+    ap.add_argument("--implicit-iterations", type=int, default=0,
+                    help="bump: budget of a steady implicit run from the freestream after the explicit one (0: none)")
     ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
     ap.add_argument("--tune", default="", help="kernel variants, key=value,... (face_stage, graph, grad_occupancy, "
                     "face_occupancy); set after the autotuner")

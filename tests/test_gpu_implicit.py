@@ -143,3 +143,29 @@ def test_implicit_runs_reach_the_threshold_on_sweep_channels(index):
     dev.close()
     assert r.status == steady.CONVERGED, (r.iterations, r.residuals / r.reference)
     assert np.all(r.alpha[r.alpha > 0] <= 1.0)
+
+
+def test_cases_solved_together_give_each_case_its_own_bits():
+    """Four sweep channels solved four at a time (one stepper, stream and host thread each) give
+    every case the residuals, CFL numbers, update fractions and final state of the same case solved
+    alone: the steppers share nothing but the device."""
+    sp = [dataclasses.replace(s, lattice=sweep.lattice_for(5000)) for s in sweep.specs(64, seed=1)[:4]]
+    print(f"implicit cases together: {len(sp)} channels of {sweep.lattice_for(5000)}", flush=True)
+    runs = {}
+    for workers in (4, 1):
+        finals = {}
+
+        def keep(k, c, dev, r):
+            finals[k] = dev.download()
+
+        solved, wall = sweep.solve_implicit_cases(sp, "fp64", max_iterations=40, workers=workers, finish=keep,
+                                                  log=lambda *a: None)
+        assert wall > 0
+        runs[workers] = (solved, finals)
+    for k in range(len(sp)):
+        a, b = runs[4][0][k]["result"], runs[1][0][k]["result"]
+        assert a.iterations == b.iterations > 0
+        np.testing.assert_array_equal(a.history, b.history)
+        np.testing.assert_array_equal(a.cfl, b.cfl)
+        np.testing.assert_array_equal(a.alpha, b.alpha)
+        np.testing.assert_array_equal(runs[4][1][k], runs[1][1][k])

@@ -80,9 +80,11 @@ def test_fused_inference_gives_the_layer_by_layer_bits(hidden, n):
     m.fit_normalization(x, t)
     m.train(x, t, epochs=2, batch=256, lr=1e-3, seed=0)  # weights away from their initial values
     xs, _ = data(n, 4)
-    fused = m.predict(xs)
     m.set_fused(False)
     layered = m.predict(xs)
     m.set_fused(True)
-    np.testing.assert_array_equal(fused, layered)
-    assert m.time_forward(4096, 2) > 0.0
+    # every inference tile (samples per workgroup) gives the same bits
+    for rows in (64, 96, 128):
+        m.set_infer_rows(rows)
+        np.testing.assert_array_equal(m.predict(xs), layered, err_msg=f"{rows} samples per workgroup")
+        assert m.time_forward(4096, 2) > 0.0
