@@ -3,8 +3,8 @@
 // limb i weighing 2^(32 i - 1088), then the counts of +inf, -inf and NaN terms. The device kernels
 // add into the same words.
 //
-// Rounding: the limbs are resolved once into the sign and the 68 base-2^32 digits of |sum| (one
-// carry pass in 128-bit arithmetic); the 64 bits from the leading one down are read as one integer
+// Rounding: the limbs are resolved into the sign and the base-2^32 digits of |sum| (carry passes in
+// 128-bit arithmetic); the 64 bits from the leading one down are read as one integer
 // whose lowest bit is OR-ed with every bit below them (rounding to odd at 64 bits), and that integer
 // is converted to a double, which rounds it to nearest even at 53 bits. Rounding to odd at p + 2 or
 // more bits then to nearest at p bits gives the correctly rounded result (Boldo and Melquiond 2008,
@@ -35,78 +35,71 @@ inline constexpr int kExactLimbs = 68;
 inline constexpr int kExactWords = kExactLimbs + 3;
 inline constexpr int kExactBias = 1088;  // limb 0's weight is 2^-kExactBias
 
-namespace exact_detail {
-
-// |sum| as base-2^32 digits plus whatever lies above the last limb (`over`), and the sign.
-struct Magnitude {
-  std::uint32_t digit[kExactLimbs];
-  unsigned __int128 over;  // multiples of 2^(32 * kExactLimbs) in units of limb 0
-  bool negative;
-};
-
-MXE_HOST_DEVICE inline Magnitude resolve(const long long* w) {
-  Magnitude m{};
-  __int128 carry = 0;
-  for (int i = 0; i < kExactLimbs; ++i) {
-    const __int128 acc = carry + w[i];
-    m.digit[i] = static_cast<std::uint32_t>(static_cast<unsigned __int128>(acc) & 0xffffffffu);
-    carry = acc >> 32;  // floor division: the digits stay in [0, 2^32)
-  }
-  m.negative = carry < 0;
-  if (!m.negative) {
-    m.over = static_cast<unsigned __int128>(carry);
-    return m;
-  }
-  // Two's complement of (carry, digits): invert, add one from the lowest digit up.
-  unsigned long long up = 1;
-  for (int i = 0; i < kExactLimbs; ++i) {
-    const unsigned long long v = static_cast<unsigned long long>(~m.digit[i]) + up;
-    m.digit[i] = static_cast<std::uint32_t>(v);
-    up = v >> 32;
-  }
-  m.over = static_cast<unsigned __int128>(-(carry + 1)) + up;
-  return m;
-}
-
-// Bit `b` (0 = the lowest bit of digit 0) of the digits.
-MXE_HOST_DEVICE inline unsigned bit_of(const Magnitude& m, int b) {
-  return b < 0 ? 0u : (m.digit[b >> 5] >> (b & 31)) & 1u;
-}
-
-// Whether any bit below position `b` is set.
-MXE_HOST_DEVICE inline bool any_below(const Magnitude& m, int b) {
-  if (b <= 0) return false;
-  const int whole = b >> 5;
-  for (int i = 0; i < whole; ++i)
-    if (m.digit[i] != 0) return true;
-  const int part = b & 31;
-  return part != 0 && (m.digit[whole] & ((1u << part) - 1u)) != 0;
-}
-
-}  // namespace exact_detail
-
-// The value of exact words, rounded once to the nearest double (ties to even).
+// The value of exact words, rounded once to the nearest double (ties to even). Two passes over the
+// limbs in one register set (no digit array, so a device thread runs it without private memory):
+// the first resolves the carries only to learn the sign; the second resolves them again into the
+// base-2^32 digits of |sum| (two's complement on the fly when negative) and keeps the highest
+// non-zero digit, the two digits under it and whether any digit below those is non-zero.
 MXE_HOST_DEVICE inline double exact_value(const long long* in) {
-  using namespace exact_detail;
   constexpr double inf = std::numeric_limits<double>::infinity();
   const long long pos_inf = in[kExactLimbs], neg_inf = in[kExactLimbs + 1], nans = in[kExactLimbs + 2];
   if (nans > 0 || (pos_inf > 0 && neg_inf > 0)) return std::numeric_limits<double>::quiet_NaN();
   if (pos_inf > 0) return inf;
   if (neg_inf > 0) return -inf;
-  const Magnitude m = resolve(in);
-  // 2^(32 * 68 - 1088) = 2^1088 and above is past the largest double.
-  if (m.over != 0) return m.negative ? -inf : inf;
-  int lead = -1;  // position of the leading one
-  for (int i = kExactLimbs - 1; i >= 0 && lead < 0; --i)
-    if (m.digit[i] != 0) lead = 32 * i + 31 - __builtin_clz(m.digit[i]);
-  if (lead < 0) return 0.0;
-  // The 64 bits lead - 63 .. lead (zeros below bit 0), rounded to odd by everything under them.
-  std::uint64_t top = 0;
-  for (int b = lead; b > lead - 64; --b) top = (top << 1) | bit_of(m, b);
-  if (any_below(m, lead - 63)) top |= 1u;
+  // Unrolled: the limbs' loads do not wait on the carry chain.
+  __int128 carry = 0;
+#pragma GCC unroll 68
+  for (int i = 0; i < kExactLimbs; ++i) carry = (carry + in[i]) >> 32;  // floor division
+  const bool negative = carry < 0;
+  carry = 0;
+  unsigned long long up = 1;  // the +1 of the two's complement, carried up the digits
+  std::uint32_t prev1 = 0, prev2 = 0, prev3 = 0;  // digits i - 1, i - 2, i - 3
+  bool low_any = false;                            // any non-zero digit below i - 3
+  int top_index = -1;                              // the highest non-zero digit so far
+  std::uint32_t top0 = 0, top1 = 0, top2 = 0;      // that digit and the two under it
+  bool top_sticky = false;                         // any non-zero digit under those three
+#pragma GCC unroll 68
+  for (int i = 0; i < kExactLimbs; ++i) {
+    const __int128 acc = carry + in[i];
+    std::uint32_t digit = static_cast<std::uint32_t>(static_cast<unsigned __int128>(acc) & 0xffffffffu);
+    carry = acc >> 32;
+    if (negative) {
+      const unsigned long long v = static_cast<unsigned long long>(~digit) + up;
+      digit = static_cast<std::uint32_t>(v);
+      up = v >> 32;
+    }
+    low_any = low_any || prev3 != 0;
+    if (digit != 0) {
+      top_index = i;
+      top0 = digit;
+      top1 = prev1;
+      top2 = prev2;
+      top_sticky = low_any || prev3 != 0;
+    }
+    prev3 = prev2;
+    prev2 = prev1;
+    prev1 = digit;
+  }
+  // Multiples of 2^(32 * kExactLimbs) = 2^1088 (in units of limb 0) and above: past the largest
+  // double.
+  const unsigned __int128 over =
+      negative ? static_cast<unsigned __int128>(-(carry + 1)) + up : static_cast<unsigned __int128>(carry);
+  if (over != 0) return negative ? -inf : inf;
+  if (top_index < 0) return 0.0;
+  int clz = 0;
+  while (((top0 << clz) & 0x80000000u) == 0) ++clz;
+  const int lead = 32 * top_index + 31 - clz;  // position of the leading one
+  // The 96 bits of the three digits; the 64 from the leading one down, rounded to odd by every bit
+  // under them (the rest of the window and the digits below it).
+  const int shift = 32 - clz;  // 1 .. 32
+  const unsigned __int128 window = (static_cast<unsigned __int128>(top0) << 64) |
+                                   (static_cast<unsigned __int128>(top1) << 32) | top2;
+  std::uint64_t top = static_cast<std::uint64_t>(window >> shift);
+  const bool sticky = top_sticky || (window & ((static_cast<unsigned __int128>(1) << shift) - 1)) != 0;
+  if (sticky) top |= 1u;
   const double rounded = static_cast<double>(top);  // to nearest even at 53 bits
   const double magnitude = std::ldexp(rounded, lead - 63 - kExactBias);
-  return m.negative ? -magnitude : magnitude;
+  return negative ? -magnitude : magnitude;
 }
 
 // A running exact sum on the host, in the same words.

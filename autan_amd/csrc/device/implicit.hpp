@@ -8,11 +8,12 @@
 //                             and added as sign * t;
 //   k_impl_invert             BlockMatrix::factor: each diagonal block's inverse by Gauss-Jordan
 //                             elimination with partial pivoting (physics/src/block_matrix.cpp:14-45);
-//   k_impl_relax              BlockMatrix::relax over the cells of one colour (a colour's cells share
-//                             no face, so they update together);
+//   k_impl_relax(_run)        BlockMatrix::relax over the cells of one colour (a colour's cells share
+//                             no face, so they update together), or of a run of small colours;
 //   k_impl_multiply           BlockMatrix::multiply;
 //   k_impl_dot, k_impl_maxabs the GCR's inner products as exact sums and its largest entry
-//                             (EulerPhysics::owned_dot, owned_max_abs);
+//                             (EulerPhysics::owned_dot, owned_max_abs), the sums also fused into
+//                             the GCR's updates (k_impl_orth_dots, k_impl_advance_dot);
 //   k_impl_admissible         EulerPhysics::admissible_step: each cell's largest fraction of du that
 //                             keeps its density and pressure above half their values, the least over
 //                             the cells (an exact minimum of the FP64 values);
@@ -174,35 +175,91 @@ struct ImplRows {
   const int2* ent;   // 2 ni
 };
 
-// BlockMatrix::relax of the cells cells[0 .. count): x_c = D_c^-1 (b_c - sum_f A_cf x_other).
+// BlockMatrix::relax of one cell by a group of five lanes, lane q computing row q: x_c = D_c^-1
+// (b_c - sum_f A_cf x_other). Row q's sum runs over the faces, then j, in the one-thread order;
+// the group's five rows reach every lane of the group by shuffles for the inverse's product. Each
+// wave holds 12 groups (lanes 60-63 idle); every lane takes part in the shuffles.
+constexpr int kRelaxGroups = kWave / 5;
+
+__device__ __forceinline__ void impl_relax_rows(const ImplRows& rows, int c, bool active, int n,
+                                                const double* __restrict__ offd, const double* __restrict__ inverse,
+                                                const double* __restrict__ b, double* __restrict__ x) {
+  const int lane = threadIdx.x % kWave;
+  const int q = lane % 5, base = lane - q;
+  double r = 0.0;
+  if (active) {
+    r = b[static_cast<long long>(q) * n + c];
+    for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
+      const int2 e = rows.ent[k];
+      const double* a = offd + 25LL * e.x + 5 * q;
+      double xo[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) r -= a[j] * xo[j];
+    }
+  }
+  double rj[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) rj[j] = __shfl(r, (base + j) % kWave);
+  if (active) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s += inverse[static_cast<long long>(5 * q + j) * n + c] * rj[j];
+    x[static_cast<long long>(q) * n + c] = s;
+  }
+}
+
+// BlockMatrix::relax of the cells cells[0 .. count) (one colour): 12 cells per wave.
 __global__ __launch_bounds__(kBlock) void k_impl_relax(ImplRows rows, const int* __restrict__ cells, int count, int n,
                                                        const double* __restrict__ offd,
                                                        const double* __restrict__ inverse,
                                                        const double* __restrict__ b, double* __restrict__ x) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= count) return;
-  const int c = cells[t];
-  double r[5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) r[i] = b[static_cast<long long>(i) * n + c];
-  for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
-    const int2 e = rows.ent[k];
-    const double* a = offd + 25LL * e.x;
-    double xo[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-#pragma unroll
-      for (int j = 0; j < 5; ++j) r[i] -= a[5 * i + j] * xo[j];
+  const int lane = threadIdx.x % kWave;
+  const int g = lane / 5;
+  const int t = (blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave)) * kRelaxGroups + g;
+  const bool active = g < kRelaxGroups && t < count;
+  impl_relax_rows(rows, active ? cells[t] : 0, active, n, offd, inverse, b, x);
+}
+
+// Launch width of k_impl_relax for `count` cells.
+inline int impl_relax_blocks(int count) {
+  constexpr int per_block = (kBlock / kWave) * kRelaxGroups;
+  return (count + per_block - 1) / per_block;
+}
+
+// A run of small colours relaxed in order by one workgroup: phase p relaxes cells[first[p] ..
+// first[p] + count[p]), the barrier between phases making each phase's x visible to the next (the
+// workgroup shares one CU's cache), as separate launches would. Saves a launch per colour where a
+// colour is too small to fill the device.
+constexpr int kRelaxRunThreads = 1024;
+constexpr int kRelaxRunMax = 32;
+// Colours of at most this many cells (two passes of the run's workgroup) join a run.
+constexpr int kRelaxRunCells = 2 * (kRelaxRunThreads / kWave) * (kWave / 5);
+struct ImplRun {
+  int first[kRelaxRunMax];
+  int count[kRelaxRunMax];
+  int phases;
+};
+
+__global__ __launch_bounds__(kRelaxRunThreads) void k_impl_relax_run(ImplRows rows, const int* __restrict__ cells,
+                                                                     ImplRun run, int n,
+                                                                     const double* __restrict__ offd,
+                                                                     const double* __restrict__ inverse,
+                                                                     const double* __restrict__ b,
+                                                                     double* __restrict__ x) {
+  constexpr int per_pass = (kRelaxRunThreads / kWave) * kRelaxGroups;
+  const int lane = threadIdx.x % kWave;
+  const int g = lane / 5;
+  const int slot = static_cast<int>(threadIdx.x / kWave) * kRelaxGroups + g;
+  for (int p = 0; p < run.phases; ++p) {
+    // wave-uniform trip count: every lane reaches the shuffles
+    for (int t0 = 0; t0 < run.count[p]; t0 += per_pass) {
+      const int t = t0 + slot;
+      const bool active = g < kRelaxGroups && t < run.count[p];
+      impl_relax_rows(rows, active ? cells[run.first[p] + t] : 0, active, n, offd, inverse, b, x);
     }
-  }
-#pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j < 5; ++j) s += inverse[static_cast<long long>(5 * i + j) * n + c] * r[j];
-    x[static_cast<long long>(i) * n + c] = s;
+    __syncthreads();
   }
 }
 
@@ -237,6 +294,23 @@ __global__ __launch_bounds__(kBlock) void k_impl_multiply(ImplRows rows, int n, 
   for (int i = 0; i < 5; ++i) y[static_cast<long long>(i) * n + c] = r[i];
 }
 
+// A block's exact-sum accumulator (carries resolved) added into the global words.
+__device__ __forceinline__ void impl_flush(long long* acc, long long* __restrict__ words) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int l = 0; l + 1 < kLimbs; ++l) {
+      const long long carry = acc[l] >> 32;
+      acc[l] -= carry * (1LL << 32);
+      acc[l + 1] += carry;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kWords; t += kBlock) {
+    const long long w = acc[t];
+    if (w != 0) atomicAdd(reinterpret_cast<unsigned long long*>(words + t), static_cast<unsigned long long>(w));
+  }
+}
+
 // sum_i u_i v_i over m values as exact words (added into `words`, cleared beforehand): each product
 // rounded to FP64, then added exactly. Grid-stride with wave-uniform trip counts.
 __global__ __launch_bounds__(kBlock) void k_impl_dot(const double* __restrict__ u, const double* __restrict__ v,
@@ -253,19 +327,97 @@ __global__ __launch_bounds__(kBlock) void k_impl_dot(const double* __restrict__ 
     const double t = active ? u[i] * v[i] : 0.0;
     exact_add_wave(acc, t, active);
   }
+  impl_flush(acc, words);
+}
+
+// The GCR's modified Gram-Schmidt step fused with the inner products that follow it: when zj is
+// given, beta = (w, w_j) / (w_j, w_j) from the words of the last inner product and the device's
+// (w_j, w_j) (each block rounds them with exact_value), z -= beta zj and w -= beta wj; then sum
+// w_i p_i into wp and, when q is given, sum w_i q_i into wq, each over the updated w. The sums are
+// exact, so fusing them changes no bit.
+__global__ __launch_bounds__(kBlock) void k_impl_orth_dots(double* __restrict__ z, double* __restrict__ w,
+                                                           const double* __restrict__ zj,
+                                                           const double* __restrict__ wj, long long m,
+                                                           const long long* __restrict__ beta_words,
+                                                           const double* __restrict__ beta_den,
+                                                           const double* __restrict__ p, long long* __restrict__ wp,
+                                                           const double* __restrict__ q, long long* __restrict__ wq) {
+  __shared__ long long acc[2][kWords];
+  __shared__ double beta_sh;
+  for (int t = threadIdx.x; t < 2 * kWords; t += kBlock) acc[t / kWords][t % kWords] = 0;
+  if (zj != nullptr && threadIdx.x == 0) beta_sh = exact_value(beta_words) / *beta_den;
   __syncthreads();
+  const double beta = zj != nullptr ? beta_sh : 0.0;
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock + wave * kWave; base < m; base += stride) {
+    const long long i = base + lane;
+    const bool active = i < m;
+    double wi = 0.0;
+    if (active) {
+      wi = w[i];
+      if (zj != nullptr) {
+        z[i] -= beta * zj[i];
+        wi -= beta * wj[i];
+        w[i] = wi;
+      }
+    }
+    exact_add_wave(acc[0], active ? wi * p[i] : 0.0, active);
+    if (q != nullptr) exact_add_wave(acc[1], active ? wi * q[i] : 0.0, active);
+  }
+  impl_flush(acc[0], wp);
+  if (q != nullptr) impl_flush(acc[1], wq);
+}
+
+// The GCR's update fused with the scalars before it and the inner product after it: each block
+// rounds (w, w) and alpha = (r, w) / (w, w) from their words (block 0 stores them for the host and
+// the next directions), then, when (w, w) > 0 (else the GCR stops before its update), x += sa z
+// (sa = scale * alpha, scale the residual's largest magnitude from its bits) and r -= alpha w;
+// then sum r_i r_i into words.
+__global__ __launch_bounds__(kBlock) void k_impl_advance_dot(double* __restrict__ x, double* __restrict__ r,
+                                                             const double* __restrict__ z,
+                                                             const double* __restrict__ w, long long m,
+                                                             const unsigned long long* __restrict__ scale_bits,
+                                                             const long long* __restrict__ ww_words,
+                                                             const long long* __restrict__ rw_words,
+                                                             double* __restrict__ ww_out, double* __restrict__ alpha_out,
+                                                             long long* __restrict__ words) {
+  __shared__ long long acc[kWords];
+  __shared__ double ww_sh, alpha_sh;
+  for (int t = threadIdx.x; t < kWords; t += kBlock) acc[t] = 0;
   if (threadIdx.x == 0) {
-    for (int l = 0; l + 1 < kLimbs; ++l) {
-      const long long carry = acc[l] >> 32;
-      acc[l] -= carry * (1LL << 32);
-      acc[l + 1] += carry;
+    const double ww = exact_value(ww_words);
+    const double alpha = exact_value(rw_words) / ww;
+    ww_sh = ww;
+    alpha_sh = alpha;
+    if (blockIdx.x == 0) {
+      *ww_out = ww;
+      *alpha_out = alpha;
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kWords; t += kBlock) {
-    const long long w = acc[t];
-    if (w != 0) atomicAdd(reinterpret_cast<unsigned long long*>(words + t), static_cast<unsigned long long>(w));
+  const bool update = ww_sh > 0.0;
+  const double alpha = alpha_sh;
+  const double sa = __longlong_as_double(static_cast<long long>(*scale_bits)) * alpha;
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  for (long long base = static_cast<long long>(blockIdx.x) * kBlock + wave * kWave; base < m; base += stride) {
+    const long long i = base + lane;
+    const bool active = i < m;
+    double ri = 0.0;
+    if (active) {
+      ri = r[i];
+      if (update) {
+        x[i] += sa * z[i];
+        ri -= alpha * w[i];
+        r[i] = ri;
+      }
+    }
+    exact_add_wave(acc, active ? ri * ri : 0.0, active);
   }
+  impl_flush(acc, words);
 }
 
 // The largest |u_i| as the bits of a non-negative double (their integer order is their order);
@@ -336,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_admissible(const double* __rest
 // y += B v, the boundary faces' fluxes' derivative through the reconstruction's gradients, which
 // the assembled blocks drop. One thread per cell owning a boundary face (`cells`, ascending): the
 // primitive tangents dW = dW/dq v of the cell and its stencil cells (the direction rounded to the
-// compute type, the products summed in it), the gradient's tangent sum_k (dW_k - dW_c) w_k (FP64
+// compute type, the products summed in it; k_impl_tangents, for every cell), the gradient's tangent sum_k (dW_k - dW_c) w_k (FP64
 // sums, boundary values held), then over its boundary faces in cell_faces order, where the
 // reconstruction did not fall back to the cell value, bnd_jac applied to phi (grad_tan . r), and on
 // warped slip and wall faces the wall-warp term's tangent phi_p M^T grad_tan p.
@@ -355,17 +507,31 @@ __device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, co
   }
 }
 
+// Every cell's primitive tangent dW/dq v (impl_tangent), 5 x n in the compute type, for the
+// boundary term's stencils.
+template <class C>
+__global__ __launch_bounds__(kBlock) void k_impl_tangents(const double* __restrict__ dwdu, const double* __restrict__ v,
+                                                          int n, C* __restrict__ out) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n) return;
+  C dw[5];
+  impl_tangent<C>(dwdu, v, n, c, dw);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) out[static_cast<long long>(i) * n + c] = dw[i];
+}
+
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_bgrad(Dev<C> d, const int* __restrict__ cells, int count,
-                                                       const double* __restrict__ dwdu, const double* __restrict__ bj,
-                                                       const double* __restrict__ v, double* __restrict__ y) {
+                                                       const C* __restrict__ tangents, const double* __restrict__ bj,
+                                                       double* __restrict__ y) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   const int c = cells[t];
   const long long n = d.nc;
   const Params<C>& pa = d.ptab[cell_case(d, c)];
   C dwc[5];
-  impl_tangent<C>(dwdu, v, n, c, dwc);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) dwc[i] = tangents[i * n + c];
   double acc[15];
 #pragma unroll
   for (int q = 0; q < 15; ++q) acc[q] = 0.0;
@@ -376,7 +542,10 @@ __global__ __launch_bounds__(kBlock) void k_impl_bgrad(Dev<C> d, const int* __re
     const int e = d.sidx[k];
     const C w0 = d.sw0[k], w1 = d.sw1[k], w2 = d.sw2[k];
     C dwn[5] = {C(0), C(0), C(0), C(0), C(0)};
-    if (e >= 0) impl_tangent<C>(dwdu, v, n, e, dwn);
+    if (e >= 0) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) dwn[i] = tangents[i * n + e];
+    }
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       const C dd = dwn[i] - dwc[i];
@@ -459,34 +628,6 @@ __global__ void k_impl_scalar(const long long* __restrict__ words, const double*
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double v = exact_value(words);
   *out = den != nullptr ? v / *den : v;
-}
-
-// z -= beta zj and w -= beta wj (beta on the device)
-__global__ __launch_bounds__(kBlock) void k_impl_orthogonalize(double* __restrict__ z, double* __restrict__ w,
-                                                               const double* __restrict__ zj,
-                                                               const double* __restrict__ wj, long long m,
-                                                               const double* __restrict__ beta_at) {
-  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= m) return;
-  const double beta = *beta_at;
-  z[i] -= beta * zj[i];
-  w[i] -= beta * wj[i];
-}
-
-// x += sa z (sa = scale * alpha) and r -= alpha w, with scale the residual's largest magnitude (its
-// bits) and alpha on the device; nothing when the direction's (w, w) is not positive, where the
-// GCR stops before its update.
-__global__ __launch_bounds__(kBlock) void k_impl_advance(double* __restrict__ x, double* __restrict__ r,
-                                                         const double* __restrict__ z, const double* __restrict__ w,
-                                                         long long m, const unsigned long long* __restrict__ scale_bits,
-                                                         const double* __restrict__ alpha_at,
-                                                         const double* __restrict__ ww_at) {
-  const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= m || !(*ww_at > 0.0)) return;
-  const double alpha = *alpha_at;
-  const double sa = __longlong_as_double(static_cast<long long>(*scale_bits)) * alpha;
-  x[i] += sa * z[i];
-  r[i] -= alpha * w[i];
 }
 
 // u += alpha du

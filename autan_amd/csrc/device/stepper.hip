@@ -481,6 +481,7 @@ class Stepper final : public StepperBase {
     roff_.upload(off, stream_);
     rent_.upload(ent, stream_);
     ccells_.upload(cells, stream_);
+    plan_relax();
     const std::size_t m = 5 * n;
     jd_ = Buffer<double>(25 * n);
     jf_ = Buffer<double>(50 * std::max<std::size_t>(ni, 1));
@@ -496,6 +497,7 @@ class Stepper final : public StepperBase {
     bits_ = Buffer<unsigned long long>(2);
     flag_ = Buffer<int>(1);
     scalars_ = Buffer<double>(kImplScalars);
+    if (nbo_ > 0) tan_ = Buffer<C>(m);
     check(hipHostMalloc(reinterpret_cast<void**>(&host_scalars_), kImplScalars * sizeof(double), hipHostMallocDefault),
           "hipHostMalloc");
     check(hipStreamSynchronize(stream_), "implicit_setup");
@@ -503,37 +505,83 @@ class Stepper final : public StepperBase {
   }
 
   // kImplSweeps symmetric sweeps on A z = b from z = 0: the colours in increasing, then decreasing
-  // order (BlockMatrix::gauss_seidel).
+  // order (BlockMatrix::gauss_seidel), as the launches of relax_plan_.
   void gauss_seidel(const double* b, double* z, long long m, int gm) {
     launch("impl_zero", [&] { k_impl_zero<<<gm, kBlock, 0, stream_>>>(z, m); });
     const ImplRows rows{roff_.get(), rent_.get()};
+    for (const RelaxLaunch& l : relax_plan_) {
+      if (l.run.phases == 0) {
+        launch("impl_relax", [&] {
+          k_impl_relax<<<impl_relax_blocks(l.count), kBlock, 0, stream_>>>(rows, ccells_.get() + l.first, l.count, nc_,
+                                                                           offd_.get(), inv_.get(), b, z);
+        });
+      } else {
+        launch("impl_relax_run", [&] {
+          k_impl_relax_run<<<1, kRelaxRunThreads, 0, stream_>>>(rows, ccells_.get(), l.run, nc_, offd_.get(),
+                                                                inv_.get(), b, z);
+        });
+      }
+    }
+  }
+
+  // The sweeps' colour sequence as launches: a colour of more than kRelaxRunCells cells is one
+  // launch over the device; consecutive smaller ones are one workgroup's run of phases.
+  struct RelaxLaunch {
+    int first = 0, count = 0;
+    ImplRun run{};
+  };
+  std::vector<RelaxLaunch> relax_plan_;
+
+  void plan_relax() {
     const int colours = static_cast<int>(colour_off_.size()) - 1;
-    auto relax = [&](int k) {
+    std::vector<int> order;
+    for (int s = 0; s < kImplSweeps; ++s) {
+      for (int k = 0; k < colours; ++k) order.push_back(k);
+      for (int k = colours - 1; k >= 0; --k) order.push_back(k);
+    }
+    relax_plan_.clear();
+    for (const int k : order) {
       const int first = colour_off_[static_cast<std::size_t>(k)];
       const int count = colour_off_[static_cast<std::size_t>(k) + 1] - first;
-      if (count == 0) return;
-      launch("impl_relax", [&] {
-        k_impl_relax<<<(count + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(rows, ccells_.get() + first, count, nc_,
-                                                                             offd_.get(), inv_.get(), b, z);
-      });
-    };
-    for (int s = 0; s < kImplSweeps; ++s) {
-      for (int k = 0; k < colours; ++k) relax(k);
-      for (int k = colours - 1; k >= 0; --k) relax(k);
+      if (count == 0) continue;
+      const bool small = count <= kRelaxRunCells;
+      RelaxLaunch* last = relax_plan_.empty() ? nullptr : &relax_plan_.back();
+      if (small && last != nullptr && last->run.phases > 0 && last->run.phases < kRelaxRunMax) {
+        last->run.first[last->run.phases] = first;
+        last->run.count[last->run.phases] = count;
+        ++last->run.phases;
+        continue;
+      }
+      RelaxLaunch l;
+      l.first = first;
+      l.count = count;
+      if (small) {
+        l.run.first[0] = first;
+        l.run.count[0] = count;
+        l.run.phases = 1;
+      }
+      relax_plan_.push_back(l);
+    }
+    // A run of one phase is an ordinary launch.
+    for (RelaxLaunch& l : relax_plan_) {
+      if (l.run.phases == 1) l.run.phases = 0;
     }
   }
 
   // Gcr::solve (solvers/src/gcr.cpp) on kr_ = -R into du_ = 0: returns the residual drop. The
   // scalars (inner products rounded by exact_value, the Gram-Schmidt and step coefficients) are
-  // computed and used on the device; the host reads (w, w) and (r, r) once per direction for the
-  // stopping rules, and takes every decision the host loop takes, on the same values.
+  // computed and used on the device, each inner product fused into the vector update before it
+  // (exact sums: the same bits in any order); the host reads (w, w) and (r, r) once per direction
+  // for the stopping rules, and takes every decision the host loop takes, on the same values.
   double gcr(long long m, int gm, int& cycles) {
-    double* sc = scalars_.get();  // [0, D) (w_k, w_k), then beta, alpha, (r, r), (r_0, r_0)
-    constexpr int kBeta = kImplDirections, kAlpha = kBeta + 1, kRr = kAlpha + 1, kR0 = kRr + 1;
+    double* sc = scalars_.get();  // [0, D) (w_k, w_k), then alpha, (r, r), (r_0, r_0)
+    constexpr int kAlpha = kImplDirections, kRr = kAlpha + 1, kR0 = kRr + 1;
+    static_assert(kR0 + 1 == kImplScalars);
+    // Grid of the kernels with exact sums: about 8 values per thread (each block resolves its
+    // limbs' carries serially before adding them), at most 1024 blocks.
+    const int gd = static_cast<int>(std::max<long long>(1, std::min<long long>(1024, m / (8LL * kBlock))));
     unsigned long long mbits = 0;
-    launch("impl_maxabs", [&] {
-      k_impl_maxabs<<<std::max(1, std::min(gm, 1024)), kBlock, 0, stream_>>>(kr_.get(), m, bits_.get() + 1);
-    });
+    launch("impl_maxabs", [&] { k_impl_maxabs<<<gd, kBlock, 0, stream_>>>(kr_.get(), m, bits_.get() + 1); });
     check(hipMemcpyAsync(&mbits, bits_.get() + 1, sizeof mbits, hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
     check(hipStreamSynchronize(stream_), "implicit_solve");
     double scale;
@@ -542,7 +590,11 @@ class Stepper final : public StepperBase {
     if (!std::isfinite(scale)) return std::numeric_limits<double>::quiet_NaN();
     if (scale == 0.0) return 0.0;
     launch("impl_scale", [&] { k_impl_scale<<<gm, kBlock, 0, stream_>>>(kr_.get(), m, scale, 0); });
-    device_dot(kr_.get(), kr_.get(), m, nullptr, sc + kR0);
+    {
+      long long* w0 = next_words();
+      launch("impl_dot", [&] { k_impl_dot<<<gd, kBlock, 0, stream_>>>(kr_.get(), kr_.get(), m, w0); });
+      launch("impl_scalar", [&] { k_impl_scalar<<<1, kWave, 0, stream_>>>(w0, nullptr, sc + kR0); });
+    }
     double r0 = 0.0, rr = 0.0;
     for (int k = 0; k < kImplDirections; ++k) {
       double* z = zs_.get() + std::size_t(k) * std::size_t(m);
@@ -555,25 +607,59 @@ class Stepper final : public StepperBase {
       // At second order the operator adds the boundary faces' gradient term.
       if (jac_second_ && nbo_ > 0) {
         const Dev<C> d = dev();
+        launch("impl_tangents", [&] {
+          k_impl_tangents<C><<<blocks(nc_), kBlock, 0, stream_>>>(jd_.get(), z, nc_, tan_.get());
+        });
         launch("impl_bgrad", [&] {
-          k_impl_bgrad<C><<<(nbo_ + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(d, bocells_.get(), nbo_, jd_.get(),
-                                                                              jb_.get(), z, w);
+          k_impl_bgrad<C><<<(nbo_ + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(d, bocells_.get(), nbo_, tan_.get(),
+                                                                              jb_.get(), w);
+        });
+      }
+      // Modified Gram-Schmidt: beta_j = (w, w_j) / (w_j, w_j) on the w left by step j - 1, each
+      // step's update fused with the next inner product, the last one's with (w, w) and (r, w);
+      // the scalars are rounded by the kernels that use them.
+      long long* prev = nullptr;
+      long long* ww_words = nullptr;
+      long long* rw_words = nullptr;
+      if (k == 0) {
+        ww_words = next_words();
+        rw_words = next_words();
+        launch("impl_orth_dots", [&] {
+          k_impl_orth_dots<<<gd, kBlock, 0, stream_>>>(z, w, nullptr, nullptr, m, nullptr, nullptr, w, ww_words,
+                                                      kr_.get(), rw_words);
+        });
+      } else {
+        prev = next_words();
+        launch("impl_orth_dots", [&] {
+          k_impl_orth_dots<<<gd, kBlock, 0, stream_>>>(z, w, nullptr, nullptr, m, nullptr, nullptr, ws_.get(), prev,
+                                                      nullptr, nullptr);
         });
       }
       for (int j = 0; j < k; ++j) {
         const double* zj = zs_.get() + std::size_t(j) * std::size_t(m);
         const double* wj = ws_.get() + std::size_t(j) * std::size_t(m);
-        device_dot(w, wj, m, sc + j, sc + kBeta);  // beta = (w, w_j) / (w_j, w_j)
-        launch("impl_orthogonalize", [&] {
-          k_impl_orthogonalize<<<gm, kBlock, 0, stream_>>>(z, w, zj, wj, m, sc + kBeta);
-        });
+        if (j + 1 < k) {
+          long long* nxt = next_words();
+          const double* wn = ws_.get() + std::size_t(j + 1) * std::size_t(m);
+          launch("impl_orth_dots", [&] {
+            k_impl_orth_dots<<<gd, kBlock, 0, stream_>>>(z, w, zj, wj, m, prev, sc + j, wn, nxt, nullptr, nullptr);
+          });
+          prev = nxt;
+        } else {
+          ww_words = next_words();
+          rw_words = next_words();
+          launch("impl_orth_dots", [&] {
+            k_impl_orth_dots<<<gd, kBlock, 0, stream_>>>(z, w, zj, wj, m, prev, sc + j, w, ww_words, kr_.get(),
+                                                        rw_words);
+          });
+        }
       }
-      device_dot(w, w, m, nullptr, sc + k);
-      device_dot(kr_.get(), w, m, sc + k, sc + kAlpha);  // alpha = (r, w) / (w, w)
-      launch("impl_advance", [&] {
-        k_impl_advance<<<gm, kBlock, 0, stream_>>>(du_.get(), kr_.get(), z, w, m, bits_.get() + 1, sc + kAlpha, sc + k);
+      long long* rr_words = next_words();
+      launch("impl_advance_dot", [&] {
+        k_impl_advance_dot<<<gd, kBlock, 0, stream_>>>(du_.get(), kr_.get(), z, w, m, bits_.get() + 1, ww_words,
+                                                      rw_words, sc + k, sc + kAlpha, rr_words);
       });
-      device_dot(kr_.get(), kr_.get(), m, nullptr, sc + kRr);
+      launch("impl_scalar", [&] { k_impl_scalar<<<1, kWave, 0, stream_>>>(rr_words, nullptr, sc + kRr); });
       check(hipMemcpyAsync(host_scalars_, sc, (kR0 + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_),
             "hipMemcpyAsync");
       check(hipStreamSynchronize(stream_), "implicit_solve");
@@ -593,14 +679,10 @@ class Stepper final : public StepperBase {
     return std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
   }
 
-  // sum u v as an exact sum in a fresh word set of the solve, rounded on the device into *out
-  // (divided by *den when given).
-  void device_dot(const double* u, const double* v, long long m, const double* den, double* out) {
+  // A fresh exact-sum word set of the solve (cleared at its start).
+  long long* next_words() {
     if (dot_next_ >= kImplDots) throw std::logic_error("implicit_solve: more inner products than planned");
-    long long* w = words_.get() + std::size_t(dot_next_++) * kWords;
-    const int g = std::max(1, std::min(static_cast<int>((m + kBlock - 1) / kBlock), 1024));
-    launch("impl_dot", [&] { k_impl_dot<<<g, kBlock, 0, stream_>>>(u, v, m, w); });
-    launch("impl_scalar", [&] { k_impl_scalar<<<1, kWave, 0, stream_>>>(w, den, out); });
+    return words_.get() + std::size_t(dot_next_++) * kWords;
   }
 
   bool implicit_ready_ = false, have_jacobian_ = false, jac_second_ = false;
@@ -614,9 +696,10 @@ class Stepper final : public StepperBase {
   Buffer<long long> words_;
   Buffer<unsigned long long> bits_;
   // The GCR's scalars on the device and the pinned host copy of the ones the host reads.
-  static constexpr int kImplScalars = kImplDirections + 4;
+  static constexpr int kImplScalars = kImplDirections + 3;
   Buffer<double> scalars_;
   double* host_scalars_ = nullptr;
+  Buffer<C> tan_;  // the boundary term's primitive tangents
 
  public:
   template <int FLUX>
