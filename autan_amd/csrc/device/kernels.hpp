@@ -371,7 +371,18 @@ __global__ __launch_bounds__(kBlock) void k_grad(Dev<C> d) {
 // bounds and minima run in the same order as k_grad.
 // OCC > 0 asks the compiler for OCC waves per SIMD (registers capped accordingly): a variant for
 // the autotuner, same bits.
-template <class C, bool LIMITER, int OCC = 0>
+// A load of data read once per launch: with NT, marked non-temporal (streamed past the caches'
+// replacement, leaving them to the data other blocks read again); the same value either way.
+template <bool NT, class T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  if constexpr (NT) {
+    return __builtin_nontemporal_load(p);
+  } else {
+    return *p;
+  }
+}
+
+template <class C, bool LIMITER, int OCC = 0, bool NT = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) void k_grad_lds(Dev<C> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   C* tab = reinterpret_cast<C*>(smem);
@@ -412,20 +423,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
   int l_next = 0;
   C w0_next = C(0), w1_next = C(0), w2_next = C(0);
   if (len > 0) {
-    l_next = d.sloc[base];
-    w0_next = d.sw0[base];
-    w1_next = d.sw1[base];
-    w2_next = d.sw2[base];
+    l_next = stream_load<NT>(d.sloc + base);
+    w0_next = stream_load<NT>(d.sw0 + base);
+    w1_next = stream_load<NT>(d.sw1 + base);
+    w2_next = stream_load<NT>(d.sw2 + base);
   }
   for (int j = 0; j < len; ++j) {
     const int l = l_next;
     const C w0 = w0_next, w1 = w1_next, w2 = w2_next;
     if (j + 1 < len) {
       const long long k = base + static_cast<long long>(j + 1) * kWave;
-      l_next = d.sloc[k];
-      w0_next = d.sw0[k];
-      w1_next = d.sw1[k];
-      w2_next = d.sw2[k];
+      l_next = stream_load<NT>(d.sloc + k);
+      w0_next = stream_load<NT>(d.sw0 + k);
+      w1_next = stream_load<NT>(d.sw1 + k);
+      w2_next = stream_load<NT>(d.sw2 + k);
     }
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -455,17 +466,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
     const long long ib = d.ioff[c / kWave] + (c % kWave);
     C r0_next = C(0), r1_next = C(0), r2_next = C(0);
     if (ilen > 0) {
-      r0_next = d.ir0[ib];
-      r1_next = d.ir1[ib];
-      r2_next = d.ir2[ib];
+      r0_next = stream_load<NT>(d.ir0 + ib);
+      r1_next = stream_load<NT>(d.ir1 + ib);
+      r2_next = stream_load<NT>(d.ir2 + ib);
     }
     for (int j = 0; j < ilen; ++j) {
       const C r0 = r0_next, r1 = r1_next, r2 = r2_next;
       if (j + 1 < ilen) {
         const long long k = ib + static_cast<long long>(j + 1) * kWave;
-        r0_next = d.ir0[k];
-        r1_next = d.ir1[k];
-        r2_next = d.ir2[k];
+        r0_next = stream_load<NT>(d.ir0 + k);
+        r1_next = stream_load<NT>(d.ir1 + k);
+        r2_next = stream_load<NT>(d.ir2 + k);
       }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {

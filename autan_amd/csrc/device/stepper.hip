@@ -924,6 +924,12 @@ class Stepper final : public StepperBase {
       grad_occ_ = value;
       return;
     }
+    if (key == "grad_nontemporal") {
+      if (value != 0 && value != 1) throw std::invalid_argument("grad_nontemporal: 0 or 1");
+      drop_graphs();
+      grad_nt_ = value != 0;
+      return;
+    }
     if (key == "face_occupancy") {
       if (value != 0 && value != 5 && value != 8) throw std::invalid_argument("face_occupancy: 0, 5 or 8");
       if (value != 0 && !std::is_same_v<C, float>) throw std::invalid_argument("face_occupancy: needs fp32+acc64");
@@ -1776,7 +1782,17 @@ class Stepper final : public StepperBase {
     if (nb_ > 0) launch("bstate", [&] { k_bstate<C><<<blocks(nb_), kBlock, 0, stream_>>>(d); });
     if (second_) {
       const std::size_t lds = std::size_t(max_table_) * 5 * sizeof(C);
-      if (staged_ && sc_.limiter && grad_occ_ == 8) {
+      if (staged_ && sc_.limiter && grad_nt_) {
+        launch("grad_limiter", [&] {
+          if (grad_occ_ == 8) {
+            k_grad_lds<C, true, 8, true><<<cell_grid(), kBlock, lds, stream_>>>(d);
+          } else if (grad_occ_ == 7) {
+            k_grad_lds<C, true, 7, true><<<cell_grid(), kBlock, lds, stream_>>>(d);
+          } else {
+            k_grad_lds<C, true, 0, true><<<cell_grid(), kBlock, lds, stream_>>>(d);
+          }
+        });
+      } else if (staged_ && sc_.limiter && grad_occ_ == 8) {
         launch("grad_limiter", [&] { k_grad_lds<C, true, 8><<<cell_grid(), kBlock, lds, stream_>>>(d); });
       } else if (staged_ && sc_.limiter && grad_occ_ == 7) {
         launch("grad_limiter", [&] { k_grad_lds<C, true, 7><<<cell_grid(), kBlock, lds, stream_>>>(d); });
@@ -2115,6 +2131,7 @@ class Stepper final : public StepperBase {
   // The gradient / face kernel built for at least this many waves per SIMD (0: the compiler's
   // choice); gradient 7 or 8, face 5 (non-planar meshes) or 8.
   int grad_occ_ = 0, face_occ_ = 0;
+  bool grad_nt_ = false;  // the gradient kernel's stencil stream loaded non-temporally
   Buffer<C> srec_, smom_;
   int face_blocks_ = 0;
   Buffer<int> tbeg_, tent_, lens_;

@@ -286,12 +286,13 @@ struct LayerTable {
 // k_gemm, and the epilogues are k_gemm's (bias + SiLU, then bias and the output scaling of
 // k_out): the outputs are the layer-by-layer path's bits whatever RT, without its HBM round trip
 // of every layer's activations.
-template <int CT, int RT>
-__global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x, int n, int d_in, int d_out,
-                                                    const float* __restrict__ norm, const bf16* __restrict__ wb,
-                                                    const float* __restrict__ params, LayerTable L,
-                                                    float* __restrict__ y) {
-  constexpr int H = 64 * CT, ld = H + kPad, kIn = 32, M = 16 * RT;
+template <int CT, int RT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_infer(const float* __restrict__ x, int n, int d_in, int d_out,
+                                                      const float* __restrict__ norm, const bf16* __restrict__ wb,
+                                                      const float* __restrict__ params, LayerTable L,
+                                                      float* __restrict__ y) {
+  constexpr int kThreads = 64 * WAVES;
+  constexpr int H = 16 * CT * WAVES, ld = H + kPad, kIn = 32, M = 16 * RT;
   extern __shared__ __align__(16) unsigned char infer_smem[];
   bf16* act = reinterpret_cast<bf16*>(infer_smem);
   const long long m0 = static_cast<long long>(blockIdx.x) * M;
@@ -374,28 +375,38 @@ __global__ __launch_bounds__(kThreads) void k_infer(const float* __restrict__ x,
   }
 }
 
-// LDS bytes of k_infer<CT, RT>'s tile.
-constexpr std::size_t infer_lds(int ct, int rt) { return std::size_t(16 * rt) * std::size_t(64 * ct + kPad) * 2; }
+// LDS bytes of the tile of 16 rt samples of width h.
+constexpr std::size_t infer_lds(int h, int rt) { return std::size_t(16 * rt) * std::size_t(h + kPad) * 2; }
 
-// Launches k_infer<CT, RT> for n samples, raising its dynamic LDS limit once per process.
-template <int CT, int RT>
+// Launches k_infer<CT, RT, WAVES> for n samples, raising its dynamic LDS limit once per process.
+template <int CT, int RT, int WAVES>
 void launch_infer(const float* x, int n, int d_in, int d_out, const float* norm, const bf16* wb, const float* params,
                   const LayerTable& L, float* y, hipStream_t s) {
-  static const hipError_t raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<CT, RT>),
+  constexpr int h = 16 * CT * WAVES;
+  static const hipError_t raised = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_infer<CT, RT, WAVES>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       static_cast<int>(infer_lds(CT, RT)));
+                                                       static_cast<int>(infer_lds(h, RT)));
   check(raised, "hipFuncSetAttribute(k_infer)");
   const int g = static_cast<int>((static_cast<long long>(n) + 16 * RT - 1) / (16 * RT));
-  k_infer<CT, RT><<<g, kThreads, infer_lds(CT, RT), s>>>(x, n, d_in, d_out, norm, wb, params, L, y);
+  k_infer<CT, RT, WAVES><<<g, 64 * WAVES, infer_lds(h, RT), s>>>(x, n, d_in, d_out, norm, wb, params, L, y);
 }
 
-template <int CT>
+// The fused forward for hidden width 64 q (q = 1 .. 4) and a tile of `rows` samples: four waves of
+// 16 q columns each (64, 96 or 128 samples), or, for 128 samples and an even q, eight waves of 8 q
+// columns each (half the accumulators per wave, so twice the waves per SIMD).
+template <int Q>
 void launch_infer_rows(int rows, const float* x, int n, int d_in, int d_out, const float* norm, const bf16* wb,
                        const float* params, const LayerTable& L, float* y, hipStream_t s) {
   switch (rows) {
-    case 64: launch_infer<CT, 4>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
-    case 96: launch_infer<CT, 6>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
-    case 128: launch_infer<CT, 8>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    case 64: launch_infer<Q, 4, 4>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    case 96: launch_infer<Q, 6, 4>(x, n, d_in, d_out, norm, wb, params, L, y, s); break;
+    case 128:
+      if constexpr (Q % 2 == 0) {
+        launch_infer<Q / 2, 8, 8>(x, n, d_in, d_out, norm, wb, params, L, y, s);
+      } else {
+        launch_infer<Q, 8, 4>(x, n, d_in, d_out, norm, wb, params, L, y, s);
+      }
+      break;
     default: throw std::invalid_argument("inference tile: 64, 96 or 128 samples per workgroup");
   }
 }
@@ -816,7 +827,7 @@ class Net {
   std::vector<Buf<float>> z_;
   int cap_ = 0;
   bool fused_ = true;
-  int infer_rows_ = 96;  // the fastest tile on an MI355X (2 waves per SIMD)
+  int infer_rows_ = 128;  // the fastest tile on an MI355X (eight waves, 4 waves per SIMD)
   Buf<float> data_x_, data_t_;
   long long n_data_ = 0;
 };

@@ -128,7 +128,7 @@ def run(n_cases: int = 64, seed: int = 1, max_iterations: int = 20000, residual_
 
 def solve_implicit_cases(specs: list[Spec], policy: str = "fp64", flux: str = "hllc", states=None, references=None,
                          max_iterations: int = 500, residual_orders: float = 8.0, start_order: str = "first",
-                         workers: int = 8, chunk: int = 64, finish=None, log=print) -> tuple[list[dict], float]:
+                         workers: int = 4, chunk: int = 64, finish=None, log=print) -> tuple[list[dict], float]:
     """Each case by the steady implicit path (EulerPhysics::steady_step: pseudo-transient continuation,
     GCR and block Gauss-Seidel on the device) on a stepper and stream of its own, `workers` cases
     at a time (host threads: every run is one library call, mxeuler_implicit_steady, which waits
@@ -187,7 +187,7 @@ def solve_implicit_cases(specs: list[Spec], policy: str = "fp64", flux: str = "h
 
 
 def run_implicit(n_cases: int = 64, seed: int = 1, max_iterations: int = 500, residual_orders: float = 8.0,
-                 policy: str = "fp32+acc64", flux: str = "hllc", cells=(2e4, 5e5), workers: int = 8,
+                 policy: str = "fp32+acc64", flux: str = "hllc", cells=(2e4, 5e5), workers: int = 4,
                  log=print) -> dict:
     """The sweep by the steady implicit path, every case from the freestream to a 1e-8 relative
     residual or the fp32+acc64 round-off plateau (solve_implicit_cases, `workers` cases at a time).

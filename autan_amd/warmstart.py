@@ -126,7 +126,7 @@ def solve(specs: list[sweep.Spec], states=None, references=None, max_iterations=
 
 
 def solve_implicit(specs: list[sweep.Spec], states=None, references=None, max_iterations=500, residual_orders=8.0,
-                   policy="fp64", flux="hllc", sample=None, start_order="first", workers=8, log=print, **_):
+                   policy="fp64", flux="hllc", sample=None, start_order="first", workers=4, log=print, **_):
     """As solve, by the steady implicit path (EulerPhysics::steady_step on the device), one stepper
     per case, `workers` cases at a time (sweep.solve_implicit_cases). Wall time counts the solves
     (mesh set-up and upload excluded)."""
@@ -223,7 +223,7 @@ def write_initial_state(path, net, spec: sweep.Spec, policy="fp32+acc64", flux="
 
 def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_case: int = 4096, batch: int = 4096,
                   lr: tuple[float, float] = (1e-3, 1e-5), seed: int = 0, log=print, method: str = "explicit",
-                  policy: str = "fp32+acc64", workers: int = 8):
+                  policy: str = "fp32+acc64", workers: int = 4):
     """The network trained on the training cases' states: each case solved from the freestream
     (explicit: `iterations` steady iterations of the batch stepper; implicit: the steady implicit
     path to a 1e-8 relative residual or its round-off plateau, at most `iterations`), cases that
@@ -256,7 +256,7 @@ def train_network(specs: list[sweep.Spec], iterations: int, epochs: int, per_cas
 
 
 def compare(net, specs: list[sweep.Spec], orders: float, max_iterations: int, log=print, method: str = "explicit",
-            policy: str = "fp32+acc64", workers: int = 8) -> dict:
+            policy: str = "fp32+acc64", workers: int = 4) -> dict:
     """The held-out cases solved from the freestream (first-order start) and from the network's state
     (second order from the first step), the latter both against its own first residual and against
     the freestream run's (the absolute threshold a warm start from another run is held to)."""

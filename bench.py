@@ -7,8 +7,9 @@ band of merged polyhedra with non-planar faces (7,915,215 cells), cells renumber
 curve; inflow Mach 0.5 from the freestream start, slip walls, characteristic farfield; face2
 least-squares gradients with the Venkatakrishnan limiter, SSP-RK3 on local time steps (CFL 0.8).
 One step = one SSP-RK3 step (three residuals, the norms of R(u), the spectral radius and time
-steps, validity checks). The run also reports a steady run's residual drop per second, the
-conservation check and the fp32+acc64 residual's distance from the fp64 one on the same state.
+steps, validity checks). The run also reports the residual drop per second of a steady run from the
+freestream by the explicit steps and by the steady implicit path, the conservation check and the
+fp32+acc64 residual's distance from the fp64 one on the same state.
 
 Other workloads (--workload): `tets`, the manufactured-solution Kuhn tetrahedra (128^3 x 6 =
 12,582,912 cells, jittered, HLLC, global CFL 0.5); `vortex`, the isentropic vortex on 32^3 periodic
@@ -241,7 +242,25 @@ def bump_extras(dev, w, args) -> dict:
         "mass_orders_per_second": round(math.log10(drop[0]) / t, 3) if drop[0] > 0 else None,
         "status": ["converged", "budget", "diverged"][run.status],
     }
-    u_steady = dev.download()
+    c = dev.conservation()
+    out["conservation"] = {
+        "mass_E": c["E"][0],
+        "mass_E_over_D": c["E"][0] / c["D"][0],
+        "E": [float(v) for v in c["E"]],
+        "D": [float(v) for v in c["D"]],
+    }
+    if w.policy == "fp32+acc64" and not args.no_fp64_check:
+        u = dev.download()
+        r32 = dev.residual(u)
+        dev64 = DeviceStepper(w.mesh, w.scheme, "fp64", w.bc_kind, w.bc, w.source)
+        r64 = dev64.residual(u)
+        dev64.close()
+        err = np.abs(r32 - r64).reshape(5, -1) / flux_scale(w)
+        out["fp32_vs_fp64"] = {
+            "state": f"after the {run.iterations}-iteration steady run",
+            "max_err_over_f_A": [float(e) for e in err.max(axis=1)],
+            "max_in_eps32": round(float(err.max() / np.finfo(np.float32).eps), 2),
+        }
     if args.implicit_iterations > 0:
         # The same channel from the same freestream state by the steady implicit path
         # (pseudo-transient continuation, GCR + multicolour block Gauss-Seidel on the device).
@@ -262,26 +281,7 @@ def bump_extras(dev, w, args) -> dict:
             "status": ["converged", "budget", "diverged"][imp.status],
         }
         log(f"implicit steady run: {imp.iterations} iterations in {t:.1f} s, orders {out['steady_implicit']['orders_dropped']}")
-    c = dev.conservation()
-    out["conservation"] = {
-        "mass_E": c["E"][0],
-        "mass_E_over_D": c["E"][0] / c["D"][0],
-        "E": [float(v) for v in c["E"]],
-        "D": [float(v) for v in c["D"]],
-    }
-    if w.policy == "fp32+acc64" and not args.no_fp64_check:
-        u = u_steady
-        r32 = dev.residual(u)
-        dev.close()
-        dev64 = DeviceStepper(w.mesh, w.scheme, "fp64", w.bc_kind, w.bc, w.source)
-        r64 = dev64.residual(u)
-        dev64.close()
-        err = np.abs(r32 - r64).reshape(5, -1) / flux_scale(w)
-        out["fp32_vs_fp64"] = {
-            "state": f"after the {run.iterations}-iteration steady run",
-            "max_err_over_f_A": [float(e) for e in err.max(axis=1)],
-            "max_in_eps32": round(float(err.max() / np.finfo(np.float32).eps), 2),
-        }
+    dev.close()
     return out
 
 
@@ -423,9 +423,21 @@ def warmstart_main(args) -> int:
             "warm_freestream_reference": [wa["result"].iterations,
                                           ["converged", "budget", "diverged"][wa["result"].status]],
             "force_free": [float(v) for v in ff], "force_rel_diff": dw, "force_coeff_diff": dc,
+            "seconds": [f.get("seconds"), wa.get("seconds")],
         })
     it_free = sum(r["freestream"][0] for r in rows)
     it_warm = sum(r["warm_freestream_reference"][0] for r in rows)
+    # The cases both starts bring to the threshold: their iterations, own seconds and loads.
+    both = [r for r in rows if r["freestream"][1] == "converged" and r["warm_freestream_reference"][1] == "converged"]
+    converged = {
+        "cases": len(both),
+        "iterations_freestream": sum(r["freestream"][0] for r in both),
+        "iterations_warm": sum(r["warm_freestream_reference"][0] for r in both),
+        "max_force_coeff_diff": max((r["force_coeff_diff"] for r in both), default=None),
+    }
+    if both and all(r["seconds"][0] is not None for r in both):
+        converged["seconds_freestream"] = round(sum(r["seconds"][0] for r in both), 2)
+        converged["seconds_warm"] = round(sum(r["seconds"][1] for r in both), 2)
     line = {
         "metric": "wall time to the residual threshold, learned vs freestream",
         "value": round(t_free / (t_warm_abs + t_infer), 3),
@@ -448,7 +460,8 @@ def warmstart_main(args) -> int:
                  "warm_s_freestream_reference": round(t_warm_abs, 2), "inference_s": round(t_infer, 2),
                  "iterations_freestream": it_free, "iterations_warm": it_warm,
                  "max_force_rel_diff": max(r["force_rel_diff"] for r in rows),
-                 "max_force_coeff_diff": max(r["force_coeff_diff"] for r in rows)},
+                 "max_force_coeff_diff": max(r["force_coeff_diff"] for r in rows),
+                 "converged_from_both": converged},
         "cases": rows,
         "total_s": round(time.time() - t0, 1),
     }
@@ -568,7 +581,7 @@ def main() -> int:
                     help="sweep / warmstart: the batch stepper's explicit SSP-RK3 steps, or the steady implicit path "
                     "(--max-iterations is then its iteration budget per case)")
     ap.add_argument("--cases", type=int, default=64, help="sweep: number of cases")
-    ap.add_argument("--workers", type=int, default=8,
+    ap.add_argument("--workers", type=int, default=4,
                     help="sweep / warmstart with --method implicit: cases solved at a time (one stepper and stream each)")
     ap.add_argument("--max-iterations", type=int, default=None,
                     help="sweep / warmstart / mms: iteration budget per case (default 20000, 600 with --method implicit)")
@@ -582,7 +595,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--steady-iterations", type=int, default=400)
-    ap.add_argument("--implicit-iterations", type=int, default=0,
+    ap.add_argument("--implicit-iterations", type=int, default=60,
                     help="bump: budget of a steady implicit run from the freestream after the explicit one (0: none)")
     ap.add_argument("--order", default="morton", choices=["morton", "rcm"], help="cell renumbering of the set-up")
     ap.add_argument("--tune", default="", help="kernel variants, key=value,... (face_stage, graph, grad_occupancy, "

@@ -51,9 +51,10 @@ int mxmlp_time_forward(mxmlp* h, int32_t n, int32_t repeats, double* ms, char* e
  * the default; hidden widths up to 256), or by the training path's layer-by-layer products (0). The
  * outputs are the same bits. */
 int mxmlp_set_fused(mxmlp* h, int32_t on, char* err, size_t err_len);
-/* Samples per workgroup of the fused forward (64, 96 or 128; default 96): each workgroup reads
- * every weight once for its samples, so more samples per workgroup read fewer L2 bytes per product.
- * The outputs are the same bits whatever the tile. */
+/* Samples per workgroup of the fused forward (64, 96 or 128; default 128): each workgroup reads
+ * every weight once for its samples, so more samples per workgroup read fewer L2 bytes per product
+ * (128 with eight waves when the hidden width is a multiple of 128). The outputs are the same bits
+ * whatever the tile. */
 int mxmlp_set_infer_rows(mxmlp* h, int32_t rows, char* err, size_t err_len);
 
 #ifdef __cplusplus
