@@ -117,12 +117,13 @@ def test_step_matches_oracle(name, policy, local):
 
 @pytest.mark.parametrize("variant", [("face_stage", 1), ("grad_occupancy", 7), ("grad_occupancy", 8),
                                      ("face_occupancy", 5), ("face_occupancy", 8), ("face_block", 512),
-                                     ("face_block", 256)])
+                                     ("face_block", 256), ("grad_nontemporal", 0)])
 @pytest.mark.parametrize("local", [False, True])
 @pytest.mark.parametrize("name", ["tets-hllc2", "poly-ausm2-warped", "bump-hybrid-hllc2"])
 def test_kernel_variants_give_the_same_bits(name, local, variant):
     """The autotuner's kernel variants (the face kernel staging its block's cells in LDS, the
-    gradient and face kernels built for more waves per SIMD, smaller face blocks) compute the same
+    gradient and face kernels built for more waves per SIMD, smaller face blocks, the gradient's
+    stencil stream loaded non-temporally or not) compute the same
     residual, norms and state as the default kernels, bit for bit (fp32+acc64)."""
     case = CASES[name]()
     plain, _ = build(case, "fp32+acc64")
