@@ -288,6 +288,20 @@ def test_periodic_surface_links_become_periodic_faces(encoding):
     np.add.at(closed, m.neighbour, -av[:, : m.n_interior].T)
     assert np.abs(closed).max() < 1e-15
     assert abs(m.cell_volume.sum() - 1.0) < 1e-13
+    # Every interior face, periodic ones included, lies between its cells as the owner sees them:
+    # the owner behind it (n . r_left > 0), the neighbour's image (its centroid plus the face's
+    # shift) in front (n . r_right < 0), both within about a cell; a shift of the wrong sign puts
+    # the image two periods off.
+    ni = m.n_interior
+    xf = m.face_centroids()[:ni]
+    xc = m.centroids()
+    n = av[:, :ni].T
+    r_left = xf - xc[m.owner[:ni]]
+    r_right = xf - (xc[m.neighbour[:ni]] + shift[:, :ni].T)
+    assert np.all(np.einsum("ij,ij->i", n, r_left) > 0)
+    assert np.all(np.einsum("ij,ij->i", n, r_right) < 0)
+    h = np.cbrt(m.cell_volume.max())
+    assert np.linalg.norm(r_right[moved[moved < ni]], axis=1).max() < 2 * h
 
 
 def test_a_uniform_flow_is_steady_on_the_periodic_box():
