@@ -403,7 +403,8 @@ class DeviceStepper:
 
     def tune(self, key: str, value: int) -> None:
         """Selects a kernel variant (same bits, different speed): "face_stage" 0/1, "graph" 0/1,
-        "grad_occupancy" 0/7/8, "face_occupancy" 0/5/8, "face_block" 256/512/1024."""
+        "grad_occupancy" 0/7/8, "face_occupancy" 0/5/8, "face_block" 256/512/1024, "grad_nontemporal"
+        0/1 (default 1)."""
         self._call(self.lib.mxeuler_tune, ctypes.c_char_p(key.encode()), _i32(value))
 
     def conservation(self) -> dict[str, np.ndarray]:

@@ -165,6 +165,7 @@ def solve_implicit_cases(specs: list[Spec], policy: str = "fp64", flux: str = "h
             ks = range(b0, min(b0 + chunk, len(specs)))
             made = [pool.submit(setup, k) for k in ks]
             built = []
+            runs = []
             try:
                 for f in made:
                     built.append(f.result())
@@ -178,7 +179,9 @@ def solve_implicit_cases(specs: list[Spec], policy: str = "fp64", flux: str = "h
                         finish(k, c, dev, r)
                     out.append({"result": r, "cells": c.mesh.n_cells, "seconds": own})
             finally:
+                # no stepper closes while a solve of the chunk may still use it
                 wait(made)
+                wait(runs)
                 for f in made:
                     if f.done() and f.exception() is None:
                         f.result()[1].close()

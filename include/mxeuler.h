@@ -194,7 +194,8 @@ int mxeuler_loads(mxeuler_stepper* s, const int32_t* faces, int64_t n, const dou
  * of their launches; "grad_occupancy" (0, 7 or 8) and "face_occupancy" (0, 5 or 8): the gradient and
  * face kernels built for at least that many waves per SIMD (0: the compiler's choice; 5 for
  * non-planar meshes only); "face_block" (256, 512 or 1024): the most faces a face block holds (the
- * blocks are rebuilt). An unknown key or a variant this stepper cannot run is an error. */
+ * blocks are rebuilt); "grad_nontemporal" (0/1, default 1): the gradient kernel loads its stencil
+ * stream non-temporally. An unknown key or a variant this stepper cannot run is an error. */
 int mxeuler_tune(mxeuler_stepper* s, const char* key, int32_t value, char* err, size_t err_len);
 /* EulerPhysics::check_conservation (physics/src/euler_physics.cpp:1750-1784; docs/design/euler.md,
  * Conservation check) on the current state: per equation k, exact sums over the cells and faces,
