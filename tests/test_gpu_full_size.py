@@ -9,7 +9,9 @@ freestream (so that gradients and limiter act in every cell). What does not depe
   threads;
 - with the farfield faces, the fp32+acc64 residual lies within the reference's 500 eps32 f_k A_i of
   the fp64 one on the same state, conservation E stays within its round-off bound, and repeated
-  runs give the same bits."""
+  runs give the same bits.
+The manufactured-solution tetrahedra at their largest level (12.6M cells) too: the device residual
+is the oracle's bit for bit."""
 
 from __future__ import annotations
 
@@ -121,3 +123,23 @@ def test_full_size_runs_repeat_their_bits(channel, stepper):
         runs.append((np.array(norms), stepper.download()))
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
+def test_full_size_tetrahedra_residual_is_the_oracles():
+    """The manufactured-solution tetrahedra at their largest level (128^3 x 6 = 12,582,912 jittered
+    Kuhn tetrahedra, sources, the exact state on the boundary), second-order HLLC with the limiter
+    (K = 5), fp32+acc64: the device residual of the manufactured state equals the oracle's bit for
+    bit."""
+    print("full-size tetrahedra 128^3 x 6: building the mesh", flush=True)
+    w = workloads.tets(128, "fp32+acc64")
+    dev = DeviceStepper(w.mesh, w.scheme, "fp32+acc64", w.bc_kind, w.bc, w.source)
+    r_dev = dev.residual(w.u)
+    dev.close()
+    print("full-size tetrahedra: oracle residual", flush=True)
+    orc.set_threads(min(16, os.cpu_count() or 1))
+    try:
+        r_ora = Oracle(w.mesh, w.scheme, "fp32+acc64", w.bc_kind, w.bc, w.source).residual(w.u)
+    finally:
+        orc.set_threads(1)
+    np.testing.assert_array_equal(r_dev, r_ora)
+    assert np.abs(r_dev).max() > 0.0
