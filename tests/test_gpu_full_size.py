@@ -11,7 +11,8 @@ freestream (so that gradients and limiter act in every cell). What does not depe
   the fp64 one on the same state, conservation E stays within its round-off bound, and repeated
   runs give the same bits.
 The manufactured-solution tetrahedra at their largest level (12.6M cells) too: the device residual
-is the oracle's bit for bit."""
+is the oracle's bit for bit; and the 64-case sweep in one batch: each case's steps are its solo
+steps' bits."""
 
 from __future__ import annotations
 
@@ -143,3 +144,33 @@ def test_full_size_tetrahedra_residual_is_the_oracles():
         orc.set_threads(1)
     np.testing.assert_array_equal(r_dev, r_ora)
     assert np.abs(r_dev).max() > 0.0
+
+
+def test_full_size_sweep_batch_steps_are_each_cases_steps():
+    """The 64-case sweep at its full size (seed 1: 6,496,211 cells of 20k-500k-cell channels) in
+    one batch stepper: three SSP-RK3 steps on local time steps give the smallest, the largest and
+    two transonic cases the norms and states they get stepped alone (fp32+acc64, bit for bit)."""
+    from autan_amd import batch, sweep
+
+    print("full-size sweep: building 64 channels", flush=True)
+    specs = sweep.specs(64, seed=1)
+    cs = [sweep.build(s, "fp32+acc64") for s in specs]
+    sizes = [c.mesh.n_cells for c in cs]
+    assert sum(sizes) == 6496211
+    pick = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 10, 59})
+    b = batch.BatchStepper(cs, "fp32+acc64")
+    b.upload()
+    alone = {j: DeviceStepper(cs[j].mesh, cs[j].scheme, "fp32+acc64", cs[j].bc_kind, cs[j].bc) for j in pick}
+    for j, d in alone.items():
+        d.upload(cs[j].u)
+    for _ in range(3):
+        r, h, inv = b.step(cfl=0.8, local=True)
+        for j, d in alone.items():
+            s = d.step(cfl=0.8, local=True)
+            np.testing.assert_array_equal(r[j], s.residuals)
+            assert inv[j] is None and s.invalid is None
+    got = b.download()
+    for j, d in alone.items():
+        np.testing.assert_array_equal(got[j], d.download())
+        d.close()
+    b.close()
