@@ -18,9 +18,12 @@
 //                             keeps its density and pressure above half their values, the least over
 //                             the cells (an exact minimum of the FP64 values);
 // and the GCR's vector updates (solvers/src/gcr.cpp). Everything in FP64, no contraction, so every
-// value equals the host restatement's. Matrix layout: diagonal and inverse blocks 25 x n SoA
-// (entry 5 i + j of cell c at (5 i + j) n + c), off-diagonal blocks 25 contiguous values per block,
-// the upper (row owner) blocks of the interior faces first, then the lower (row neighbour) ones.
+// value equals the host restatement's. Layouts (chosen for the relaxation, which streams a colour's
+// rows): the rows are numbered by position in the colour-major cell order (the cells of colour 0
+// ascending, then colour 1's, ...); the diagonal and inverse blocks are 25 contiguous values per
+// position; a row's off-diagonal blocks are its entries, 25 contiguous values each, so a colour's
+// rows and blocks are one contiguous run; the solve's vectors (the right-hand side, du, the GCR's
+// directions) hold a cell's five values together, value q of cell c at 5 c + q.
 #pragma once
 
 #include "exact.hpp"
@@ -54,7 +57,8 @@ __device__ __forceinline__ void impl_block(const double* __restrict__ v, long lo
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __restrict__ dwdu,
                                                       const double* __restrict__ fj, const double* __restrict__ bj,
-                                                      double cfl, double* __restrict__ diag) {
+                                                      double cfl, const int* __restrict__ rank,
+                                                      double* __restrict__ diag) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= d.nc) return;
   double dg[25];
@@ -80,15 +84,17 @@ __global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __
       impl_add_product(dg, j, dc, -1.0);
     }
   }
+  double* out = diag + 25LL * rank[c];
 #pragma unroll
-  for (int q = 0; q < 25; ++q) diag[static_cast<long long>(q) * d.nc + c] = dg[q];
+  for (int q = 0; q < 25; ++q) out[q] = dg[q];
 }
 
-// The off-diagonal blocks of every interior face: offd[25 f ..] = J_R D_neighbour (row owner),
-// offd[25 (ni + f) ..] = -J_L D_owner (row neighbour).
+// The off-diagonal blocks of every interior face, at their entries (at[f].x in the owner's row,
+// at[f].y in the neighbour's): J_R D_neighbour (row owner), -J_L D_owner (row neighbour).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __restrict__ dwdu,
-                                                     const double* __restrict__ fj, double* __restrict__ offd) {
+                                                     const double* __restrict__ fj, const int2* __restrict__ at,
+                                                     double* __restrict__ offd) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= d.ni) return;
   const int2 lr = d.lr[f];
@@ -98,7 +104,8 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
   impl_add_product(out, j, dm, 1.0);
-  double* up = offd + 25LL * f;
+  const int2 k = at[f];
+  double* up = offd + 25LL * k.x;
 #pragma unroll
   for (int q = 0; q < 25; ++q) up[q] = out[q];
   impl_block(fj, d.ni, f, 0, j);
@@ -106,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
   impl_add_product(out, j, dm, -1.0);
-  double* lo = offd + 25LL * (d.ni + f);
+  double* lo = offd + 25LL * k.y;
 #pragma unroll
   for (int q = 0; q < 25; ++q) lo[q] = out[q];
 }
@@ -114,17 +121,19 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
 // Each diagonal block's inverse: Gauss-Jordan on [A | I] with partial pivoting (the first row
 // holding the largest |pivot| below the diagonal, rows swapped whole, the pivot row scaled by the
 // pivot's reciprocal, every other row reduced). A zero or non-finite pivot marks the cell in
-// *singular (the least such cell).
-__global__ __launch_bounds__(kBlock) void k_impl_invert(const double* __restrict__ diag, int n,
-                                                        double* __restrict__ inverse, int* __restrict__ singular) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= n) return;
+// *singular (the least such cell). One thread per position p, cell[p] its cell.
+__global__ __launch_bounds__(kBlock) void k_impl_invert(const double* __restrict__ diag, const int* __restrict__ cell,
+                                                        int n, double* __restrict__ inverse,
+                                                        int* __restrict__ singular) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const double* dp = diag + 25LL * p;
   double m[5][10];
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
-      m[i][j] = diag[static_cast<long long>(5 * i + j) * n + c];
+      m[i][j] = dp[5 * i + j];
       m[i][5 + j] = i == j ? 1.0 : 0.0;
     }
   }
@@ -160,19 +169,22 @@ __global__ __launch_bounds__(kBlock) void k_impl_invert(const double* __restrict
       for (int j = 0; j < 10; ++j) m[i][j] -= f * m[k][j];
     }
   }
-  if (!ok) atomicMin(singular, c);
+  if (!ok) atomicMin(singular, cell[p]);
+  double* ip = inverse + 25LL * p;
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
 #pragma unroll
-    for (int j = 0; j < 5; ++j) inverse[static_cast<long long>(5 * i + j) * n + c] = m[i][5 + j];
+    for (int j = 0; j < 5; ++j) ip[5 * i + j] = m[i][5 + j];
   }
 }
 
-// One row's entries: the cell's interior faces in ascending face order, each as (its block's index
-// in offd, the other cell).
+// The rows by position p (colour-major): cell[p] the row's cell, its entries off[p] .. off[p + 1]
+// (the cell's interior faces in ascending face order), entry k's block at offd + 25 k and its other
+// cell other[k].
 struct ImplRows {
   const int* off;    // n + 1
-  const int2* ent;   // 2 ni
+  const int* other;  // 2 ni
+  const int* cell;   // n
 };
 
 // BlockMatrix::relax of one cell by a group of five lanes, lane q computing row q: x_c = D_c^-1
@@ -181,20 +193,21 @@ struct ImplRows {
 // wave holds 12 groups (lanes 60-63 idle); every lane takes part in the shuffles.
 constexpr int kRelaxGroups = kWave / 5;
 
-__device__ __forceinline__ void impl_relax_rows(const ImplRows& rows, int c, bool active, int n,
+__device__ __forceinline__ void impl_relax_rows(const ImplRows& rows, int p, bool active,
                                                 const double* __restrict__ offd, const double* __restrict__ inverse,
                                                 const double* __restrict__ b, double* __restrict__ x) {
   const int lane = threadIdx.x % kWave;
   const int q = lane % 5, base = lane - q;
+  const long long c = active ? rows.cell[p] : 0;
   double r = 0.0;
   if (active) {
-    r = b[static_cast<long long>(q) * n + c];
-    for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
-      const int2 e = rows.ent[k];
-      const double* a = offd + 25LL * e.x + 5 * q;
+    r = b[5 * c + q];
+    for (int k = rows.off[p]; k < rows.off[p + 1]; ++k) {
+      const double* a = offd + 25LL * k + 5 * q;
+      const long long o = rows.other[k];
       double xo[5];
 #pragma unroll
-      for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
+      for (int j = 0; j < 5; ++j) xo[j] = x[5 * o + j];
 #pragma unroll
       for (int j = 0; j < 5; ++j) r -= a[j] * xo[j];
     }
@@ -203,15 +216,17 @@ __device__ __forceinline__ void impl_relax_rows(const ImplRows& rows, int c, boo
 #pragma unroll
   for (int j = 0; j < 5; ++j) rj[j] = __shfl(r, (base + j) % kWave);
   if (active) {
+    const double* ip = inverse + 25LL * p + 5 * q;
     double s = 0.0;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) s += inverse[static_cast<long long>(5 * q + j) * n + c] * rj[j];
-    x[static_cast<long long>(q) * n + c] = s;
+    for (int j = 0; j < 5; ++j) s += ip[j] * rj[j];
+    x[5 * c + q] = s;
   }
 }
 
-// BlockMatrix::relax of the cells cells[0 .. count) (one colour): 12 cells per wave.
-__global__ __launch_bounds__(kBlock) void k_impl_relax(ImplRows rows, const int* __restrict__ cells, int count, int n,
+// BlockMatrix::relax of the rows at positions first .. first + count (one colour): 12 cells per
+// wave.
+__global__ __launch_bounds__(kBlock) void k_impl_relax(ImplRows rows, int first, int count,
                                                        const double* __restrict__ offd,
                                                        const double* __restrict__ inverse,
                                                        const double* __restrict__ b, double* __restrict__ x) {
@@ -219,7 +234,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_relax(ImplRows rows, const int*
   const int g = lane / 5;
   const int t = (blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave)) * kRelaxGroups + g;
   const bool active = g < kRelaxGroups && t < count;
-  impl_relax_rows(rows, active ? cells[t] : 0, active, n, offd, inverse, b, x);
+  impl_relax_rows(rows, first + t, active, offd, inverse, b, x);
 }
 
 // Launch width of k_impl_relax for `count` cells.
@@ -228,8 +243,8 @@ inline int impl_relax_blocks(int count) {
   return (count + per_block - 1) / per_block;
 }
 
-// A run of small colours relaxed in order by one workgroup: phase p relaxes cells[first[p] ..
-// first[p] + count[p]), the barrier between phases making each phase's x visible to the next (the
+// A run of small colours relaxed in order by one workgroup: phase p relaxes the rows at positions
+// first[p] .. first[p] + count[p], the barrier between phases making each phase's x visible to the next (the
 // workgroup shares one CU's cache), as separate launches would. Saves a launch per colour where a
 // colour is too small to fill the device.
 constexpr int kRelaxRunThreads = 1024;
@@ -242,8 +257,7 @@ struct ImplRun {
   int phases;
 };
 
-__global__ __launch_bounds__(kRelaxRunThreads) void k_impl_relax_run(ImplRows rows, const int* __restrict__ cells,
-                                                                     ImplRun run, int n,
+__global__ __launch_bounds__(kRelaxRunThreads) void k_impl_relax_run(ImplRows rows, ImplRun run,
                                                                      const double* __restrict__ offd,
                                                                      const double* __restrict__ inverse,
                                                                      const double* __restrict__ b,
@@ -257,41 +271,33 @@ __global__ __launch_bounds__(kRelaxRunThreads) void k_impl_relax_run(ImplRows ro
     for (int t0 = 0; t0 < run.count[p]; t0 += per_pass) {
       const int t = t0 + slot;
       const bool active = g < kRelaxGroups && t < run.count[p];
-      impl_relax_rows(rows, active ? cells[run.first[p] + t] : 0, active, n, offd, inverse, b, x);
+      impl_relax_rows(rows, run.first[p] + t, active, offd, inverse, b, x);
     }
     __syncthreads();
   }
 }
 
-// BlockMatrix::multiply: y_c = D_c x_c + sum_f A_cf x_other, the diagonal block first.
+// BlockMatrix::multiply: y_c = D_c x_c + sum_f A_cf x_other, the diagonal block first; five lanes
+// per row position (12 per wave), lane q summing row q in the one-thread order.
 __global__ __launch_bounds__(kBlock) void k_impl_multiply(ImplRows rows, int n, const double* __restrict__ diag,
                                                           const double* __restrict__ offd,
                                                           const double* __restrict__ x, double* __restrict__ y) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= n) return;
-  double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  double xc[5];
+  const int lane = threadIdx.x % kWave;
+  const int g = lane / 5, q = lane % 5;
+  const int p = (blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave)) * kRelaxGroups + g;
+  if (g >= kRelaxGroups || p >= n) return;
+  const long long c = rows.cell[p];
+  const double* dp = diag + 25LL * p + 5 * q;
+  double r = 0.0;
 #pragma unroll
-  for (int j = 0; j < 5; ++j) xc[j] = x[static_cast<long long>(j) * n + c];
+  for (int j = 0; j < 5; ++j) r += dp[j] * x[5 * c + j];
+  for (int k = rows.off[p]; k < rows.off[p + 1]; ++k) {
+    const double* a = offd + 25LL * k + 5 * q;
+    const long long o = rows.other[k];
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
-#pragma unroll
-    for (int j = 0; j < 5; ++j) r[i] += diag[static_cast<long long>(5 * i + j) * n + c] * xc[j];
+    for (int j = 0; j < 5; ++j) r += a[j] * x[5 * o + j];
   }
-  for (int k = rows.off[c]; k < rows.off[c + 1]; ++k) {
-    const int2 e = rows.ent[k];
-    const double* a = offd + 25LL * e.x;
-    double xo[5];
-#pragma unroll
-    for (int j = 0; j < 5; ++j) xo[j] = x[static_cast<long long>(j) * n + e.y];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-#pragma unroll
-      for (int j = 0; j < 5; ++j) r[i] += a[5 * i + j] * xo[j];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 5; ++i) y[static_cast<long long>(i) * n + c] = r[i];
+  y[5 * c + q] = r;
 }
 
 // A block's exact-sum accumulator (carries resolved) added into the global words.
@@ -468,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_admissible(const double* __rest
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       v0[k] = u[static_cast<long long>(k) * n + c];
-      dv[k] = du[static_cast<long long>(k) * n + c];
+      dv[k] = du[5LL * c + k];
     }
     double a = 1.0;
     if (dv[0] < 0.0) {
@@ -497,7 +503,7 @@ __device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, co
                                              long long c, C* dw) {
   C vc[5];
 #pragma unroll
-  for (int j = 0; j < 5; ++j) vc[j] = static_cast<C>(v[j * n + c]);
+  for (int j = 0; j < 5; ++j) vc[j] = static_cast<C>(v[5 * c + j]);
 #pragma unroll
   for (int i = 0; i < 5; ++i) {
     C t = C(0);
@@ -597,19 +603,18 @@ __global__ __launch_bounds__(kBlock) void k_impl_bgrad(Dev<C> d, const int* __re
     }
   }
 #pragma unroll
-  for (int i = 0; i < 5; ++i) y[i * n + c] += dres[i];
+  for (int i = 0; i < 5; ++i) y[5 * static_cast<long long>(c) + i] += dres[i];
 }
 
 // ---- the GCR's vector operations (solvers/src/gcr.cpp), m values each
 
-// y = -x
+// y = -x, x as the residual holds it (value q of cell c at q n + c), y cell-major (at 5 c + q).
 __global__ __launch_bounds__(kBlock) void k_impl_negate(const double* __restrict__ x, double* __restrict__ y,
-                                                        long long m) {
+                                                        long long n) {
   const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < m) y[i] = -x[i];
+  if (i < 5 * n) y[i] = -x[(i % 5) * n + i / 5];
 }
 
-// x = 0
 __global__ __launch_bounds__(kBlock) void k_impl_zero(double* __restrict__ x, long long m) {
   const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
   if (i < m) x[i] = 0.0;
@@ -630,11 +635,11 @@ __global__ void k_impl_scalar(const long long* __restrict__ words, const double*
   *out = den != nullptr ? v / *den : v;
 }
 
-// u += alpha du
+// u += alpha du, u as the state holds it (value q of cell c at q n + c), du cell-major.
 __global__ __launch_bounds__(kBlock) void k_impl_update(double* __restrict__ u, const double* __restrict__ du,
-                                                        long long m, double alpha) {
+                                                        long long n, double alpha) {
   const long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i < m) u[i] += alpha * du[i];
+  if (i < 5 * n) u[i] += alpha * du[5 * (i % n) + i / n];
 }
 
 }  // namespace mxe

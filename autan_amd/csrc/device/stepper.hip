@@ -347,12 +347,19 @@ class Stepper final : public StepperBase {
     dot_next_ = 0;
     launch("impl_clear", [&] { k_impl_clear<<<1, 1024, 0, stream_>>>(words_.get(), kImplDots, bits_.get(), flag_.get()); });
     launch("impl_diag", [&] {
-      k_impl_diag<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl, diag_.get());
+      k_impl_diag<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl, rank_.get(),
+                                                          diag_.get());
     });
-    if (ni_ > 0) launch("impl_off", [&] { k_impl_off<C><<<blocks(ni_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), offd_.get()); });
-    launch("impl_invert", [&] { k_impl_invert<<<blocks(nc_), kBlock, 0, stream_>>>(diag_.get(), nc_, inv_.get(), flag_.get()); });
+    if (ni_ > 0) {
+      launch("impl_off", [&] {
+        k_impl_off<C><<<blocks(ni_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), rat_.get(), offd_.get());
+      });
+    }
+    launch("impl_invert", [&] {
+      k_impl_invert<<<blocks(nc_), kBlock, 0, stream_>>>(diag_.get(), ccells_.get(), nc_, inv_.get(), flag_.get());
+    });
     launch("impl_zero", [&] { k_impl_zero<<<gm, kBlock, 0, stream_>>>(du_.get(), m); });
-    launch("impl_negate", [&] { k_impl_negate<<<gm, kBlock, 0, stream_>>>(res_.get(), kr_.get(), m); });
+    launch("impl_negate", [&] { k_impl_negate<<<gm, kBlock, 0, stream_>>>(res_.get(), kr_.get(), nc_); });
     int singular = INT32_MAX;
     check(hipMemcpyAsync(&singular, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream_), "hipMemcpyAsync");
     check(hipStreamSynchronize(stream_), "implicit_solve");
@@ -381,7 +388,7 @@ class Stepper final : public StepperBase {
     std::memcpy(&alpha, &abits, sizeof alpha);
     report[2] = alpha;
     if (alpha < kImplMinStep) return;
-    launch("impl_update", [&] { k_impl_update<<<gm, kBlock, 0, stream_>>>(u_.get(), du_.get(), m, alpha); });
+    launch("impl_update", [&] { k_impl_update<<<gm, kBlock, 0, stream_>>>(u_.get(), du_.get(), nc_, alpha); });
     prim_fresh_ = false;
     check(hipStreamSynchronize(stream_), "implicit_solve");
   }
@@ -397,13 +404,13 @@ class Stepper final : public StepperBase {
     check(hipStreamSynchronize(stream_), "implicit_worst_cell");
     auto pressure = [&](std::size_t c, double a) {
       double v[5];
-      for (std::size_t k = 0; k < 5; ++k) v[k] = u[k * n + c] + a * du[k * n + c];
+      for (std::size_t k = 0; k < 5; ++k) v[k] = u[k * n + c] + a * du[5 * c + k];  // du cell-major
       return (sc_.gamma - 1.0) * (v[4] - 0.5 * (v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) / v[0]);
     };
     std::size_t worst = 0;
     double lowest = std::numeric_limits<double>::infinity();
     for (std::size_t c = 0; c < n; ++c) {
-      const double r = std::min((u[c] + du[c]) / u[c], pressure(c, 1.0) / pressure(c, 0.0));
+      const double r = std::min((u[c] + du[5 * c]) / u[c], pressure(c, 1.0) / pressure(c, 0.0));
       if (r < lowest) {
         lowest = r;
         worst = c;
@@ -478,8 +485,30 @@ class Stepper final : public StepperBase {
     }
     nbo_ = static_cast<int>(bo.size());
     if (nbo_ > 0) bocells_.upload(bo, stream_);
-    roff_.upload(off, stream_);
-    rent_.upload(ent, stream_);
+    // The rows in the colour-major order of `cells`: row p is cell cells[p]'s, its entries (the same
+    // ascending faces) numbered consecutively, so that a colour's rows and their blocks are one
+    // contiguous run; each interior face's two blocks sit at its entries in the two rows.
+    std::vector<int> rank(std::max<std::size_t>(n, 1)), poff(n + 1, 0);
+    for (std::size_t p = 0; p < n; ++p) {
+      const auto c = static_cast<std::size_t>(cells[p]);
+      rank[c] = static_cast<int>(p);
+      poff[p + 1] = poff[p] + (off[c + 1] - off[c]);
+    }
+    std::vector<int> other(std::max<std::size_t>(2 * ni, 1));
+    std::vector<int2> face_at(std::max<std::size_t>(ni, 1));
+    std::vector<int> pfill(poff.begin(), poff.end() - 1);
+    for (std::size_t f = 0; f < ni; ++f) {
+      const int o = owner_host_[f], nb = neighbour_host_[f];
+      const int ko = pfill[static_cast<std::size_t>(rank[static_cast<std::size_t>(o)])]++;
+      const int kn = pfill[static_cast<std::size_t>(rank[static_cast<std::size_t>(nb)])]++;
+      other[static_cast<std::size_t>(ko)] = nb;
+      other[static_cast<std::size_t>(kn)] = o;
+      face_at[f] = make_int2(ko, kn);
+    }
+    roff_.upload(poff, stream_);
+    rother_.upload(other, stream_);
+    rat_.upload(face_at, stream_);
+    rank_.upload(rank, stream_);
     ccells_.upload(cells, stream_);
     plan_relax();
     const std::size_t m = 5 * n;
@@ -508,17 +537,16 @@ class Stepper final : public StepperBase {
   // order (BlockMatrix::gauss_seidel), as the launches of relax_plan_.
   void gauss_seidel(const double* b, double* z, long long m, int gm) {
     launch("impl_zero", [&] { k_impl_zero<<<gm, kBlock, 0, stream_>>>(z, m); });
-    const ImplRows rows{roff_.get(), rent_.get()};
+    const ImplRows rows = impl_rows();
     for (const RelaxLaunch& l : relax_plan_) {
       if (l.run.phases == 0) {
         launch("impl_relax", [&] {
-          k_impl_relax<<<impl_relax_blocks(l.count), kBlock, 0, stream_>>>(rows, ccells_.get() + l.first, l.count, nc_,
-                                                                           offd_.get(), inv_.get(), b, z);
+          k_impl_relax<<<impl_relax_blocks(l.count), kBlock, 0, stream_>>>(rows, l.first, l.count, offd_.get(),
+                                                                           inv_.get(), b, z);
         });
       } else {
         launch("impl_relax_run", [&] {
-          k_impl_relax_run<<<1, kRelaxRunThreads, 0, stream_>>>(rows, ccells_.get(), l.run, nc_, offd_.get(),
-                                                                inv_.get(), b, z);
+          k_impl_relax_run<<<1, kRelaxRunThreads, 0, stream_>>>(rows, l.run, offd_.get(), inv_.get(), b, z);
         });
       }
     }
@@ -600,9 +628,9 @@ class Stepper final : public StepperBase {
       double* z = zs_.get() + std::size_t(k) * std::size_t(m);
       double* w = ws_.get() + std::size_t(k) * std::size_t(m);
       gauss_seidel(kr_.get(), z, m, gm);
-      const ImplRows rows{roff_.get(), rent_.get()};
+      const ImplRows rows = impl_rows();
       launch("impl_multiply", [&] {
-        k_impl_multiply<<<blocks(nc_), kBlock, 0, stream_>>>(rows, nc_, diag_.get(), offd_.get(), z, w);
+        k_impl_multiply<<<impl_relax_blocks(nc_), kBlock, 0, stream_>>>(rows, nc_, diag_.get(), offd_.get(), z, w);
       });
       // At second order the operator adds the boundary faces' gradient term.
       if (jac_second_ && nbo_ > 0) {
@@ -679,6 +707,8 @@ class Stepper final : public StepperBase {
     return std::isfinite(rr) ? std::sqrt(rr) / r0 : std::numeric_limits<double>::quiet_NaN();
   }
 
+  ImplRows impl_rows() const { return ImplRows{roff_.get(), rother_.get(), ccells_.get()}; }
+
   // A fresh exact-sum word set of the solve (cleared at its start).
   long long* next_words() {
     if (dot_next_ >= kImplDots) throw std::logic_error("implicit_solve: more inner products than planned");
@@ -690,8 +720,8 @@ class Stepper final : public StepperBase {
   Buffer<int> bocells_;
   int dot_next_ = 0;
   std::vector<int> colour_host_, colour_off_;
-  Buffer<int> roff_, ccells_, flag_;
-  Buffer<int2> rent_;
+  Buffer<int> roff_, rother_, rank_, ccells_, flag_;
+  Buffer<int2> rat_;  // each interior face's two entries (owner's row, neighbour's row)
   Buffer<double> jd_, jf_, jb_, diag_, inv_, offd_, kr_, du_, zs_, ws_;
   Buffer<long long> words_;
   Buffer<unsigned long long> bits_;
