@@ -33,18 +33,23 @@ namespace mxe {
 
 static_assert(kWords == kExactWords, "the device's exact-sum words are exact_value's");
 
-// out += sign * (j d) for 5 x 5 row-major blocks, each entry's product summed over k from zero.
-__device__ __forceinline__ void impl_add_product(double* out, const double* j, const double* dm, double sign) {
+// Row r of out += sign * (j d) for 5 x 5 row-major blocks: out_r[c] += sign * sum_k j_r[k] dm[5 k + c],
+// each entry's product summed over k from zero.
+__device__ __forceinline__ void impl_add_product_row(double* out_r, const double* j_r, const double* dm, double sign) {
 #pragma unroll
-  for (int r = 0; r < 5; ++r) {
+  for (int c = 0; c < 5; ++c) {
+    double t = 0.0;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double t = 0.0;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) t += j[5 * r + k] * dm[5 * k + c];
-      out[5 * r + c] += sign * t;
-    }
+    for (int k = 0; k < 5; ++k) t += j_r[k] * dm[5 * k + c];
+    out_r[c] += sign * t;
   }
+}
+
+// Row r of a block stored as impl_block reads it.
+__device__ __forceinline__ void impl_block_row(const double* __restrict__ v, long long n, long long e, int k, int r,
+                                               double* b_r) {
+#pragma unroll
+  for (int c = 0; c < 5; ++c) b_r[c] = v[(25LL * k + 5 * r + c) * n + e];
 }
 
 __device__ __forceinline__ void impl_block(const double* __restrict__ v, long long n, long long e, int k, double* b) {
@@ -52,45 +57,61 @@ __device__ __forceinline__ void impl_block(const double* __restrict__ v, long lo
   for (int q = 0; q < 25; ++q) b[q] = v[(25LL * k + q) * n + e];
 }
 
+// Five lanes per cell or face (12 per wave, lanes 60-63 idle), lane r computing row r of its blocks.
+constexpr int kRowGroups = kWave / 5;
+
+__device__ __forceinline__ int impl_group_item(int& r) {
+  const int lane = threadIdx.x % kWave;
+  const int g = lane / 5;
+  r = lane % 5;
+  if (g >= kRowGroups) return -1;
+  return (blockIdx.x * (kBlock / kWave) + static_cast<int>(threadIdx.x / kWave)) * kRowGroups + g;
+}
+
+inline int impl_group_blocks(long long items) {
+  constexpr int per_block = (kBlock / kWave) * kRowGroups;
+  return static_cast<int>((items + per_block - 1) / per_block);
+}
+
 // The diagonal blocks: lam_c / cfl on the diagonal, then the cell's faces in cell_faces order.
-// dwdu (25 x nc), fj (50 x ni), bj (25 x nb) as k_dwdu and k_face_jacobian write them.
+// dwdu (25 x nc), fj (50 x ni), bj (25 x nb) as k_dwdu and k_face_jacobian write them. Five lanes
+// per cell; stored at the cell's row position.
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __restrict__ dwdu,
                                                       const double* __restrict__ fj, const double* __restrict__ bj,
                                                       double cfl, const int* __restrict__ rank,
                                                       double* __restrict__ diag) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= d.nc) return;
-  double dg[25];
-#pragma unroll
-  for (int q = 0; q < 25; ++q) dg[q] = 0.0;
+  int r;
+  const int c = impl_group_item(r);
+  if (c < 0 || c >= d.nc) return;
+  double dg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double shift = static_cast<double>(d.lam[c]) / cfl;
-#pragma unroll
-  for (int i = 0; i < 5; ++i) dg[6 * i] += shift;
+  dg[r] += shift;
   double dc[25];
   impl_block(dwdu, d.nc, c, 0, dc);
   for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
     const int e = d.inc[k];
     const int f = e & 0x7fffffff;
-    double j[25];
+    double j[5];
     if (f >= d.ni) {
-      impl_block(bj, d.nb, f - d.ni, 0, j);
-      impl_add_product(dg, j, dc, 1.0);
+      impl_block_row(bj, d.nb, f - d.ni, 0, r, j);
+      impl_add_product_row(dg, j, dc, 1.0);
     } else if (e >= 0) {
-      impl_block(fj, d.ni, f, 0, j);
-      impl_add_product(dg, j, dc, 1.0);
+      impl_block_row(fj, d.ni, f, 0, r, j);
+      impl_add_product_row(dg, j, dc, 1.0);
     } else {
-      impl_block(fj, d.ni, f, 1, j);
-      impl_add_product(dg, j, dc, -1.0);
+      impl_block_row(fj, d.ni, f, 1, r, j);
+      impl_add_product_row(dg, j, dc, -1.0);
     }
   }
-  double* out = diag + 25LL * rank[c];
+  double* out = diag + 25LL * rank[c] + 5 * r;
 #pragma unroll
-  for (int q = 0; q < 25; ++q) out[q] = dg[q];
+  for (int q = 0; q < 5; ++q) out[q] = dg[q];
 }
 
 // The off-diagonal blocks of every interior face, at their entries (at[f].x in the owner's row,
-// at[f].y in the neighbour's): J_R D_neighbour (row owner), -J_L D_owner (row neighbour).
+// at[f].y in the neighbour's): J_R D_neighbour (row owner), -J_L D_owner (row neighbour). One
+// thread per face (five lanes per face would load each cell's dW/dq block five times over).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __restrict__ dwdu,
                                                      const double* __restrict__ fj, const int2* __restrict__ at,
@@ -98,13 +119,14 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
   const int f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= d.ni) return;
   const int2 lr = d.lr[f];
+  const int2 k = at[f];
   double j[25], dm[25], out[25];
   impl_block(fj, d.ni, f, 1, j);
   impl_block(dwdu, d.nc, lr.y, 0, dm);
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
-  impl_add_product(out, j, dm, 1.0);
-  const int2 k = at[f];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) impl_add_product_row(out + 5 * r, j + 5 * r, dm, 1.0);
   double* up = offd + 25LL * k.x;
 #pragma unroll
   for (int q = 0; q < 25; ++q) up[q] = out[q];
@@ -112,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
   impl_block(dwdu, d.nc, lr.x, 0, dm);
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
-  impl_add_product(out, j, dm, -1.0);
+#pragma unroll
+  for (int r = 0; r < 5; ++r) impl_add_product_row(out + 5 * r, j + 5 * r, dm, -1.0);
   double* lo = offd + 25LL * k.y;
 #pragma unroll
   for (int q = 0; q < 25; ++q) lo[q] = out[q];

@@ -347,8 +347,8 @@ class Stepper final : public StepperBase {
     dot_next_ = 0;
     launch("impl_clear", [&] { k_impl_clear<<<1, 1024, 0, stream_>>>(words_.get(), kImplDots, bits_.get(), flag_.get()); });
     launch("impl_diag", [&] {
-      k_impl_diag<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl, rank_.get(),
-                                                          diag_.get());
+      k_impl_diag<C><<<impl_group_blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl,
+                                                                     rank_.get(), diag_.get());
     });
     if (ni_ > 0) {
       launch("impl_off", [&] {
