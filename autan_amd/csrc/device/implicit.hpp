@@ -57,6 +57,13 @@ __device__ __forceinline__ void impl_block(const double* __restrict__ v, long lo
   for (int q = 0; q < 25; ++q) b[q] = v[(25LL * k + q) * n + e];
 }
 
+// Cell c's 5 x 5 dW/dq block from the cell-major copy (25 contiguous values per cell).
+__device__ __forceinline__ void impl_cell_block(const double* __restrict__ dwdu, long long c, double* b) {
+  const double* p = dwdu + 25 * c;
+#pragma unroll
+  for (int q = 0; q < 25; ++q) b[q] = p[q];
+}
+
 // Five lanes per cell or face (12 per wave, lanes 60-63 idle), lane r computing row r of its blocks.
 constexpr int kRowGroups = kWave / 5;
 
@@ -74,34 +81,38 @@ inline int impl_group_blocks(long long items) {
 }
 
 // The diagonal blocks: lam_c / cfl on the diagonal, then the cell's faces in cell_faces order.
-// dwdu (25 x nc), fj (50 x ni), bj (25 x nb) as k_dwdu and k_face_jacobian write them. Five lanes
-// per cell; stored at the cell's row position.
+// An interior face's term is the negated off-diagonal block of the transposed position, which
+// k_impl_off has stored: the owner's +J_L D_owner is -A(neighbour, owner) (entry at[f].y), the
+// neighbour's -J_R D_neighbour is -A(owner, neighbour) (entry at[f].x). Each such block is the
+// same sum from zero, negated exactly, and a running sum from +0 is never -0, so adding the
+// negated entry gives the bits of adding the product; it reads each entry's 25 contiguous values
+// instead of recomputing the product from the face blocks. A boundary face's term J_b D_c from
+// bj (25 x nb) and the cell's dW/dq block (cell-major). Five lanes per cell; stored at the cell's
+// row position.
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __restrict__ dwdu,
-                                                      const double* __restrict__ fj, const double* __restrict__ bj,
-                                                      double cfl, const int* __restrict__ rank,
-                                                      double* __restrict__ diag) {
+                                                      const double* __restrict__ bj, const double* __restrict__ offd,
+                                                      const int2* __restrict__ at, double cfl,
+                                                      const int* __restrict__ rank, double* __restrict__ diag) {
   int r;
   const int c = impl_group_item(r);
   if (c < 0 || c >= d.nc) return;
   double dg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   const double shift = static_cast<double>(d.lam[c]) / cfl;
   dg[r] += shift;
-  double dc[25];
-  impl_block(dwdu, d.nc, c, 0, dc);
   for (int k = d.cfo[c]; k < d.cfo[c + 1]; ++k) {
     const int e = d.inc[k];
     const int f = e & 0x7fffffff;
-    double j[5];
     if (f >= d.ni) {
+      double dc[25], j[5];
+      impl_cell_block(dwdu, c, dc);
       impl_block_row(bj, d.nb, f - d.ni, 0, r, j);
       impl_add_product_row(dg, j, dc, 1.0);
-    } else if (e >= 0) {
-      impl_block_row(fj, d.ni, f, 0, r, j);
-      impl_add_product_row(dg, j, dc, 1.0);
     } else {
-      impl_block_row(fj, d.ni, f, 1, r, j);
-      impl_add_product_row(dg, j, dc, -1.0);
+      const int2 a = at[f];
+      const double* o = offd + 25LL * (e >= 0 ? a.y : a.x) + 5 * r;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) dg[q] += -o[q];
     }
   }
   double* out = diag + 25LL * rank[c] + 5 * r;
@@ -111,7 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_impl_diag(Dev<C> d, const double* __
 
 // The off-diagonal blocks of every interior face, at their entries (at[f].x in the owner's row,
 // at[f].y in the neighbour's): J_R D_neighbour (row owner), -J_L D_owner (row neighbour). One
-// thread per face (five lanes per face would load each cell's dW/dq block five times over).
+// thread per face: five lanes per face would each load the cells' whole dW/dq blocks, and a block
+// computed a row at a time stores in pieces too small (measured twice the time).
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __restrict__ dwdu,
                                                      const double* __restrict__ fj, const int2* __restrict__ at,
@@ -122,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
   const int2 k = at[f];
   double j[25], dm[25], out[25];
   impl_block(fj, d.ni, f, 1, j);
-  impl_block(dwdu, d.nc, lr.y, 0, dm);
+  impl_cell_block(dwdu, lr.y, dm);
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
 #pragma unroll
@@ -131,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_off(Dev<C> d, const double* __r
 #pragma unroll
   for (int q = 0; q < 25; ++q) up[q] = out[q];
   impl_block(fj, d.ni, f, 0, j);
-  impl_block(dwdu, d.nc, lr.x, 0, dm);
+  impl_cell_block(dwdu, lr.x, dm);
 #pragma unroll
   for (int q = 0; q < 25; ++q) out[q] = 0.0;
 #pragma unroll
@@ -163,15 +175,22 @@ __global__ __launch_bounds__(kBlock) void k_impl_invert(const double* __restrict
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    int p = k;
+    // The pivot row by its value's magnitude carried along, so that no register is indexed at
+    // run time (the first strictly larger one wins, as when comparing with m[piv][k]).
+    int piv = k;
+    double best = fabs(m[k][k]);
 #pragma unroll
     for (int i = k + 1; i < 5; ++i) {
-      if (fabs(m[i][k]) > fabs(m[p][k])) p = i;
+      const double a = fabs(m[i][k]);
+      if (a > best) {
+        best = a;
+        piv = i;
+      }
     }
-    // Registers cannot be indexed by p: swap by selects, row by row.
+    // Swapped by selects, row by row.
 #pragma unroll
     for (int i = k + 1; i < 5; ++i) {
-      if (i == p) {
+      if (i == piv) {
 #pragma unroll
         for (int j = 0; j < 10; ++j) {
           const double t = m[k][j];
@@ -521,9 +540,10 @@ __global__ __launch_bounds__(kBlock) void k_impl_admissible(const double* __rest
 // sums, boundary values held), then over its boundary faces in cell_faces order, where the
 // reconstruction did not fall back to the cell value, bnd_jac applied to phi (grad_tan . r), and on
 // warped slip and wall faces the wall-warp term's tangent phi_p M^T grad_tan p.
+// dwdu cell-major (25 c + 5 i + j), v cell-major (5 c + j).
 template <class C>
-__device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, const double* __restrict__ v, long long n,
-                                             long long c, C* dw) {
+__device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, const double* __restrict__ v, long long c,
+                                             C* dw) {
   C vc[5];
 #pragma unroll
   for (int j = 0; j < 5; ++j) vc[j] = static_cast<C>(v[5 * c + j]);
@@ -531,7 +551,7 @@ __device__ __forceinline__ void impl_tangent(const double* __restrict__ dwdu, co
   for (int i = 0; i < 5; ++i) {
     C t = C(0);
 #pragma unroll
-    for (int j = 0; j < 5; ++j) t = t + static_cast<C>(dwdu[(5 * i + j) * n + c]) * vc[j];
+    for (int j = 0; j < 5; ++j) t = t + static_cast<C>(dwdu[25 * c + 5 * i + j]) * vc[j];
     dw[i] = t;
   }
 }
@@ -544,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_impl_tangents(const double* __restri
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= n) return;
   C dw[5];
-  impl_tangent<C>(dwdu, v, n, c, dw);
+  impl_tangent<C>(dwdu, v, c, dw);
 #pragma unroll
   for (int i = 0; i < 5; ++i) out[static_cast<long long>(i) * n + c] = dw[i];
 }

@@ -113,26 +113,46 @@ __device__ __forceinline__ const Dual<C, N>& component(const W5<Dual<C, N>>& w, 
 }
 
 // primitives_jacobian (kernels/euler.py:434-440): dW/dq of every cell, q the stored state
-// State(u - U_ref); entry (i, j) at (5 i + j) n_cells + c, widened to FP64.
-template <class C>
+// State(u - U_ref); entry (i, j) at (5 i + j) n_cells + c, widened to FP64, or with CELL_MAJOR at
+// 25 c + 5 i + j (the implicit solve's layout: a cell's block one contiguous run), the block's
+// 256 cells staged in LDS so that the stores stay contiguous across lanes.
+template <class C, bool CELL_MAJOR = false>
 __global__ __launch_bounds__(kBlock) void k_dwdu(Dev<C> d, const double* __restrict__ u, double* __restrict__ out) {
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= d.nc) return;
-  using D = Dual<C, 5>;
-  const Params<C>& pa = d.ptab[cell_case(d, c)];
-  D dq[5], ur[5];
+  const int c0 = blockIdx.x * kBlock;
+  const int c = c0 + threadIdx.x;
+  double v[25];
+  if (c < d.nc) {
+    using D = Dual<C, 5>;
+    const Params<C>& pa = d.ptab[cell_case(d, c)];
+    D dq[5], ur[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    dq[k] = D(static_cast<C>(u[static_cast<long long>(k) * d.nc + c] - pa.ref[k]));
-    dq[k].d[k] = C(1);
-    ur[k] = D(pa.ur[k]);
+    for (int k = 0; k < 5; ++k) {
+      dq[k] = D(static_cast<C>(u[static_cast<long long>(k) * d.nc + c] - pa.ref[k]));
+      dq[k].d[k] = C(1);
+      ur[k] = D(pa.ur[k]);
+    }
+    const W5<D> w = to_primitive(dq, ur, D(pa.g));
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const D& wi = component(w, i);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) v[5 * i + j] = static_cast<double>(wi.d[j]);
+    }
   }
-  const W5<D> w = to_primitive(dq, ur, D(pa.g));
+  if constexpr (CELL_MAJOR) {
+    __shared__ double tile[25 * kBlock];
+    if (c < d.nc) {
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
-    const D& wi = component(w, i);
+      for (int q = 0; q < 25; ++q) tile[25 * threadIdx.x + q] = v[q];
+    }
+    __syncthreads();
+    const int count = 25 * (d.nc - c0 < kBlock ? d.nc - c0 : kBlock);
+    double* o = out + 25LL * c0;
+    for (int t = threadIdx.x; t < count; t += kBlock) o[t] = tile[t];
+  } else {
+    if (c >= d.nc) return;
 #pragma unroll
-    for (int j = 0; j < 5; ++j) out[static_cast<long long>(5 * i + j) * d.nc + c] = static_cast<double>(wi.d[j]);
+    for (int q = 0; q < 25; ++q) out[static_cast<long long>(q) * d.nc + c] = v[q];
   }
 }
 

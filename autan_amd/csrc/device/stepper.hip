@@ -312,7 +312,7 @@ class Stepper final : public StepperBase {
     implicit_setup();
     evaluate(norms, invalid);
     const Dev<C> d = dev();
-    launch("dwdu", [&] { k_dwdu<C><<<blocks(nc_), kBlock, 0, stream_>>>(d, u_.get(), jd_.get()); });
+    launch("dwdu", [&] { k_dwdu<C, true><<<blocks(nc_), kBlock, 0, stream_>>>(d, u_.get(), jd_.get()); });
     launch("face_jacobian", [&] {
       switch (sc_.flux) {
         case MXEULER_ROE:
@@ -346,15 +346,15 @@ class Stepper final : public StepperBase {
     const int gm = static_cast<int>((m + kBlock - 1) / kBlock);
     dot_next_ = 0;
     launch("impl_clear", [&] { k_impl_clear<<<1, 1024, 0, stream_>>>(words_.get(), kImplDots, bits_.get(), flag_.get()); });
-    launch("impl_diag", [&] {
-      k_impl_diag<C><<<impl_group_blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), jb_.get(), cfl,
-                                                                     rank_.get(), diag_.get());
-    });
     if (ni_ > 0) {
       launch("impl_off", [&] {
         k_impl_off<C><<<blocks(ni_), kBlock, 0, stream_>>>(d, jd_.get(), jf_.get(), rat_.get(), offd_.get());
       });
     }
+    launch("impl_diag", [&] {
+      k_impl_diag<C><<<impl_group_blocks(nc_), kBlock, 0, stream_>>>(d, jd_.get(), jb_.get(), offd_.get(), rat_.get(),
+                                                                     cfl, rank_.get(), diag_.get());
+    });
     launch("impl_invert", [&] {
       k_impl_invert<<<blocks(nc_), kBlock, 0, stream_>>>(diag_.get(), ccells_.get(), nc_, inv_.get(), flag_.get());
     });
