@@ -22,6 +22,7 @@
 namespace mxm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 using bf16 = __bf16;
 
@@ -281,11 +282,13 @@ struct LayerTable {
 // The whole forward of 16 RT samples per workgroup (inference): the normalised inputs, then every
 // layer's activations, stay in one 16 RT x (hidden + 8) BF16 LDS tile (dynamic shared memory); the
 // weights stream from L2, each weight column read once per workgroup and used for all 16 RT rows,
-// so the L2 bytes per product fall as RT grows. Each of the four waves computes 16 CT output
+// so the L2 bytes per product fall as RT grows. Each of the WAVES waves computes 16 CT output
 // columns of the tile's rows as RT x CT matrix-core tiles, K in steps of 32 in the order of
 // k_gemm, and the epilogues are k_gemm's (bias + SiLU, then bias and the output scaling of
 // k_out): the outputs are the layer-by-layer path's bits whatever RT, without its HBM round trip
-// of every layer's activations.
+// of every layer's activations. The hidden layers' tiles are computed transposed (W H^T), so that
+// a lane's four results are four consecutive outputs of one sample: one 8-byte LDS store of two
+// packed BF16 conversions each, instead of four 2-byte stores.
 template <int CT, int RT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_infer(const float* __restrict__ x, int n, int d_in, int d_out,
                                                       const float* __restrict__ norm, const bf16* __restrict__ wb,
@@ -334,22 +337,27 @@ __global__ __launch_bounds__(64 * WAVES) void k_infer(const float* __restrict__ 
 #pragma unroll
         for (int i = 0; i < RT; ++i) {
           const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[(16 * i + row) * ld + k0 + kq]);
+          // The weights as the first operand: the tile is W H^T, so each lane ends with four
+          // consecutive outputs of one sample. Every output is the same sum of the same exact
+          // BF16 products as in k_gemm's H W^T.
 #pragma unroll
-          for (int j = 0; j < CT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < CT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a, acc[i][j], 0, 0, 0);
         }
       }
       __syncthreads();  // every wave has read the layer's input: the tile takes its output
 #pragma unroll
-      for (int i = 0; i < RT; ++i) {
+      for (int j = 0; j < CT; ++j) {
+        // Lane l holds outputs f .. f + 3 of sample 16 i + l mod 16 of each row tile i.
+        const int f = nb + 16 * j + 4 * (lane / 16);
+        float bn[4];
 #pragma unroll
-        for (int j = 0; j < CT; ++j) {
-          const int nn = nb + 16 * j + lane % 16;
-          const float bn = bias[nn];
+        for (int r = 0; r < 4; ++r) bn[r] = bias[f + r];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int mr = 16 * i + 4 * (lane / 16) + r;
-            act[mr * ld + nn] = static_cast<bf16>(silu(acc[i][j][r] + bn));
-          }
+        for (int i = 0; i < RT; ++i) {
+          f32x4 h;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) h[r] = silu(acc[i][j][r] + bn[r]);
+          *reinterpret_cast<bf16x4*>(&act[(16 * i + lane % 16) * ld + f]) = __builtin_convertvector(h, bf16x4);
         }
       }
     } else {
