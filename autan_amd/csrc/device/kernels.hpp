@@ -524,6 +524,20 @@ __device__ __forceinline__ int bits_index(C v) {
   }
 }
 
+// A slot record's left index with its sign bit set: the face's nine moments are all +0 (bit for
+// bit), so the warp terms take zeros in registers instead of reading them; the same values, so
+// the same bits. Left indices are cells, never negative otherwise.
+constexpr unsigned kFlatFace = 0x80000000u;
+
+template <class C>
+__device__ __forceinline__ bool plus_zero(C v) {
+  if constexpr (std::is_same_v<C, float>) {
+    return __float_as_uint(v) == 0u;
+  } else {
+    return __double_as_longlong(v) == 0LL;
+  }
+}
+
 // Where face_core reads the cells' primitives and gradient records: global memory, or LDS for the
 // cells [cb, cb + ncb) of a face block staged there (others from global memory).
 template <class C>
@@ -557,7 +571,7 @@ struct StagedCells {
 
 // {flux}_face[_planar] and {flux}_boundary[_planar] (kernels/euler.py:687-1049) for the face `g`
 // (12 values, above): the area-weighted flux in C. The warp terms read the face's nine moments at
-// m[q * ms]. With LAMBDA it also leaves the face's convective spectral radius
+// m[q * ms], or take zeros when the left index carries kFlatFace. With LAMBDA it also leaves the face's convective spectral radius
 // (kernels/euler.py:1091-1112) as each side takes it: lam[0] = max(s_owner, s_other) A for the
 // owner, lam[1] = max(s_other, s_owner) A for the neighbour, s = |v . n| + c of the cells' (not
 // reconstructed) primitives and of the boundary state.
@@ -566,7 +580,9 @@ __device__ __forceinline__ F5<C> face_core(const Dev<C>& d, const Params<C>& pa,
                                            const C* m9, long long ms, C* lam) {
   const V3<C> n{g[0], g[1], g[2]};
   const C area = g[3];
-  const int left = bits_index(g[7]), right = bits_index(g[11]);
+  const int lbits = bits_index(g[7]), right = bits_index(g[11]);
+  const bool flat = lbits < 0;  // kFlatFace: the moments are all +0
+  const int left = lbits & 0x7fffffff;
   F5<C> out;
   if (right >= 0) {
     W5<C> wl = src.w(left), wr = src.w(right);
@@ -588,8 +604,13 @@ __device__ __forceinline__ F5<C> face_core(const Dev<C>& d, const Params<C>& pa,
     for (int k = 0; k < 5; ++k) out.f[k] = area * fx.f[k];
     if constexpr (!PLANAR) {
       C m[9];
+      if (flat) {
 #pragma unroll
-      for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
+        for (int q = 0; q < 9; ++q) m[q] = C(0);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
+      }
       if constexpr (!SECOND) {
 #pragma unroll
         for (int q = 0; q < 20; ++q) gl[q] = gr[q] = C(0);
@@ -630,8 +651,13 @@ __device__ __forceinline__ F5<C> face_core(const Dev<C>& d, const Params<C>& pa,
       for (int k = 0; k < 5; ++k) out.f[k] = area * wfx[k];
       if constexpr (!PLANAR) {
         C m[9];
+        if (flat) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
+          for (int q = 0; q < 9; ++q) m[q] = C(0);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) m[q] = m9[q * ms];
+        }
         const C p4 = gl[15 + 4];
         const C g4[3] = {gl[12], gl[13], gl[14]};
         const C ww[5] = {C(0.0), p4 * (m[0] * g4[0] + m[3] * g4[1] + m[6] * g4[2]),
@@ -676,35 +702,44 @@ __device__ __forceinline__ F5<C> face_flux(const Dev<C>& d, int f, C* lam = null
 }
 
 // The fused face kernel's slot records (face_core's 12 values) of the faces sface[0 .. ns), built
-// from the per-face arrays as face_flux assembles them, and their moments (non-planar meshes).
+// from the per-face arrays as face_flux assembles them, and their moments (non-planar meshes); a
+// face whose nine moments are all +0 has kFlatFace in its left index, and face_core then reads
+// none of them.
 template <class C>
 __global__ __launch_bounds__(kBlock) void k_slot_records(Dev<C> d, const int* __restrict__ sface, long long ns,
                                                          C* __restrict__ srec, C* __restrict__ smom) {
   const long long sl = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
   if (sl >= ns) return;
   const int f = sface[sl];
+  bool flat = false;
+  if (smom != nullptr) {
+    flat = true;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const C m = d.moment[static_cast<long long>(q) * d.nf + f];
+      smom[9 * sl + q] = m;
+      flat = flat && plus_zero(m);
+    }
+  }
+  const unsigned fl = flat ? kFlatFace : 0u;
   C g[12];
   load_n<C, 8>(d.fgeo + 8LL * f, g);
   if (f < d.ni) {
     const int2 lr = d.lr[f];
     C rr[4];
     load_n<C, 4>(d.frr + 4LL * f, rr);
-    g[7] = index_bits<C>(lr.x);
+    g[7] = index_bits<C>(static_cast<int>(static_cast<unsigned>(lr.x) | fl));
     g[8] = rr[0];
     g[9] = rr[1];
     g[10] = rr[2];
     g[11] = index_bits<C>(lr.y);
   } else {
     const int b = f - d.ni;
-    g[7] = index_bits<C>(d.bowner[b]);
+    g[7] = index_bits<C>(static_cast<int>(static_cast<unsigned>(d.bowner[b]) | fl));
     g[8] = g[9] = g[10] = C(0);
     g[11] = index_bits<C>(-b - 1);
   }
   store_n<C, 12>(srec + 12 * sl, g);
-  if (smom != nullptr) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) smom[9 * sl + q] = d.moment[static_cast<long long>(q) * d.nf + f];
-  }
 }
 
 // The face loops' fluxes stored per face for the per-cell gather (the store half of the

@@ -3,7 +3,8 @@
 algorithmic size (padding excluded), extended to the fused kernels of this library.
 
 Counts: N cells, Fi interior faces, Fb boundary faces, S gradient-stencil entries, I incidences
-(cell-face pairs), Nbo distinct owners of boundary faces. Sizes: s = sizeof(state) = sizeof(grad) =
+(cell-face pairs), Nbo distinct owners of boundary faces, Fm faces whose moments are not all +0
+(the fused face kernels read no moment of the others). Sizes: s = sizeof(state) = sizeof(grad) =
 sizeof(geom value) = 4 in fp32+acc64 and 8 in fp64; r = 8 (residual, conserved, FP64 volume);
 i = 4 (indices).
 """
@@ -27,6 +28,7 @@ class Counts:
     limiter: bool = True
     second_order: bool = True
     planar: bool = True
+    Fm: int = -1  # faces with a moment not +0 (-1: every face)
 
     @property
     def F(self) -> int:
@@ -36,6 +38,8 @@ class Counts:
     def of(cls, mesh, source=True, limiter=True, second_order=True, planar=None) -> "Counts":
         nb = mesh.n_boundary
         nbo = int(np.unique(mesh.owner[mesh.n_interior :]).size) if nb else 0
+        mom = np.asarray(mesh.face_moment, dtype=np.float64).reshape(9, mesh.n_faces)
+        fm = int(np.count_nonzero(np.any(mom.view(np.uint64) != 0, axis=0)))
         return cls(
             N=mesh.n_cells,
             Fi=mesh.n_interior,
@@ -47,6 +51,7 @@ class Counts:
             limiter=limiter,
             second_order=second_order,
             planar=mesh.all_planar if planar is None else planar,
+            Fm=fm,
         )
 
 
@@ -95,7 +100,8 @@ def kernel_bytes(c: Counts, policy: str = "fp32+acc64", local: bool = False) -> 
     # The fused face kernel: the flux kernel's inputs and the gather's, the fluxes staying on chip
     # (LDS); the first residual of a step also writes lambda (and the local dt) from the FP64 volume
     # and adds the norms of the residual it writes.
-    out["faces"] = out["flux"] - F * 5 * s + out["gather"] - F * 5 * s
+    fm = F if c.Fm < 0 else c.Fm
+    out["faces"] = out["flux"] - F * 5 * s + out["gather"] - F * 5 * s - (F - fm) * moment
     out["faces_first"] = out["faces"] + N * (r + s + (r if local else 0))
     # The face kernel that also updates the stage (no residual written): u and the stage read, the
     # new stage written, the FP64 volume and local dt read; then the new stage's primitives (on the
