@@ -163,30 +163,24 @@ __device__ __forceinline__ bool case_second(const Dev<C>& d, int cs) {
   return d.cflags == nullptr || (d.cflags[cs] & 2) != 0;
 }
 
+// A cell's (or boundary face's) primitive record: rho, u, v, w, p - p_ref, kPrimStride values per
+// record and no padding, so that the gathers of neighbouring cells share cache lines.
+constexpr int kPrimStride = 5;
+
 template <class C>
 __device__ __forceinline__ W5<C> load_w(const C* __restrict__ p, long long i) {
-  const C* q = p + 8 * i;
-  if constexpr (std::is_same_v<C, float>) {
-    const float4 a = *reinterpret_cast<const float4*>(q);
-    return {a.x, a.y, a.z, a.w, q[4]};
-  } else {
-    const double2 a = *reinterpret_cast<const double2*>(q);
-    const double2 b = *reinterpret_cast<const double2*>(q + 2);
-    return {a.x, a.y, b.x, b.y, q[4]};
-  }
+  const C* q = p + kPrimStride * i;
+  return {q[0], q[1], q[2], q[3], q[4]};
 }
 
 template <class C>
 __device__ __forceinline__ void store_w(C* __restrict__ p, long long i, const W5<C>& w) {
-  C* q = p + 8 * i;
-  if constexpr (std::is_same_v<C, float>) {
-    *reinterpret_cast<float4*>(q) = make_float4(w.r, w.u, w.v, w.w);
-    *reinterpret_cast<float4*>(q + 4) = make_float4(w.p, 0.f, 0.f, 0.f);
-  } else {
-    *reinterpret_cast<double2*>(q) = make_double2(w.r, w.u);
-    *reinterpret_cast<double2*>(q + 2) = make_double2(w.v, w.w);
-    *reinterpret_cast<double2*>(q + 4) = make_double2(w.p, 0.0);
-  }
+  C* q = p + kPrimStride * i;
+  q[0] = w.r;
+  q[1] = w.u;
+  q[2] = w.v;
+  q[3] = w.w;
+  q[4] = w.p;
 }
 
 // N consecutive values at a 16 B aligned address.
@@ -1177,19 +1171,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
     for (int t = threadIdx.x; t < kSums * kWords; t += kBlock) acc[t] = 0;
   }
   const int cb = d.cbeg[blk], ncb = d.cbeg[blk + 1] - cb;
-  // STAGE: the block's own cells' primitives (8 per cell) and gradient records (20) copied to LDS
-  // first, as contiguous runs; faces read them there and only their other blocks' cells from
-  // global memory.
+  // STAGE: the block's own cells' primitives (kPrimStride per cell) and gradient records (20)
+  // copied to LDS first, as contiguous runs; faces read them there and only their other blocks'
+  // cells from global memory.
   C* sp = reinterpret_cast<C*>(smem + lf_bytes + (FIRST ? kSums * kWords * 8 : 0));
-  C* sr = sp + 8 * kBlock;
+  C* sr = sp + kPrimStride * kBlock;  // 16 B aligned: kPrimStride * kBlock values
   if constexpr (STAGE) {
     constexpr int kVec = 16 / sizeof(C);
     using V = std::conditional_t<std::is_same_v<C, float>, float4, double2>;
-    const V* gp = reinterpret_cast<const V*>(d.prim + 8LL * cb);
+    const C* gp = d.prim + static_cast<long long>(kPrimStride) * cb;
     const V* gr = reinterpret_cast<const V*>(d.rec + 20LL * cb);
-    V* lp = reinterpret_cast<V*>(sp);
     V* lr = reinterpret_cast<V*>(sr);
-    for (int i = threadIdx.x; i < ncb * (8 / kVec); i += kBlock) lp[i] = gp[i];
+    for (int i = threadIdx.x; i < ncb * kPrimStride; i += kBlock) sp[i] = gp[i];
     for (int i = threadIdx.x; i < ncb * (20 / kVec); i += kBlock) lr[i] = gr[i];
     __syncthreads();
   }
@@ -1417,8 +1410,8 @@ __global__ __launch_bounds__(kBlock) void k_face_force(Dev<C> d, const int* __re
     drho += static_cast<double>(g[j]) * r;
     dp += static_cast<double>(g[12 + j]) * r;
   }
-  const double rho_c = static_cast<double>(d.prim[8 * c]);
-  const double p_c = pa.p_ref + static_cast<double>(d.prim[8 * c + 4]);
+  const double rho_c = static_cast<double>(d.prim[kPrimStride * c]);
+  const double p_c = pa.p_ref + static_cast<double>(d.prim[kPrimStride * c + 4]);
   const double rho_f = rho_c + static_cast<double>(g[15]) * drho;
   double p_f = p_c + static_cast<double>(g[19]) * dp;
   if (!(rho_f > 0.0 && p_f > 0.0)) p_f = p_c;

@@ -222,9 +222,9 @@ class Stepper final : public StepperBase {
     u_ = Buffer<double>(5 * std::size_t(nc_));
     stage_ = Buffer<double>(5 * std::size_t(nc_));
     res_ = Buffer<double>(5 * std::size_t(nc_));
-    prim_ = Buffer<C>(8 * std::size_t(nc_));
-    prim2_ = Buffer<C>(8 * std::size_t(nc_));
-    prim_b_ = Buffer<C>(8 * std::size_t(std::max(nb_, 1)));
+    prim_ = Buffer<C>(kPrimStride * std::size_t(nc_));
+    prim2_ = Buffer<C>(kPrimStride * std::size_t(nc_));
+    prim_b_ = Buffer<C>(kPrimStride * std::size_t(std::max(nb_, 1)));
     rec_ = Buffer<C>(20 * std::size_t(nc_));
     fflux_ = Buffer<C>(8 * std::size_t(nf_));
     lam_ = Buffer<C>(std::size_t(nc_));
@@ -841,13 +841,13 @@ class Stepper final : public StepperBase {
     const std::size_t n = std::size_t(nc_);
     for (std::size_t c = 0; c < n; ++c) {
       for (std::size_t k = 0; k < 5; ++k) {
-        if (prim != nullptr) prim[k * n + c] = p[8 * c + k];
+        if (prim != nullptr) prim[k * n + c] = p[kPrimStride * c + k];
         if (phi != nullptr) phi[k * n + c] = r[20 * c + 15 + k];
       }
       for (std::size_t q = 0; q < 15 && grad != nullptr; ++q) grad[q * n + c] = r[20 * c + q];
     }
     for (std::size_t b = 0; b < std::size_t(nb_) && prim_b != nullptr; ++b) {
-      for (std::size_t k = 0; k < 5; ++k) prim_b[k * std::size_t(nb_) + b] = pb[8 * b + k];
+      for (std::size_t k = 0; k < 5; ++k) prim_b[k * std::size_t(nb_) + b] = pb[kPrimStride * b + k];
     }
   }
 
@@ -1504,7 +1504,7 @@ class Stepper final : public StepperBase {
     const std::size_t comp = first ? 7 : 5;
     const std::size_t fl = (comp * std::size_t(max_slots_) * sizeof(C) + 15) & ~std::size_t(15);
     return fl + (first ? std::size_t(kSums) * kWords * sizeof(long long) : 0) +
-           (stage ? std::size_t(28) * kBlock * sizeof(C) : 0);
+           (stage ? std::size_t(kPrimStride + 20) * kBlock * sizeof(C) : 0);
   }
 
   void build_stencil(const mxeuler_topology& t, const mxeuler_stencil& st) {
