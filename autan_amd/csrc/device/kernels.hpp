@@ -28,6 +28,9 @@ constexpr int kWave = 64;
 constexpr int kLimbs = 68;        // exact-sum limbs of 32 bits, limb i weighs 2^(32 i - 1088)
 constexpr int kWords = kLimbs + 3;  // then the counts of +inf, -inf and NaN terms
 constexpr int kSums = 5;
+// LDS bytes of a block's exact-sum words, rounded up to 16 so that what follows them stays 16 B
+// aligned.
+constexpr int kSumsLds = (kSums * kWords * 8 + 15) & ~15;
 
 // Results a step leaves for the host: one copy back per step.
 struct Small {
@@ -1174,7 +1177,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(OCC > 0 
   // STAGE: the block's own cells' primitives (kPrimStride per cell) and gradient records (20)
   // copied to LDS first, as contiguous runs; faces read them there and only their other blocks'
   // cells from global memory.
-  C* sp = reinterpret_cast<C*>(smem + lf_bytes + (FIRST ? kSums * kWords * 8 : 0));
+  C* sp = reinterpret_cast<C*>(smem + lf_bytes + (FIRST ? kSumsLds : 0));
   C* sr = sp + kPrimStride * kBlock;  // 16 B aligned: kPrimStride * kBlock values
   if constexpr (STAGE) {
     constexpr int kVec = 16 / sizeof(C);

@@ -1503,7 +1503,7 @@ class Stepper final : public StepperBase {
   std::size_t faces_lds(bool first, bool stage = false) const {
     const std::size_t comp = first ? 7 : 5;
     const std::size_t fl = (comp * std::size_t(max_slots_) * sizeof(C) + 15) & ~std::size_t(15);
-    return fl + (first ? std::size_t(kSums) * kWords * sizeof(long long) : 0) +
+    return fl + (first ? std::size_t(kSumsLds) : 0) +
            (stage ? std::size_t(kPrimStride + 20) * kBlock * sizeof(C) : 0);
   }
 
